@@ -1,0 +1,788 @@
+// lucas_capi.cpp -- extern "C" boundary of libgpulucas (declared in
+// include/gpulucas.h): contexts, the reference's mersenneTest / errorTest
+// entry points, checkpoints, batches and the serving loop.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "gpulucas.h"
+#include "lucas_engine.h"
+#include "lucas_host.h"
+
+using namespace lucas;
+
+struct lucas_ctx {
+    Engine* engine = nullptr;
+    Job* job = nullptr;
+};
+
+struct lucas_batch {
+    Engine* engine = nullptr;
+    std::vector<Job*> jobs;
+};
+
+namespace {
+
+double now_s() {
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+void fill_info(const Plan& pl, lucas_plan_info* out) {
+    memset(out, 0, sizeof(*out));
+    out->p = pl.p;
+    out->N = pl.N;
+    out->q = pl.q;
+    out->n_wide = pl.b;
+    out->carry_depth = pl.ncarry;
+    out->A = pl.A;
+    out->B = pl.B;
+    out->C = pl.C;
+    out->halo = pl.halo;
+    out->col_wgs = pl.col_wgs;
+    out->row_wgs = pl.row_wgs;
+    out->device_bytes = job_bytes(pl);
+}
+
+int plan_for(int p, int N, int A, int C, Plan* pl) {
+    if (p < 7) return LUCAS_ERR_INVALID_ARG;
+    if (N == 0) N = default_length(p);
+    if (N <= 0) return LUCAS_ERR_INVALID_ARG;
+    return make_plan(p, N, A, C, pl).empty() ? LUCAS_OK : LUCAS_ERR_INVALID_ARG;
+}
+
+int ctx_new(const Plan& pl, int device, lucas_ctx** out) {
+    lucas_ctx* c = new lucas_ctx();
+    c->engine = new Engine(device);
+    int rc = c->engine->init();
+    if (rc == LUCAS_OK) rc = job_create(pl, device, c->engine->stream(), &c->job);
+    if (rc != LUCAS_OK) {
+        delete c->engine;
+        delete c;
+        return rc;
+    }
+    c->engine->set_jobs({c->job});
+    *out = c;
+    return LUCAS_OK;
+}
+
+double error_limit_of(const lucas_options* opt) {
+    return (opt && opt->error_limit > 0) ? opt->error_limit : LUCAS_DEFAULT_ERROR_LIMIT;
+}
+
+// ---- checkpoint format -----------------------------------------------------
+// 48-byte header + int32 digits[N]:
+//   char magic[8] = "GLUAMD01"; int32 p, N; int64 iter; uint64 fnv1a(digits);
+//   uint64 reserved[2]
+struct CkptHeader {
+    char magic[8];
+    int32_t p, N;
+    int64_t iter;
+    uint64_t checksum;
+    uint64_t reserved[2];
+};
+static_assert(sizeof(CkptHeader) == 48, "checkpoint header layout");
+
+uint64_t fnv1a(const void* data, size_t n) {
+    const unsigned char* b = (const unsigned char*)data;
+    uint64_t h = 1469598103934665603ULL;
+    for (size_t i = 0; i < n; ++i) {
+        h ^= b[i];
+        h *= 1099511628211ULL;
+    }
+    return h;
+}
+
+int write_checkpoint(const char* path, int p, int N, long long iter, const int32_t* digits) {
+    std::string tmp = std::string(path) + ".tmp", bak = std::string(path) + ".bak";
+    FILE* f = fopen(tmp.c_str(), "wb");
+    if (!f) return LUCAS_ERR_IO;
+    CkptHeader h;
+    memset(&h, 0, sizeof(h));
+    memcpy(h.magic, "GLUAMD01", 8);
+    h.p = p;
+    h.N = N;
+    h.iter = iter;
+    h.checksum = fnv1a(digits, sizeof(int32_t) * (size_t)N);
+    bool ok = fwrite(&h, sizeof(h), 1, f) == 1 && fwrite(digits, sizeof(int32_t), (size_t)N, f) == (size_t)N;
+    ok = (fflush(f) == 0) && ok;
+    if (ok) fsync(fileno(f));
+    ok = (fclose(f) == 0) && ok;
+    if (!ok) {
+        unlink(tmp.c_str());
+        return LUCAS_ERR_IO;
+    }
+    // rotate: path -> path.bak, tmp -> path (writeCheckpoint, gpuLucas.cu:1641-1665)
+    unlink(bak.c_str());
+    rename(path, bak.c_str());
+    if (rename(tmp.c_str(), path) != 0) return LUCAS_ERR_IO;
+    return LUCAS_OK;
+}
+
+int read_checkpoint_file(const char* path, int* p, int* N, long long* iter, std::vector<int32_t>* digits) {
+    FILE* f = fopen(path, "rb");
+    if (!f) return LUCAS_ERR_IO;
+    CkptHeader h;
+    int rc = LUCAS_ERR_IO;
+    if (fread(&h, sizeof(h), 1, f) == 1 && memcmp(h.magic, "GLUAMD01", 8) == 0 && h.N > 0 && h.N <= (1 << 28) &&
+        h.p > 0 && h.iter >= 0) {
+        digits->resize((size_t)h.N);
+        if (fread(digits->data(), sizeof(int32_t), (size_t)h.N, f) == (size_t)h.N &&
+            fnv1a(digits->data(), sizeof(int32_t) * (size_t)h.N) == h.checksum) {
+            *p = h.p;
+            *N = h.N;
+            *iter = h.iter;
+            rc = LUCAS_OK;
+        }
+    }
+    fclose(f);
+    return rc;
+}
+
+// primary, then ".bak" (readCheckpoint, gpuLucas.cu:1569-1591)
+int read_checkpoint(const char* path, int* p, int* N, long long* iter, std::vector<int32_t>* digits) {
+    if (read_checkpoint_file(path, p, N, iter, digits) == LUCAS_OK) return LUCAS_OK;
+    std::string bak = std::string(path) + ".bak";
+    return read_checkpoint_file(bak.c_str(), p, N, iter, digits);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lucas_version(void) { return "gpulucas-amd 0.1.0 (gfx950)"; }
+
+const char* lucas_status_string(int status) {
+    switch (status) {
+        case LUCAS_OK: return "ok";
+        case LUCAS_ERR_INVALID_ARG: return "invalid argument";
+        case LUCAS_ERR_OOM: return "out of device memory";
+        case LUCAS_ERR_ROUNDOFF: return "roundoff limit exceeded";
+        case LUCAS_ERR_DEVICE: return "HIP device error";
+        case LUCAS_ERR_IO: return "checkpoint I/O error";
+        case LUCAS_ERR_NO_DEVICE: return "no usable GPU";
+        case LUCAS_ERR_BUSY: return "busy";
+        default: return "unknown status";
+    }
+}
+
+int lucas_device_count(int* count) {
+    if (!count) return LUCAS_ERR_INVALID_ARG;
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) {
+        (void)hipGetLastError();
+        *count = 0;
+        return LUCAS_ERR_NO_DEVICE;
+    }
+    *count = c;
+    return LUCAS_OK;
+}
+
+int lucas_default_N(int p) { return default_length(p); }
+
+int lucas_plan(int p, int N, lucas_plan_info* out) { return lucas_plan_shape(p, N, 0, 0, out); }
+
+int lucas_plan_shape(int p, int N, int A, int C, lucas_plan_info* out) {
+    if (!out) return LUCAS_ERR_INVALID_ARG;
+    Plan pl;
+    int rc = plan_for(p, N, A, C, &pl);
+    if (rc != LUCAS_OK) return rc;
+    fill_info(pl, out);
+    return LUCAS_OK;
+}
+
+int lucas_digit_tables(int p, int N, int* widths, double* weights, double* inv_weights) {
+    if (p < 7 || !valid_length(N) || N > p) return LUCAS_ERR_INVALID_ARG;
+    if (widths) digit_widths(p, N, widths);
+    for (long long j = 0; j < N; ++j) {
+        const long long r = r_of(j, p, N);
+        const long double a = exp2l((long double)r / (long double)N);
+        if (weights) weights[j] = (double)a;
+        if (inv_weights) inv_weights[j] = (double)(1.0L / (a * (long double)N));
+    }
+    return LUCAS_OK;
+}
+
+int lucas_digits_res64(int p, int N, const int32_t* digits, uint64_t* res64, int* is_zero, int32_t* normalized) {
+    if (p < 7 || !valid_length(N) || N > p || !digits) return LUCAS_ERR_INVALID_ARG;
+    normalize_digits(p, N, digits, normalized, res64, is_zero);
+    return LUCAS_OK;
+}
+
+int lucas_ctx_create(int p, int N, int device, lucas_ctx** out) {
+    return lucas_ctx_create_shape(p, N, 0, 0, device, out);
+}
+
+int lucas_ctx_create_shape(int p, int N, int A, int C, int device, lucas_ctx** out) {
+    if (!out) return LUCAS_ERR_INVALID_ARG;
+    Plan pl;
+    int rc = plan_for(p, N, A, C, &pl);
+    if (rc != LUCAS_OK) return rc;
+    return ctx_new(pl, device, out);
+}
+
+int lucas_ctx_destroy(lucas_ctx* ctx) {
+    if (!ctx) return LUCAS_ERR_INVALID_ARG;
+    if (ctx->engine) ctx->engine->sync();
+    job_destroy(ctx->job);
+    delete ctx->engine;
+    delete ctx;
+    return LUCAS_OK;
+}
+
+int lucas_ctx_info(const lucas_ctx* ctx, lucas_plan_info* out) {
+    if (!ctx || !out) return LUCAS_ERR_INVALID_ARG;
+    fill_info(ctx->job->plan, out);
+    return LUCAS_OK;
+}
+
+int lucas_ctx_step(lucas_ctx* ctx, long long iters) {
+    if (!ctx || iters < 0) return LUCAS_ERR_INVALID_ARG;
+    int rc = ctx->engine->step(iters);
+    if (rc == LUCAS_OK) rc = ctx->engine->sync();
+    return rc;
+}
+
+long long lucas_ctx_iteration(const lucas_ctx* ctx) { return ctx ? ctx->job->iter : -1; }
+
+int lucas_ctx_get_digits(lucas_ctx* ctx, int32_t* digits) {
+    if (!ctx || !digits) return LUCAS_ERR_INVALID_ARG;
+    return job_get_digits(ctx->job, ctx->engine->stream(), digits);
+}
+
+int lucas_ctx_set_digits(lucas_ctx* ctx, const int32_t* digits, long long iter) {
+    if (!ctx || !digits || iter < 0) return LUCAS_ERR_INVALID_ARG;
+    std::vector<int> w(ctx->job->plan.N);
+    digit_widths(ctx->job->plan.p, ctx->job->plan.N, w.data());
+    for (int j = 0; j < ctx->job->plan.N; ++j) {
+        const long long lim = 1LL << (w[j] + 1);
+        if (digits[j] <= -lim || digits[j] >= lim) return LUCAS_ERR_INVALID_ARG;
+    }
+    int rc = job_set_digits(ctx->job, ctx->engine->stream(), digits);
+    if (rc == LUCAS_OK) ctx->job->iter = iter;
+    return rc;
+}
+
+int lucas_ctx_res64(lucas_ctx* ctx, uint64_t* res64, int* is_zero) {
+    if (!ctx) return LUCAS_ERR_INVALID_ARG;
+    std::vector<int32_t> d(ctx->job->plan.N);
+    int rc = job_get_digits(ctx->job, ctx->engine->stream(), d.data());
+    if (rc != LUCAS_OK) return rc;
+    normalize_digits(ctx->job->plan.p, ctx->job->plan.N, d.data(), nullptr, res64, is_zero);
+    return LUCAS_OK;
+}
+
+int lucas_ctx_max_roundoff(lucas_ctx* ctx, int reset, double* out) {
+    if (!ctx || !out) return LUCAS_ERR_INVALID_ARG;
+    return job_read_maxerr(ctx->job, ctx->engine->stream(), reset, out);
+}
+
+int lucas_ctx_time_kernels(lucas_ctx* ctx, int iters, double* col_us, double* row_us) {
+    if (!ctx || !col_us || !row_us) return LUCAS_ERR_INVALID_ARG;
+    return ctx->engine->time_kernels(iters, col_us, row_us);
+}
+
+int lucas_ctx_time_iterations(lucas_ctx* ctx, long long iters, double* ms_per_iter) {
+    if (!ctx || iters <= 0 || !ms_per_iter) return LUCAS_ERR_INVALID_ARG;
+    int rc = ctx->engine->sync();
+    if (rc != LUCAS_OK) return rc;
+    const double t0 = now_s();
+    rc = ctx->engine->step(iters);
+    if (rc == LUCAS_OK) rc = ctx->engine->sync();
+    *ms_per_iter = 1e3 * (now_s() - t0) / (double)iters;
+    return rc;
+}
+
+int lucas_ctx_save(lucas_ctx* ctx, const char* path) {
+    if (!ctx || !path) return LUCAS_ERR_INVALID_ARG;
+    std::vector<int32_t> d(ctx->job->plan.N);
+    int rc = job_get_digits(ctx->job, ctx->engine->stream(), d.data());
+    if (rc != LUCAS_OK) return rc;
+    return write_checkpoint(path, ctx->job->plan.p, ctx->job->plan.N, ctx->job->iter, d.data());
+}
+
+int lucas_ctx_load(const char* path, int device, lucas_ctx** out) {
+    if (!path || !out) return LUCAS_ERR_INVALID_ARG;
+    int p = 0, N = 0;
+    long long iter = 0;
+    std::vector<int32_t> d;
+    int rc = read_checkpoint(path, &p, &N, &iter, &d);
+    if (rc != LUCAS_OK) return rc;
+    lucas_ctx* c = nullptr;
+    rc = lucas_ctx_create(p, N, device, &c);
+    if (rc != LUCAS_OK) return rc;
+    rc = lucas_ctx_set_digits(c, d.data(), iter);
+    if (rc != LUCAS_OK) {
+        lucas_ctx_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return LUCAS_OK;
+}
+
+int lucas_mersenne_test(int p, int N, const lucas_options* opt, lucas_result* out) {
+    if (!out || p < 7) return LUCAS_ERR_INVALID_ARG;
+    memset(out, 0, sizeof(*out));
+    const int device = opt ? opt->device : 0;
+    lucas_ctx* ctx = nullptr;
+    int rc = LUCAS_ERR_IO;
+    if (opt && opt->resume && opt->checkpoint_path) {
+        rc = lucas_ctx_load(opt->checkpoint_path, device, &ctx);
+        if (rc == LUCAS_OK && ctx->job->plan.p != p) {
+            lucas_ctx_destroy(ctx);
+            return LUCAS_ERR_INVALID_ARG;  // checkpoint of another exponent (gpuLucas.cu:1616-1620)
+        }
+    }
+    if (rc != LUCAS_OK) {
+        rc = lucas_ctx_create(p, N, device, &ctx);
+        if (rc != LUCAS_OK) return rc;
+    }
+    const long long total = (long long)p - 2;
+    const long long start = ctx->job->iter;
+    const long long every = (opt && opt->checkpoint_every > 0 && opt->checkpoint_path) ? opt->checkpoint_every : 0;
+    const double t0 = now_s();
+    while (rc == LUCAS_OK && ctx->job->iter < total) {
+        long long chunk = total - ctx->job->iter;
+        if (every) chunk = std::min(chunk, every - (ctx->job->iter % every));
+        chunk = std::min(chunk, 1LL << 20);
+        rc = lucas_ctx_step(ctx, chunk);
+        if (rc == LUCAS_OK && every && ctx->job->iter % every == 0 && ctx->job->iter < total)
+            rc = lucas_ctx_save(ctx, opt->checkpoint_path);
+    }
+    const double t1 = now_s();
+    double err = 0.0;
+    if (rc == LUCAS_OK) rc = lucas_ctx_max_roundoff(ctx, 0, &err);
+    uint64_t r = 0;
+    int zero = 0;
+    if (rc == LUCAS_OK) rc = lucas_ctx_res64(ctx, &r, &zero);
+    out->p = p;
+    out->N = ctx->job->plan.N;
+    out->res64 = r;
+    out->is_prime = zero;
+    out->iterations = ctx->job->iter - start;
+    out->max_roundoff = err;
+    out->ms_per_iter = out->iterations > 0 ? 1e3 * (t1 - t0) / (double)out->iterations : 0.0;
+    lucas_ctx_destroy(ctx);
+    if (rc == LUCAS_OK && err >= error_limit_of(opt)) rc = LUCAS_ERR_ROUNDOFF;
+    return rc;
+}
+
+int lucas_error_test(int p, int N, int iterations, const lucas_options* opt, lucas_result* out) {
+    if (!out || p < 7) return LUCAS_ERR_INVALID_ARG;
+    memset(out, 0, sizeof(*out));
+    if (iterations <= 0) iterations = std::min(2500, std::max(100, p / 10000));  // gpuLucas.cu:529-541
+    if (iterations < 100) iterations = 100;                                   // gpuLucas.cu:1195-1196
+    lucas_ctx* ctx = nullptr;
+    int rc = lucas_ctx_create(p, N, opt ? opt->device : 0, &ctx);
+    if (rc != LUCAS_OK) return rc;
+    double ms = 0.0, err = 0.0;
+    // a short warm-up launch outside the timed region builds the graph
+    rc = lucas_ctx_time_iterations(ctx, iterations, &ms);
+    if (rc == LUCAS_OK) rc = lucas_ctx_max_roundoff(ctx, 0, &err);
+    uint64_t r = 0;
+    int zero = 0;
+    if (rc == LUCAS_OK) rc = lucas_ctx_res64(ctx, &r, &zero);
+    out->p = p;
+    out->N = ctx->job->plan.N;
+    out->res64 = r;
+    out->is_prime = 0;
+    out->iterations = iterations;
+    out->max_roundoff = err;
+    out->ms_per_iter = ms;
+    lucas_ctx_destroy(ctx);
+    if (rc == LUCAS_OK && err >= error_limit_of(opt)) rc = LUCAS_ERR_ROUNDOFF;
+    return rc;
+}
+
+int lucas_autotune_N(int p, int iters, const lucas_options* opt, lucas_plan_info* out) {
+    if (!out || p < 7) return LUCAS_ERR_INVALID_ARG;
+    if (iters < 20) iters = 50;
+    const double limit = error_limit_of(opt);
+    const int device = opt ? opt->device : 0;
+    // candidates: admissible N between 15.5 and 20.5 bits per word, smallest first
+    std::vector<int> cands;
+    for (long long N = std::max(4LL, (long long)(p / 20.5)); N <= (long long)(p / 15.5) && cands.size() < 8; ++N) {
+        if (!valid_length(N)) continue;
+        Plan pl;
+        if (make_plan(p, (int)N, 0, 0, &pl).empty()) cands.push_back((int)N);
+    }
+    if (cands.empty()) return LUCAS_ERR_INVALID_ARG;
+    double best_ms = 1e30;
+    Plan best;
+    int last_rc = LUCAS_ERR_ROUNDOFF;
+    for (int N : cands) {
+        Plan pl;
+        make_plan(p, N, 0, 0, &pl);
+        lucas_ctx* ctx = nullptr;
+        int rc = ctx_new(pl, device, &ctx);
+        if (rc != LUCAS_OK) {
+            last_rc = rc;
+            continue;
+        }
+        // steady-state roundoff from a pseudo-random balanced start
+        std::vector<int> w(N);
+        digit_widths(p, N, w.data());
+        std::vector<int32_t> d(N);
+        uint64_t s = 0x9E3779B97F4A7C15ULL ^ (uint64_t)p;
+        for (int j = 0; j < N; ++j) {
+            s ^= s << 13;
+            s ^= s >> 7;
+            s ^= s << 17;
+            const int32_t half = 1 << (w[j] - 1);
+            d[j] = (int32_t)(s % (uint64_t)(2 * half)) - half;
+        }
+        double ms = 0, err = 0;
+        rc = lucas_ctx_set_digits(ctx, d.data(), 0);
+        if (rc == LUCAS_OK) rc = lucas_ctx_time_iterations(ctx, 4, &ms);  // warm-up
+        if (rc == LUCAS_OK) rc = lucas_ctx_time_iterations(ctx, iters, &ms);
+        if (rc == LUCAS_OK) rc = lucas_ctx_max_roundoff(ctx, 0, &err);
+        lucas_ctx_destroy(ctx);
+        if (rc != LUCAS_OK) {
+            last_rc = rc;
+            continue;
+        }
+        if (err < limit && ms < best_ms) {
+            best_ms = ms;
+            best = pl;
+        }
+    }
+    if (best.N == 0) return last_rc;
+    fill_info(best, out);
+    return LUCAS_OK;
+}
+
+// ---- batches ----------------------------------------------------------------
+
+int lucas_batch_create(const int* ps, const int* Ns, int njobs, int device, lucas_batch** out) {
+    if (!ps || njobs <= 0 || !out) return LUCAS_ERR_INVALID_ARG;
+    std::vector<Plan> plans(njobs);
+    for (int i = 0; i < njobs; ++i) {
+        int rc = plan_for(ps[i], Ns ? Ns[i] : 0, 0, 0, &plans[i]);
+        if (rc != LUCAS_OK) return rc;
+    }
+    lucas_batch* b = new lucas_batch();
+    b->engine = new Engine(device);
+    int rc = b->engine->init();
+    for (int i = 0; i < njobs && rc == LUCAS_OK; ++i) {
+        Job* j = nullptr;
+        rc = job_create(plans[i], device, b->engine->stream(), &j);
+        if (rc == LUCAS_OK) b->jobs.push_back(j);
+    }
+    if (rc != LUCAS_OK) {
+        for (Job* j : b->jobs) job_destroy(j);
+        delete b->engine;
+        delete b;
+        return rc;
+    }
+    b->engine->set_jobs(b->jobs);
+    *out = b;
+    return LUCAS_OK;
+}
+
+int lucas_batch_destroy(lucas_batch* b) {
+    if (!b) return LUCAS_ERR_INVALID_ARG;
+    b->engine->sync();
+    for (Job* j : b->jobs) job_destroy(j);
+    delete b->engine;
+    delete b;
+    return LUCAS_OK;
+}
+
+int lucas_batch_step(lucas_batch* b, long long iters) {
+    if (!b || iters < 0) return LUCAS_ERR_INVALID_ARG;
+    int rc = b->engine->step(iters);
+    if (rc == LUCAS_OK) rc = b->engine->sync();
+    return rc;
+}
+
+int lucas_batch_res64(lucas_batch* b, uint64_t* res64, int* is_zero, double* max_roundoff) {
+    if (!b) return LUCAS_ERR_INVALID_ARG;
+    for (size_t i = 0; i < b->jobs.size(); ++i) {
+        Job* j = b->jobs[i];
+        std::vector<int32_t> d(j->plan.N);
+        int rc = job_get_digits(j, b->engine->stream(), d.data());
+        if (rc != LUCAS_OK) return rc;
+        uint64_t r = 0;
+        int z = 0;
+        normalize_digits(j->plan.p, j->plan.N, d.data(), nullptr, &r, &z);
+        if (res64) res64[i] = r;
+        if (is_zero) is_zero[i] = z;
+        if (max_roundoff) {
+            rc = job_read_maxerr(j, b->engine->stream(), 0, &max_roundoff[i]);
+            if (rc != LUCAS_OK) return rc;
+        }
+    }
+    return LUCAS_OK;
+}
+
+int lucas_batch_get_digits(lucas_batch* b, int job, int32_t* digits) {
+    if (!b || job < 0 || job >= (int)b->jobs.size() || !digits) return LUCAS_ERR_INVALID_ARG;
+    return job_get_digits(b->jobs[job], b->engine->stream(), digits);
+}
+
+int lucas_batch_time_iterations(lucas_batch* b, long long iters, double* ms_per_iter) {
+    if (!b || iters <= 0 || !ms_per_iter) return LUCAS_ERR_INVALID_ARG;
+    int rc = b->engine->sync();
+    if (rc != LUCAS_OK) return rc;
+    const double t0 = now_s();
+    rc = b->engine->step(iters);
+    if (rc == LUCAS_OK) rc = b->engine->sync();
+    *ms_per_iter = 1e3 * (now_s() - t0) / (double)iters;
+    return rc;
+}
+
+}  // extern "C"
+
+// ---- serving ----------------------------------------------------------------
+//
+// Continuous batching: a worker thread owns the device.  Each tick it admits
+// queued requests (FIFO) while job slots and the HBM budget allow, advances
+// every resident job by one chunk in a single batched launch sequence, and
+// retires the jobs that reached their iteration count.  A job's arithmetic is
+// independent of its neighbours in the batch, so its residue is the same
+// alone or in any mix.
+
+struct lucas_server {
+    struct Pending {
+        long long id;
+        Plan plan;
+        long long iters;
+    };
+    struct Active {
+        long long id;
+        Job* job;
+        long long target;
+    };
+    lucas_server_options opt{};
+    std::string ckdir;
+    Engine* engine = nullptr;
+    std::thread worker;
+    std::mutex mu;
+    std::condition_variable cv;       // work arrived / stop
+    std::condition_variable done_cv;  // a job finished
+    std::deque<Pending> queue;
+    std::map<long long, lucas_job_result> records;
+    std::vector<Active> active;
+    size_t budget = 0, used = 0;
+    long long next_id = 1;
+    long long outstanding = 0;
+    bool stopping = false;
+    double t0 = 0;
+    long long iters_done = 0;
+    double busy_s = 0;
+    int peak_jobs = 0;
+    int worker_rc = LUCAS_OK;
+
+    void run();
+};
+
+void lucas_server::run() {
+    hipSetDevice(opt.device);
+    const int max_jobs = opt.max_jobs > 0 ? opt.max_jobs : 256;
+    const long long chunk_max = opt.chunk_iters > 0 ? opt.chunk_iters : 256;
+    for (;;) {
+        std::vector<Pending> admit;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv.wait(lk, [&] { return stopping || !queue.empty() || !active.empty(); });
+            if (stopping) break;
+            while (!queue.empty() && (int)(active.size() + admit.size()) < max_jobs) {
+                const size_t need = job_bytes(queue.front().plan);
+                if (used + need > budget && !(active.empty() && admit.empty())) break;
+                used += need;
+                admit.push_back(queue.front());
+                queue.pop_front();
+            }
+        }
+        bool changed = !admit.empty();
+        for (const Pending& pnd : admit) {
+            Job* j = nullptr;
+            int rc = job_create(pnd.plan, opt.device, engine->stream(), &j);
+            std::lock_guard<std::mutex> lk(mu);
+            lucas_job_result& rec = records[pnd.id];
+            rec.start_s = now_s() - t0;
+            if (rc != LUCAS_OK) {
+                used -= job_bytes(pnd.plan);
+                rec.done = 1;
+                rec.status = rc;
+                rec.finish_s = rec.start_s;
+                --outstanding;
+                done_cv.notify_all();
+                continue;
+            }
+            active.push_back({pnd.id, j, pnd.iters});
+        }
+        if (active.empty()) continue;
+        if (changed) {
+            std::vector<Job*> js;
+            for (const Active& a : active) js.push_back(a.job);
+            engine->set_jobs(js);
+        }
+        long long chunk = chunk_max;
+        for (const Active& a : active) chunk = std::min(chunk, a.target - a.job->iter);
+        const double tb = now_s();
+        int rc = chunk > 0 ? engine->step(chunk) : LUCAS_OK;
+        if (rc == LUCAS_OK) rc = engine->sync();
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            busy_s += now_s() - tb;
+            if (rc != LUCAS_OK) worker_rc = rc;
+            iters_done += chunk * (long long)active.size();
+            peak_jobs = std::max(peak_jobs, (int)active.size());
+        }
+        // retire finished jobs (and checkpoint the others when due)
+        std::vector<Active> keep;
+        bool retired = false;
+        for (Active& a : active) {
+            const bool finished = a.job->iter >= a.target || rc != LUCAS_OK;
+            if (!finished) {
+                if (!ckdir.empty() && opt.checkpoint_every > 0 && a.job->iter % opt.checkpoint_every < chunk) {
+                    std::vector<int32_t> d(a.job->plan.N);
+                    if (job_get_digits(a.job, engine->stream(), d.data()) == LUCAS_OK) {
+                        std::string path = ckdir + "/" + std::to_string(a.job->plan.p) + ".chk";
+                        write_checkpoint(path.c_str(), a.job->plan.p, a.job->plan.N, a.job->iter, d.data());
+                    }
+                }
+                keep.push_back(a);
+                continue;
+            }
+            lucas_job_result res;
+            memset(&res, 0, sizeof(res));
+            res.status = rc;
+            if (rc == LUCAS_OK) {
+                std::vector<int32_t> d(a.job->plan.N);
+                res.status = job_get_digits(a.job, engine->stream(), d.data());
+                if (res.status == LUCAS_OK) normalize_digits(a.job->plan.p, a.job->plan.N, d.data(), nullptr, &res.res64, &res.is_zero);
+                if (res.status == LUCAS_OK) res.status = job_read_maxerr(a.job, engine->stream(), 0, &res.max_roundoff);
+            }
+            const size_t bytes = job_bytes(a.job->plan);
+            job_destroy(a.job);
+            retired = true;
+            std::lock_guard<std::mutex> lk(mu);
+            lucas_job_result& rec = records[a.id];
+            rec.done = 1;
+            rec.status = res.status;
+            rec.res64 = res.res64;
+            rec.is_zero = res.is_zero;
+            rec.max_roundoff = res.max_roundoff;
+            rec.finish_s = now_s() - t0;
+            used -= bytes;
+            --outstanding;
+            done_cv.notify_all();
+        }
+        if (retired) {
+            std::lock_guard<std::mutex> lk(mu);
+            active.swap(keep);
+            std::vector<Job*> js;
+            for (const Active& a : active) js.push_back(a.job);
+            engine->set_jobs(js);
+        }
+    }
+    // shut down: release resident jobs
+    for (Active& a : active) job_destroy(a.job);
+    active.clear();
+}
+
+extern "C" {
+
+int lucas_server_start(const lucas_server_options* opt, lucas_server** out) {
+    if (!out) return LUCAS_ERR_INVALID_ARG;
+    lucas_server* s = new lucas_server();
+    if (opt) s->opt = *opt;
+    if (opt && opt->checkpoint_dir) s->ckdir = opt->checkpoint_dir;
+    s->opt.checkpoint_dir = nullptr;
+    s->engine = new Engine(s->opt.device);
+    int rc = s->engine->init();
+    if (rc != LUCAS_OK) {
+        delete s->engine;
+        delete s;
+        return rc;
+    }
+    size_t freeb = 0, total = 0;
+    if (hipMemGetInfo(&freeb, &total) != hipSuccess) freeb = 0;
+    s->budget = s->opt.hbm_budget_bytes ? s->opt.hbm_budget_bytes : (size_t)(0.8 * (double)freeb);
+    s->t0 = now_s();
+    s->worker = std::thread([s] { s->run(); });
+    *out = s;
+    return LUCAS_OK;
+}
+
+int lucas_server_submit(lucas_server* s, int p, int N, long long iterations, long long* id) {
+    if (!s || iterations <= 0) return LUCAS_ERR_INVALID_ARG;
+    Plan pl;
+    int rc = plan_for(p, N, 0, 0, &pl);
+    if (rc != LUCAS_OK) return rc;
+    if (job_bytes(pl) > s->budget) return LUCAS_ERR_BUSY;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stopping) return LUCAS_ERR_BUSY;
+    const long long jid = s->next_id++;
+    lucas_job_result& rec = s->records[jid];
+    memset(&rec, 0, sizeof(rec));
+    rec.id = jid;
+    rec.p = p;
+    rec.N = pl.N;
+    rec.iterations = iterations;
+    rec.submit_s = now_s() - s->t0;
+    s->queue.push_back({jid, pl, iterations});
+    ++s->outstanding;
+    s->cv.notify_all();
+    if (id) *id = jid;
+    return LUCAS_OK;
+}
+
+int lucas_server_poll(lucas_server* s, long long id, lucas_job_result* out) {
+    if (!s || !out) return LUCAS_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->records.find(id);
+    if (it == s->records.end()) return LUCAS_ERR_INVALID_ARG;
+    *out = it->second;
+    return LUCAS_OK;
+}
+
+int lucas_server_drain(lucas_server* s, double timeout_s) {
+    if (!s) return LUCAS_ERR_INVALID_ARG;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto pred = [&] { return s->outstanding == 0; };
+    if (timeout_s > 0) {
+        if (!s->done_cv.wait_for(lk, std::chrono::duration<double>(timeout_s), pred)) return LUCAS_ERR_BUSY;
+    } else {
+        s->done_cv.wait(lk, pred);
+    }
+    return s->worker_rc;
+}
+
+int lucas_server_stats(lucas_server* s, long long* iterations, double* busy_s, int* peak_jobs) {
+    if (!s) return LUCAS_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (iterations) *iterations = s->iters_done;
+    if (busy_s) *busy_s = s->busy_s;
+    if (peak_jobs) *peak_jobs = s->peak_jobs;
+    return LUCAS_OK;
+}
+
+int lucas_server_stop(lucas_server* s) {
+    if (!s) return LUCAS_ERR_INVALID_ARG;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stopping = true;
+        s->cv.notify_all();
+    }
+    if (s->worker.joinable()) s->worker.join();
+    delete s->engine;
+    delete s;
+    return LUCAS_OK;
+}
+
+}  // extern "C"

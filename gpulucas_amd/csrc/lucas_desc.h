@@ -1,0 +1,57 @@
+// lucas_desc.h -- per-exponent job descriptor shared by the host engine and the
+// HIP kernels.  One descriptor describes one Mersenne exponent p transformed at
+// length N; a kernel launch covers a list of descriptors (one exponent in the
+// plain path, many in the batched and served paths).
+//
+// Data layout in HBM for one job (M = N/2 complex FP64 words, M = A*B):
+//   element m = c + B*r packs digits (2m, 2m+1) as one complex word; the M
+//   words form a row-major A x B matrix (row r, column c).
+//   * spec  : written by the column kernel, read by the row kernel.
+//   * spec2 : written by the row kernel, read by the column kernel.
+//   * digits: N int32 pseudo-balanced digits (state between step() calls).
+// Per-digit metadata is regenerated from small tables: digit j has
+// r_j = (-j*p) mod N, width q+1 iff r_j < (p mod N), IBDWT weight 2^(r_j/N).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#define LUCAS_MAX_RADICES 16
+// Elements (complex words) each thread may hold while an FFT stage is in flight.
+#define LUCAS_EMAX 16
+// Threads per workgroup for both kernels.
+#define LUCAS_THREADS 256
+// Deepest carry-save restoration supported (reference carry depth n <= 6).
+#define LUCAS_NMAX 6
+
+// Column-kernel modes.
+#define LUCAS_IN_DIGITS 1   // start from int32 digits instead of spec2
+#define LUCAS_OUT_DIGITS 2  // stop after the carry and write int32 digits
+
+struct LucasJobDesc {
+    int p, N, M, A, B;      // exponent, signal length, complex length, M = A*B
+    int q, b;               // q = p / N (low word width), b = p % N (# wide words)
+    int ncarry;             // carry-save depth n (products spread over n+1 digits)
+    int halo;               // halo columns needed by the carry: ceil((n+1)/2)
+    int C;                  // columns per column workgroup (B % C == 0)
+    int col_wgs, row_wgs;   // workgroups per launch of each kernel
+    int nradA, nradB;
+    int radA[LUCAS_MAX_RADICES];
+    int radB[LUCAS_MAX_RADICES];
+    // Twiddle tables (forward sign, exp(-2*pi*i*t/L)), rounded from long double.
+    const double2* twA;     // omega_A^t, t < A
+    const double2* twB;     // omega_B^t, t < B
+    const double2* twMA;    // omega_M^t, t < A
+    const double2* twMB;    // omega_M^t, t < B
+    // Weight tables: j = Jlo + 2B*r;  r_j = (rlo[Jlo] + rhi[r]) mod N.
+    const int* rlo;         // (-Jlo*p) mod N, Jlo < 2B
+    const int* rhi;         // (-2B*r*p) mod N, r < A
+    const double* wlo;      // 2^(rlo/N)
+    const double* whi;      // 2^(rhi/N)
+    const double* iwlo;     // 2^(-rlo/N)
+    const double* iwhi;     // 2^(-rhi/N) / (4M)   (inverse-FFT scale folded in)
+    // State.
+    double2* spec;
+    double2* spec2;
+    int* digits;
+    unsigned int* maxerr;   // max |x - rint(x)| since last reset, as float bits
+};

@@ -1,0 +1,340 @@
+// lucas_engine.cpp -- device memory, launch sequencing and HIP graphs.
+//
+// Iteration order for `iters` squarings of every job (digits in, digits out):
+//   col(IN_DIGITS)                      weight + forward column FFT
+//   [row ; col] x (iters - 1)           one LL iteration each
+//   row ; col(OUT_DIGITS)               last iteration, stop after the carry
+// The steady-state [row ; col] pairs are replayed from a captured hipGraph.
+// This replaces the reference's per-iteration launch train (2 cuFFT execs + 4
+// kernels with a cutilCheckMsg after each, gpuLucas.cu:845-942).
+#include "lucas_engine.h"
+
+#include <string.h>
+
+#include "gpulucas.h"
+
+namespace lucas {
+
+hipError_t launch_col_kernel(const LucasJobDesc* jobs, const int2* map, int nwg, size_t lds, int mode,
+                             hipStream_t stream);
+hipError_t launch_row_kernel(const LucasJobDesc* jobs, const int2* map, int nwg, size_t lds, hipStream_t stream);
+
+static const int kGraphIters = 32;
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int job_create(const Plan& plan, int device, hipStream_t stream, Job** out) {
+    if (hipSetDevice(device) != hipSuccess) return LUCAS_ERR_NO_DEVICE;
+    Job* job = new Job();
+    job->plan = plan;
+    job->device = device;
+    job->bytes = job_bytes(plan);
+    hipError_t e = hipMalloc((void**)&job->mem, job->bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        delete job;
+        return e == hipErrorOutOfMemory ? LUCAS_ERR_OOM : LUCAS_ERR_DEVICE;
+    }
+    Tables t;
+    make_tables(plan, &t);
+    // carve the allocation (layout mirrors job_bytes)
+    std::vector<char> host(job->bytes, 0);
+    size_t off = 0;
+    auto take = [&](size_t n) {
+        size_t o = off;
+        off += align256(n);
+        return o;
+    };
+    const size_t o_spec = take(sizeof(double2) * plan.M);
+    const size_t o_spec2 = take(sizeof(double2) * plan.M);
+    const size_t o_dig = take(sizeof(int) * plan.N);
+    const size_t o_twA = take(sizeof(double2) * plan.A);
+    const size_t o_twMA = take(sizeof(double2) * plan.A);
+    const size_t o_twB = take(sizeof(double2) * plan.B);
+    const size_t o_twMB = take(sizeof(double2) * plan.B);
+    const size_t o_rlo = take(sizeof(int) * 2 * plan.B);
+    const size_t o_rhi = take(sizeof(int) * plan.A);
+    const size_t o_wlo = take(sizeof(double) * 2 * plan.B);
+    const size_t o_iwlo = take(sizeof(double) * 2 * plan.B);
+    const size_t o_whi = take(sizeof(double) * plan.A);
+    const size_t o_iwhi = take(sizeof(double) * plan.A);
+    const size_t o_err = take(sizeof(unsigned int));
+    if (off > job->bytes) {
+        hipFree(job->mem);
+        delete job;
+        return LUCAS_ERR_INVALID_ARG;
+    }
+    // Host image of everything from the digits on (spectra need no init).
+    int* dig = (int*)(host.data() + o_dig);
+    dig[0] = 4;  // s_0 = 4 with a_0 = 1 (loadValue4ToFFTarray, gpuLucas.cu:718-726)
+    memcpy(host.data() + o_twA, t.twA.data(), sizeof(double) * t.twA.size());
+    memcpy(host.data() + o_twMA, t.twMA.data(), sizeof(double) * t.twMA.size());
+    memcpy(host.data() + o_twB, t.twB.data(), sizeof(double) * t.twB.size());
+    memcpy(host.data() + o_twMB, t.twMB.data(), sizeof(double) * t.twMB.size());
+    memcpy(host.data() + o_rlo, t.rlo.data(), sizeof(int) * t.rlo.size());
+    memcpy(host.data() + o_rhi, t.rhi.data(), sizeof(int) * t.rhi.size());
+    memcpy(host.data() + o_wlo, t.wlo.data(), sizeof(double) * t.wlo.size());
+    memcpy(host.data() + o_iwlo, t.iwlo.data(), sizeof(double) * t.iwlo.size());
+    memcpy(host.data() + o_whi, t.whi.data(), sizeof(double) * t.whi.size());
+    memcpy(host.data() + o_iwhi, t.iwhi.data(), sizeof(double) * t.iwhi.size());
+    e = hipMemcpyAsync(job->mem + o_dig, host.data() + o_dig, job->bytes - o_dig, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) {
+        hipFree(job->mem);
+        delete job;
+        return LUCAS_ERR_DEVICE;
+    }
+    LucasJobDesc& d = job->desc;
+    d.p = plan.p;
+    d.N = plan.N;
+    d.M = plan.M;
+    d.A = plan.A;
+    d.B = plan.B;
+    d.q = plan.q;
+    d.b = plan.b;
+    d.ncarry = plan.ncarry;
+    d.halo = plan.halo;
+    d.C = plan.C;
+    d.col_wgs = plan.col_wgs;
+    d.row_wgs = plan.row_wgs;
+    d.nradA = (int)plan.radA.size();
+    d.nradB = (int)plan.radB.size();
+    for (int i = 0; i < d.nradA; ++i) d.radA[i] = plan.radA[i];
+    for (int i = 0; i < d.nradB; ++i) d.radB[i] = plan.radB[i];
+    char* m = job->mem;
+    d.spec = (double2*)(m + o_spec);
+    d.spec2 = (double2*)(m + o_spec2);
+    d.digits = (int*)(m + o_dig);
+    d.twA = (const double2*)(m + o_twA);
+    d.twMA = (const double2*)(m + o_twMA);
+    d.twB = (const double2*)(m + o_twB);
+    d.twMB = (const double2*)(m + o_twMB);
+    d.rlo = (const int*)(m + o_rlo);
+    d.rhi = (const int*)(m + o_rhi);
+    d.wlo = (const double*)(m + o_wlo);
+    d.iwlo = (const double*)(m + o_iwlo);
+    d.whi = (const double*)(m + o_whi);
+    d.iwhi = (const double*)(m + o_iwhi);
+    d.maxerr = (unsigned int*)(m + o_err);
+    job->iter = 0;
+    *out = job;
+    return LUCAS_OK;
+}
+
+void job_destroy(Job* job) {
+    if (!job) return;
+    if (job->mem) {
+        hipSetDevice(job->device);
+        hipFree(job->mem);
+    }
+    delete job;
+}
+
+int job_get_digits(Job* job, hipStream_t stream, int32_t* digits) {
+    hipError_t e = hipMemcpyAsync(digits, job->desc.digits, sizeof(int) * job->plan.N, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    return e == hipSuccess ? LUCAS_OK : LUCAS_ERR_DEVICE;
+}
+
+int job_set_digits(Job* job, hipStream_t stream, const int32_t* digits) {
+    hipError_t e = hipMemcpyAsync(job->desc.digits, digits, sizeof(int) * job->plan.N, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    return e == hipSuccess ? LUCAS_OK : LUCAS_ERR_DEVICE;
+}
+
+int job_read_maxerr(Job* job, hipStream_t stream, int reset, double* out) {
+    unsigned int bits = 0;
+    hipError_t e = hipMemcpyAsync(&bits, job->desc.maxerr, sizeof(bits), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return LUCAS_ERR_DEVICE;
+    float f;
+    memcpy(&f, &bits, sizeof(f));
+    *out = (double)f;
+    if (reset) {
+        const unsigned int zero = 0;
+        e = hipMemcpyAsync(job->desc.maxerr, &zero, sizeof(zero), hipMemcpyHostToDevice, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) return LUCAS_ERR_DEVICE;
+    }
+    return LUCAS_OK;
+}
+
+Engine::Engine(int device) : device_(device) {}
+
+Engine::~Engine() {
+    hipSetDevice(device_);
+    if (gexec_) hipGraphExecDestroy(gexec_);
+    if (graph_) hipGraphDestroy(graph_);
+    if (d_desc_) hipFree(d_desc_);
+    if (d_colmap_) hipFree(d_colmap_);
+    if (d_rowmap_) hipFree(d_rowmap_);
+    if (ev0_) hipEventDestroy(ev0_);
+    if (ev1_) hipEventDestroy(ev1_);
+    if (stream_) hipStreamDestroy(stream_);
+}
+
+int Engine::init() {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        (void)hipGetLastError();
+        return LUCAS_ERR_NO_DEVICE;
+    }
+    if (device_ < 0 || device_ >= count) return LUCAS_ERR_INVALID_ARG;
+    if (hipSetDevice(device_) != hipSuccess) return LUCAS_ERR_NO_DEVICE;
+    if (hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking) != hipSuccess) return LUCAS_ERR_DEVICE;
+    if (hipEventCreate(&ev0_) != hipSuccess || hipEventCreate(&ev1_) != hipSuccess) return LUCAS_ERR_DEVICE;
+    return LUCAS_OK;
+}
+
+void Engine::set_jobs(const std::vector<Job*>& jobs) {
+    jobs_ = jobs;
+    dirty_ = true;
+}
+
+int Engine::prepare() {
+    if (!dirty_) return LUCAS_OK;
+    hipSetDevice(device_);
+    if (gexec_) {
+        hipGraphExecDestroy(gexec_);
+        gexec_ = nullptr;
+    }
+    if (graph_) {
+        hipGraphDestroy(graph_);
+        graph_ = nullptr;
+    }
+    std::vector<LucasJobDesc> desc;
+    std::vector<int2> cmap, rmap;
+    lds_col_ = lds_row_ = 0;
+    for (size_t i = 0; i < jobs_.size(); ++i) {
+        const Job* j = jobs_[i];
+        desc.push_back(j->desc);
+        for (int w = 0; w < j->plan.col_wgs; ++w) cmap.push_back(make_int2((int)i, w));
+        for (int w = 0; w < j->plan.row_wgs; ++w) rmap.push_back(make_int2((int)i, w));
+        if (j->plan.lds_col > lds_col_) lds_col_ = j->plan.lds_col;
+        if (j->plan.lds_row > lds_row_) lds_row_ = j->plan.lds_row;
+    }
+    auto grow = [&](void** ptr, size_t* cap, size_t need) -> bool {
+        if (need <= *cap) return true;
+        if (*ptr) hipFree(*ptr);
+        *ptr = nullptr;
+        *cap = 0;
+        size_t n = need * 2;
+        if (hipMalloc(ptr, n) != hipSuccess) return false;
+        *cap = n;
+        return true;
+    };
+    if (!grow((void**)&d_desc_, &cap_desc_, sizeof(LucasJobDesc) * (desc.size() + 1)) ||
+        !grow((void**)&d_colmap_, &cap_col_, sizeof(int2) * (cmap.size() + 1)) ||
+        !grow((void**)&d_rowmap_, &cap_row_, sizeof(int2) * (rmap.size() + 1)))
+        return LUCAS_ERR_OOM;
+    hipError_t e = hipSuccess;
+    if (!desc.empty()) {
+        e = hipMemcpyAsync(d_desc_, desc.data(), sizeof(LucasJobDesc) * desc.size(), hipMemcpyHostToDevice, stream_);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_colmap_, cmap.data(), sizeof(int2) * cmap.size(), hipMemcpyHostToDevice, stream_);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_rowmap_, rmap.data(), sizeof(int2) * rmap.size(), hipMemcpyHostToDevice, stream_);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+    }
+    if (e != hipSuccess) return LUCAS_ERR_DEVICE;
+    ncol_ = (int)cmap.size();
+    nrow_ = (int)rmap.size();
+    dirty_ = false;
+    return LUCAS_OK;
+}
+
+int Engine::launch_col(int mode) {
+    return launch_col_kernel(d_desc_, d_colmap_, ncol_, lds_col_, mode, stream_) == hipSuccess ? LUCAS_OK
+                                                                                               : LUCAS_ERR_DEVICE;
+}
+
+int Engine::launch_row() {
+    return launch_row_kernel(d_desc_, d_rowmap_, nrow_, lds_row_, stream_) == hipSuccess ? LUCAS_OK
+                                                                                       : LUCAS_ERR_DEVICE;
+}
+
+int Engine::build_graph() {
+    if (gexec_) return LUCAS_OK;
+    hipError_t e = hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return LUCAS_ERR_DEVICE;
+    int rc = LUCAS_OK;
+    for (int i = 0; i < kGraphIters && rc == LUCAS_OK; ++i) {
+        rc = launch_row();
+        if (rc == LUCAS_OK) rc = launch_col(0);
+    }
+    e = hipStreamEndCapture(stream_, &graph_);
+    if (rc != LUCAS_OK || e != hipSuccess) {
+        if (graph_) hipGraphDestroy(graph_);
+        graph_ = nullptr;
+        (void)hipGetLastError();
+        return LUCAS_ERR_DEVICE;
+    }
+    e = hipGraphInstantiate(&gexec_, graph_, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        hipGraphDestroy(graph_);
+        graph_ = nullptr;
+        gexec_ = nullptr;
+        return LUCAS_ERR_DEVICE;
+    }
+    graph_iters_ = kGraphIters;
+    return LUCAS_OK;
+}
+
+int Engine::step(long long iters) {
+    if (iters <= 0) return iters == 0 ? LUCAS_OK : LUCAS_ERR_INVALID_ARG;
+    if (jobs_.empty()) return LUCAS_OK;
+    hipSetDevice(device_);
+    int rc = prepare();
+    if (rc != LUCAS_OK) return rc;
+    rc = launch_col(LUCAS_IN_DIGITS);
+    long long rem = iters - 1;
+    if (rc == LUCAS_OK && rem >= kGraphIters) {
+        rc = build_graph();
+        while (rc == LUCAS_OK && rem >= graph_iters_) {
+            if (hipGraphLaunch(gexec_, stream_) != hipSuccess) rc = LUCAS_ERR_DEVICE;
+            rem -= graph_iters_;
+        }
+    }
+    while (rc == LUCAS_OK && rem > 0) {
+        rc = launch_row();
+        if (rc == LUCAS_OK) rc = launch_col(0);
+        --rem;
+    }
+    if (rc == LUCAS_OK) rc = launch_row();
+    if (rc == LUCAS_OK) rc = launch_col(LUCAS_OUT_DIGITS);
+    if (rc == LUCAS_OK)
+        for (Job* j : jobs_) j->iter += iters;
+    return rc;
+}
+
+int Engine::sync() {
+    hipSetDevice(device_);
+    return hipStreamSynchronize(stream_) == hipSuccess ? LUCAS_OK : LUCAS_ERR_DEVICE;
+}
+
+int Engine::time_kernels(int iters, double* col_us, double* row_us) {
+    if (iters <= 0 || jobs_.empty()) return LUCAS_ERR_INVALID_ARG;
+    hipSetDevice(device_);
+    int rc = prepare();
+    if (rc != LUCAS_OK) return rc;
+    // valid spectra in both buffers (digits untouched)
+    rc = launch_col(LUCAS_IN_DIGITS);
+    if (rc == LUCAS_OK) rc = launch_row();
+    if (rc != LUCAS_OK) return rc;
+    float ms = 0.f;
+    if (hipEventRecord(ev0_, stream_) != hipSuccess) return LUCAS_ERR_DEVICE;
+    for (int i = 0; i < iters && rc == LUCAS_OK; ++i) rc = launch_col(0);
+    if (rc != LUCAS_OK || hipEventRecord(ev1_, stream_) != hipSuccess) return LUCAS_ERR_DEVICE;
+    if (hipEventSynchronize(ev1_) != hipSuccess || hipEventElapsedTime(&ms, ev0_, ev1_) != hipSuccess)
+        return LUCAS_ERR_DEVICE;
+    *col_us = 1000.0 * ms / iters;
+    if (hipEventRecord(ev0_, stream_) != hipSuccess) return LUCAS_ERR_DEVICE;
+    for (int i = 0; i < iters && rc == LUCAS_OK; ++i) rc = launch_row();
+    if (rc != LUCAS_OK || hipEventRecord(ev1_, stream_) != hipSuccess) return LUCAS_ERR_DEVICE;
+    if (hipEventSynchronize(ev1_) != hipSuccess || hipEventElapsedTime(&ms, ev0_, ev1_) != hipSuccess)
+        return LUCAS_ERR_DEVICE;
+    *row_us = 1000.0 * ms / iters;
+    return LUCAS_OK;
+}
+
+}  // namespace lucas

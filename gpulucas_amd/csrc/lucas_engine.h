@@ -1,0 +1,74 @@
+// lucas_engine.h -- device-side job state and the launch engine (HIP runtime).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "lucas_desc.h"
+#include "lucas_host.h"
+
+namespace lucas {
+
+// One exponent's device-resident state: residue digits, two spectrum
+// buffers, weight/twiddle tables and the roundoff accumulator, carved from a
+// single allocation.
+struct Job {
+    Plan plan;
+    int device = 0;
+    char* mem = nullptr;
+    size_t bytes = 0;
+    LucasJobDesc desc{};
+    long long iter = 0;
+};
+
+// Creates a job with s_0 = 4 (digits = [4, 0, ...]).  Returns a LUCAS_* code.
+int job_create(const Plan& plan, int device, hipStream_t stream, Job** out);
+void job_destroy(Job* job);
+int job_get_digits(Job* job, hipStream_t stream, int32_t* digits);
+int job_set_digits(Job* job, hipStream_t stream, const int32_t* digits);
+int job_read_maxerr(Job* job, hipStream_t stream, int reset, double* out);
+
+// A set of jobs advanced together: one launch of each kernel per iteration
+// covers every job (workgroup -> (job, tile) maps built on the host).
+class Engine {
+public:
+    explicit Engine(int device);
+    ~Engine();
+    int init();
+    int device() const { return device_; }
+    hipStream_t stream() const { return stream_; }
+
+    void set_jobs(const std::vector<Job*>& jobs);  // composition change
+    const std::vector<Job*>& jobs() const { return jobs_; }
+
+    // Advances every job by `iters` (>= 1) iterations; digits in, digits out.
+    int step(long long iters);
+    // Mean device microseconds of each kernel over `iters` back-to-back
+    // launches (state is left valid: digits are not touched).
+    int time_kernels(int iters, double* col_us, double* row_us);
+    int sync();
+
+private:
+    int prepare();
+    int launch_col(int mode);
+    int launch_row();
+    int build_graph();
+
+    int device_;
+    hipStream_t stream_ = nullptr;
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    std::vector<Job*> jobs_;
+    bool dirty_ = true;
+    LucasJobDesc* d_desc_ = nullptr;
+    int2* d_colmap_ = nullptr;
+    int2* d_rowmap_ = nullptr;
+    size_t cap_desc_ = 0, cap_col_ = 0, cap_row_ = 0;
+    int ncol_ = 0, nrow_ = 0;
+    size_t lds_col_ = 0, lds_row_ = 0;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t gexec_ = nullptr;
+    int graph_iters_ = 0;
+};
+
+}  // namespace lucas

@@ -1,0 +1,239 @@
+// lucas_host.cpp -- planning, extended-precision tables and residue arithmetic
+// (host only).  Replaces the reference's host setup:
+//   initConstantSymbols     gpuLucas.cu:222-237   (q, bases, masks)
+//   setSliceAndDice         gpuLucas.cu:323-343   (carry depth n from q)
+//   computeWeightVectors    gpuLucas.cu:673-686   (QD dd_real weights)
+//   computeBitsPerWord(+8)  gpuLucas.cu:688-714   (widths and HI masks)
+//   print_residue           gpuLucas.cu:1288-1352 (Res64)
+//   rebalanceIrrIntSEQGPU   IrrBaseBalanced.cu:133-169 (final normalisation)
+// Widths and weights derive from the exact integer r_j = (-j p) mod N:
+// ceil(jp/N) - jp/N = r_j / N, so no cancellation-prone real arithmetic (the
+// reason the reference needed double-double) is left; only 2^(r/N) and the
+// twiddles use long double, rounded once to double.
+#include "lucas_host.h"
+
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <limits>
+
+#include "lucas_desc.h"
+
+namespace lucas {
+
+static const long double kPi = 3.141592653589793238462643383279502884L;
+
+bool valid_length(long long N) {
+    if (N < 4 || (N & 1)) return false;
+    for (int f : {2, 3, 5, 7})
+        while (N % f == 0) N /= f;
+    return N == 1;
+}
+
+int carry_depth(int q) {
+    if (q >= 21) return -1;
+    if (q >= 18) return 2;
+    if (q >= 16) return 3;
+    if (q >= 14) return 4;
+    if (q >= 12) return 5;
+    return 6;
+}
+
+std::vector<int> radix_plan(int L) {
+    std::vector<int> r;
+    int twos = 0;
+    while (L % 2 == 0) { L /= 2; ++twos; }
+    // 2^k as 8s, with a 4x4 instead of 8x2 when k mod 3 == 1.
+    int eights = twos / 3, rem = twos % 3;
+    if (rem == 1 && eights > 0) { --eights; rem = 4; }
+    for (int i = 0; i < eights; ++i) r.push_back(8);
+    if (rem == 1) r.push_back(2);
+    if (rem == 2) r.push_back(4);
+    if (rem == 4) { r.push_back(4); r.push_back(4); }
+    for (int f : {7, 5, 3})
+        while (L % f == 0) { L /= f; r.push_back(f); }
+    if (L != 1) r.clear();
+    return r;
+}
+
+static long long stage_cost(const std::vector<int>& rad, long long elems) {
+    // per-thread element passes through LDS for one FFT of the tile
+    long long per = (elems + LUCAS_THREADS - 1) / LUCAS_THREADS;
+    return (long long)rad.size() * per;
+}
+
+std::string make_plan(int p, int N, int forceA, int forceC, Plan* out) {
+    if (p < 7) return "p must be >= 7";
+    if (!valid_length(N)) return "N must be even and of the form 2^a 3^b 5^c 7^d";
+    if ((long long)N > (long long)p) return "N must not exceed p";
+    Plan pl;
+    pl.p = p;
+    pl.N = N;
+    pl.M = N / 2;
+    pl.q = p / N;
+    pl.b = p % N;
+    pl.ncarry = carry_depth(pl.q);
+    if (pl.ncarry < 0) return "p/N >= 21 bits per word is out of range";
+    pl.halo = (pl.ncarry + 2) / 2;  // ceil((n+1)/2)
+    // The restoration keeps product bits below the sum of n+1 digit widths;
+    // a product has at most 2(q+1) + log2(N) + 1 bits, and only products
+    // below 2^53 are exact FP64 integers in the first place.
+    {
+        int coverage = (pl.ncarry + 1) * pl.q;
+        int need = std::min(53, 2 * (pl.q + 1) + (int)ceil(log2((double)N)) + 1);
+        if (coverage < need) return "word size too small for the carry-save restoration";
+    }
+    const long long cap = (long long)LUCAS_EMAX * LUCAS_THREADS;
+    long long best = std::numeric_limits<long long>::max();
+    Plan bestp;
+    for (int A = 2; A <= pl.M; ++A) {
+        if (pl.M % A) continue;
+        if (forceA && A != forceA) continue;
+        const int B = pl.M / A;
+        if (B < pl.halo) continue;
+        if (2LL * B > cap) continue;
+        std::vector<int> ra = radix_plan(A), rb = radix_plan(B);
+        if (ra.empty() || rb.empty()) continue;
+        if ((int)ra.size() > LUCAS_MAX_RADICES || (int)rb.size() > LUCAS_MAX_RADICES) continue;
+        for (int C : {1, 2, 4, 8, 16, 32, 64}) {
+            if (forceC && C != forceC) continue;
+            if (B % C) continue;
+            if ((long long)A * (C + pl.halo) > cap) continue;
+            const long long colw = B / C, roww = A / 2 + 1;
+            long long col_t = stage_cost(ra, (long long)A * (C + pl.halo)) + stage_cost(ra, (long long)A * C) + 6;
+            long long row_t = 2 * stage_cost(rb, 2LL * B) + 6;
+            // waves of workgroups beyond ~2 per CU serialise
+            col_t *= (colw + 511) / 512;
+            row_t *= (roww + 511) / 512;
+            // redundant halo work and total work both cost bandwidth
+            long long total = (long long)A * (2 * C + pl.halo) * colw / 256 + 4LL * B * roww / 256;
+            long long cost = 64 * (col_t + row_t) + total / 64;
+            if (cost < best) {
+                best = cost;
+                bestp = pl;
+                bestp.A = A;
+                bestp.B = B;
+                bestp.C = C;
+                bestp.radA = ra;
+                bestp.radB = rb;
+                bestp.col_wgs = (int)colw;
+                bestp.row_wgs = (int)roww;
+            }
+        }
+    }
+    if (bestp.A == 0) return "no admissible (A, B, C) tiling for this N";
+    bestp.lds_col = sizeof(double) * 2 * (size_t)bestp.A * (bestp.C + bestp.halo);
+    bestp.lds_row = sizeof(double) * 2 * 2 * (size_t)bestp.B;
+    *out = bestp;
+    return "";
+}
+
+double max_bits_per_word(int N) {
+    double bits = 19.3 - 0.35 * log2((double)N / 65536.0);
+    return std::min(20.2, std::max(16.0, bits));
+}
+
+int default_length(int p) {
+    if (p < 7) return 0;
+    for (long long N = 4; N <= (long long)p; N += 2) {
+        if (!valid_length(N)) continue;
+        if ((double)p / (double)N > max_bits_per_word((int)N)) continue;
+        Plan pl;
+        if (make_plan(p, (int)N, 0, 0, &pl).empty()) return (int)N;
+    }
+    return 0;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+size_t job_bytes(const Plan& pl) {
+    size_t s = 0;
+    s += align256(sizeof(double) * 2 * (size_t)pl.M) * 2;  // spec, spec2
+    s += align256(sizeof(int) * (size_t)pl.N);             // digits
+    s += align256(sizeof(double) * 2 * (size_t)pl.A) * 2;  // twA, twMA
+    s += align256(sizeof(double) * 2 * (size_t)pl.B) * 2;  // twB, twMB
+    s += align256(sizeof(int) * 2 * (size_t)pl.B) + align256(sizeof(int) * (size_t)pl.A);
+    s += align256(sizeof(double) * 2 * (size_t)pl.B) * 2;  // wlo, iwlo
+    s += align256(sizeof(double) * (size_t)pl.A) * 2;      // whi, iwhi
+    s += 256;                                               // maxerr
+    return s;
+}
+
+static void root_table(long long count, long long denom, std::vector<double>* out) {
+    out->resize(2 * (size_t)count);
+    for (long long t = 0; t < count; ++t) {
+        long double ang = 2.0L * kPi * (long double)t / (long double)denom;
+        (*out)[2 * t] = (double)cosl(ang);
+        (*out)[2 * t + 1] = (double)(-sinl(ang));
+    }
+}
+
+void make_tables(const Plan& pl, Tables* t) {
+    root_table(pl.A, pl.A, &t->twA);
+    root_table(pl.B, pl.B, &t->twB);
+    root_table(pl.A, pl.M, &t->twMA);
+    root_table(pl.B, pl.M, &t->twMB);
+    const long long N = pl.N, p = pl.p;
+    t->rlo.resize(2 * (size_t)pl.B);
+    t->wlo.resize(2 * (size_t)pl.B);
+    t->iwlo.resize(2 * (size_t)pl.B);
+    for (long long J = 0; J < 2LL * pl.B; ++J) {
+        long long r = r_of(J, p, N);
+        t->rlo[J] = (int)r;
+        t->wlo[J] = (double)exp2l((long double)r / (long double)N);
+        t->iwlo[J] = (double)exp2l(-(long double)r / (long double)N);
+    }
+    t->rhi.resize(pl.A);
+    t->whi.resize(pl.A);
+    t->iwhi.resize(pl.A);
+    for (long long r = 0; r < pl.A; ++r) {
+        long long rr = r_of(2LL * pl.B * r, p, N);
+        t->rhi[r] = (int)rr;
+        t->whi[r] = (double)exp2l((long double)rr / (long double)N);
+        // inverse transform is unnormalised and the square step is 4x scaled
+        t->iwhi[r] = (double)(exp2l(-(long double)rr / (long double)N) / (4.0L * (long double)pl.M));
+    }
+}
+
+void digit_widths(int p, int N, int* widths) {
+    const int q = p / N, b = p % N;
+    for (long long j = 0; j < N; ++j) widths[j] = q + (r_of(j, p, N) < b ? 1 : 0);
+}
+
+void normalize_digits(int p, int N, const int32_t* digits, int32_t* out, uint64_t* res64, int* is_zero) {
+    std::vector<int> w(N);
+    digit_widths(p, N, w.data());
+    std::vector<long long> d(digits, digits + N);
+    long long carry = 0;
+    for (int pass = 0; pass < 64; ++pass) {
+        for (int j = 0; j < N; ++j) {
+            long long t = d[j] + carry;
+            long long v = t & ((1LL << w[j]) - 1);
+            carry = (t - v) >> w[j];
+            d[j] = v;
+        }
+        if (carry == 0) break;  // otherwise 2^p == 1: wrap into digit 0
+    }
+    bool allzero = true, allones = true;
+    for (int j = 0; j < N; ++j) {
+        if (d[j] != 0) allzero = false;
+        if (d[j] != (1LL << w[j]) - 1) allones = false;
+    }
+    const bool zero = allzero || allones;
+    uint64_t r = 0;
+    if (!zero) {
+        int off = 0;
+        for (int j = 0; j < N && off < 64; ++j) {
+            r |= (uint64_t)d[j] << off;
+            off += w[j];
+        }
+    }
+    if (out) {
+        for (int j = 0; j < N; ++j) out[j] = zero ? 0 : (int32_t)d[j];
+    }
+    if (res64) *res64 = r;
+    if (is_zero) *is_zero = zero ? 1 : 0;
+}
+
+}  // namespace lucas

@@ -1,0 +1,62 @@
+// lucas_host.h -- host-side planning, extended-precision tables and residue
+// arithmetic of the engine (no HIP calls; unit-testable without a GPU).
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+namespace lucas {
+
+struct Plan {
+    int p = 0, N = 0, M = 0;
+    int A = 0, B = 0, C = 0;
+    int q = 0, b = 0;       // q = p / N, b = p % N
+    int ncarry = 0;         // reference carry depth n
+    int halo = 0;           // ceil((n+1)/2)
+    int col_wgs = 0, row_wgs = 0;
+    std::vector<int> radA, radB;
+    size_t lds_col = 0, lds_row = 0;  // bytes of LDS per workgroup
+};
+
+// N = 2^a 3^b 5^c 7^d and even.
+bool valid_length(long long N);
+// Reference carry depth rule (setSliceAndDice, gpuLucas.cu:323-343); -1 if q >= 21.
+int carry_depth(int q);
+// Radix sequence for a Stockham FFT of length L (radices 8, 4, 2, 3, 5, 7).
+std::vector<int> radix_plan(int L);
+// Full plan for (p, N) with optional forced A / C (0 = choose).  Returns an
+// empty string on success or the reason for refusal.
+std::string make_plan(int p, int N, int A, int C, Plan* out);
+// Default N for p: smallest admissible length under the bits-per-word rule.
+int default_length(int p);
+// Bits per word allowed at length N by the default rule.
+double max_bits_per_word(int N);
+
+// Device footprint of one job under `plan` (state + tables), bytes.
+size_t job_bytes(const Plan& plan);
+
+// Tables (host copies).
+struct Tables {
+    std::vector<double> twA, twB, twMA, twMB;  // interleaved re,im
+    std::vector<int> rlo, rhi;
+    std::vector<double> wlo, whi, iwlo, iwhi;
+};
+void make_tables(const Plan& plan, Tables* t);
+
+// r_j = (-j p) mod N.
+inline long long r_of(long long j, long long p, long long N) {
+    long long r = (-(j % N) * (p % N)) % N;
+    return r < 0 ? r + N : r;
+}
+// Reference digit widths and weights for comparison (computeBitsPerWord,
+// computeWeightVectors), computed exactly / in long double.
+void digit_widths(int p, int N, int* widths);
+
+// Normalise a pseudo-balanced digit vector mod 2^p - 1 into digits in
+// [0, 2^w_j); returns Res64 and whether the residue is 0.
+void normalize_digits(int p, int N, const int32_t* digits, int32_t* out, uint64_t* res64, int* is_zero);
+
+}  // namespace lucas

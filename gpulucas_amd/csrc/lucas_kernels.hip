@@ -1,0 +1,490 @@
+// lucas_kernels.hip -- CDNA4 (gfx950) kernels for one Lucas-Lehmer squaring
+// s <- s^2 - 2 mod 2^p - 1 by Crandall's balanced-digit IBDWT, as computed by
+// the reference's mersenneIter (gpuLucas.cu:845-942).  Two kernels make one
+// iteration, each one HBM round trip of the 8N-byte state:
+//
+//   lucas_row : forward row FFT (length B) + real-signal untangle, pointwise
+//               square, re-tangle (replaces ComplexPointwiseSqr,
+//               gpuLucas.cu:647-665, and the cuFFT D2Z/Z2D pair at
+//               gpuLucas.cu:869/877) + inverse row FFT.
+//   lucas_col : inverse column FFT (length A) with its outer twiddles,
+//               unweight, -2, round (invDWTproductMinus2, gpuLucas.cu:774-805),
+//               carry-save radix restoration (llintToIrrBal<n>,
+//               IrrBaseBalanced.cu:52-110), single ripple carry and weighting
+//               (loadIntToDoubleIBDWT, gpuLucas.cu:730-766), forward column FFT
+//               with outer twiddles.
+//
+// The complex FFT of length M = N/2 is a two-pass (A x B) Stockham
+// decomposition; each pass runs from LDS with one workgroup per tile.  The
+// restoration needs the products of the n+1 preceding digits, which live in
+// the H = ceil((n+1)/2) columns to the left of a column tile: each column
+// workgroup recomputes those halo columns instead of exchanging them.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lucas_desc.h"
+
+namespace {
+
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
+}
+// a * conj(b)
+__device__ __forceinline__ double2 cmulc(double2 a, double2 b) {
+    return make_double2(fma(a.x, b.x, a.y * b.y), fma(a.y, b.x, -a.x * b.y));
+}
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+// multiply by SIGN*i
+template <int SIGN>
+__device__ __forceinline__ double2 mul_si(double2 a) {
+    return SIGN < 0 ? make_double2(a.y, -a.x) : make_double2(-a.y, a.x);
+}
+
+// ---------------------------------------------------------------------------
+// Small DFTs, y_s = sum_r v_r exp(SIGN*2*pi*i*r*s/R), in place.
+// ---------------------------------------------------------------------------
+template <int R, int SIGN> struct Dft;
+
+template <int SIGN> struct Dft<2, SIGN> {
+    __device__ __forceinline__ static void run(double2* v) {
+        double2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    }
+};
+
+template <int SIGN> struct Dft<3, SIGN> {
+    __device__ __forceinline__ static void run(double2* v) {
+        const double S3 = 0.8660254037844386;  // sqrt(3)/2
+        double2 t = cadd(v[1], v[2]);
+        double2 u = cscale(mul_si<SIGN>(csub(v[1], v[2])), S3);
+        double2 m = make_double2(v[0].x - 0.5 * t.x, v[0].y - 0.5 * t.y);
+        v[0] = cadd(v[0], t);
+        v[1] = cadd(m, u);
+        v[2] = csub(m, u);
+    }
+};
+
+template <int SIGN> struct Dft<4, SIGN> {
+    __device__ __forceinline__ static void run(double2* v) {
+        double2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+        double2 s13 = cadd(v[1], v[3]), d13 = mul_si<SIGN>(csub(v[1], v[3]));
+        v[0] = cadd(s02, s13);
+        v[2] = csub(s02, s13);
+        v[1] = cadd(d02, d13);
+        v[3] = csub(d02, d13);
+    }
+};
+
+template <int SIGN> struct Dft<5, SIGN> {
+    __device__ __forceinline__ static void run(double2* v) {
+        const double C1 = 0.30901699437494745, C2 = -0.8090169943749475;
+        const double S1 = 0.9510565162951535, S2 = 0.5877852522924731;
+        double2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+        double2 t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+        double2 a0 = v[0];
+        double2 b1 = make_double2(a0.x + C1 * t1.x + C2 * t2.x, a0.y + C1 * t1.y + C2 * t2.y);
+        double2 b2 = make_double2(a0.x + C2 * t1.x + C1 * t2.x, a0.y + C2 * t1.y + C1 * t2.y);
+        double2 d1 = mul_si<SIGN>(make_double2(S1 * t3.x + S2 * t4.x, S1 * t3.y + S2 * t4.y));
+        double2 d2 = mul_si<SIGN>(make_double2(S2 * t3.x - S1 * t4.x, S2 * t3.y - S1 * t4.y));
+        v[0] = cadd(a0, cadd(t1, t2));
+        v[1] = cadd(b1, d1);
+        v[4] = csub(b1, d1);
+        v[2] = cadd(b2, d2);
+        v[3] = csub(b2, d2);
+    }
+};
+
+template <int SIGN> struct Dft<7, SIGN> {
+    __device__ __forceinline__ static void run(double2* v) {
+        const double C1 = 0.6234898018587335, C2 = -0.2225209339563144, C3 = -0.9009688679024191;
+        const double S1 = 0.7818314824680298, S2 = 0.9749279121818236, S3 = 0.4338837391175581;
+        double2 t1 = cadd(v[1], v[6]), t2 = cadd(v[2], v[5]), t3 = cadd(v[3], v[4]);
+        double2 u1 = csub(v[1], v[6]), u2 = csub(v[2], v[5]), u3 = csub(v[3], v[4]);
+        double2 a0 = v[0];
+        double2 b1 = make_double2(a0.x + C1 * t1.x + C2 * t2.x + C3 * t3.x, a0.y + C1 * t1.y + C2 * t2.y + C3 * t3.y);
+        double2 b2 = make_double2(a0.x + C2 * t1.x + C3 * t2.x + C1 * t3.x, a0.y + C2 * t1.y + C3 * t2.y + C1 * t3.y);
+        double2 b3 = make_double2(a0.x + C3 * t1.x + C1 * t2.x + C2 * t3.x, a0.y + C3 * t1.y + C1 * t2.y + C2 * t3.y);
+        double2 d1 = mul_si<SIGN>(make_double2(S1 * u1.x + S2 * u2.x + S3 * u3.x, S1 * u1.y + S2 * u2.y + S3 * u3.y));
+        double2 d2 = mul_si<SIGN>(make_double2(S2 * u1.x - S3 * u2.x - S1 * u3.x, S2 * u1.y - S3 * u2.y - S1 * u3.y));
+        double2 d3 = mul_si<SIGN>(make_double2(S3 * u1.x - S1 * u2.x + S2 * u3.x, S3 * u1.y - S1 * u2.y + S2 * u3.y));
+        v[0] = cadd(a0, cadd(t1, cadd(t2, t3)));
+        v[1] = cadd(b1, d1);
+        v[6] = csub(b1, d1);
+        v[2] = cadd(b2, d2);
+        v[5] = csub(b2, d2);
+        v[3] = cadd(b3, d3);
+        v[4] = csub(b3, d3);
+    }
+};
+
+template <int SIGN> struct Dft<8, SIGN> {
+    __device__ __forceinline__ static void run(double2* v) {
+        const double R2 = 0.7071067811865476;
+        double2 e[4] = {v[0], v[2], v[4], v[6]};
+        double2 o[4] = {v[1], v[3], v[5], v[7]};
+        Dft<4, SIGN>::run(e);
+        Dft<4, SIGN>::run(o);
+        // twiddles exp(SIGN*2*pi*i*s/8), s = 1..3
+        double2 o1 = cmul(o[1], make_double2(R2, SIGN * R2));
+        double2 o2 = mul_si<SIGN>(o[2]);
+        double2 o3 = cmul(o[3], make_double2(-R2, SIGN * R2));
+        v[0] = cadd(e[0], o[0]);
+        v[4] = csub(e[0], o[0]);
+        v[1] = cadd(e[1], o1);
+        v[5] = csub(e[1], o1);
+        v[2] = cadd(e[2], o2);
+        v[6] = csub(e[2], o2);
+        v[3] = cadd(e[3], o3);
+        v[7] = csub(e[3], o3);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// One Stockham autosort stage over `nseq` sequences of length L held in LDS.
+// Element (idx, seq) lives at lds[seq*sSeq + idx*sIdx].  Stage with radix R
+// after Ns = product of the previous radices:
+//   for j < L/R, k = j mod Ns:
+//     v_r = x[j + r*L/R] * w_L^(r*k*L/(Ns*R))
+//     y[(j/Ns)*Ns*R + k + s*Ns] = DFT_R(v)_s
+// Reads of one stage all happen before its writes (registers, then barrier),
+// so one LDS buffer serves all stages.  SEQ_FAST maps consecutive threads to
+// consecutive sequences (column tiles) instead of consecutive butterflies.
+// ---------------------------------------------------------------------------
+template <int R, int SIGN, bool SEQ_FAST>
+__device__ __forceinline__ void fft_stage(double2* lds, int L, int Ns, int nseq, int sIdx, int sSeq,
+                                          const double2* __restrict__ tw, int tid) {
+    constexpr int BM = (LUCAS_EMAX + R - 1) / R;
+    const int nb = L / R;
+    const int total = nb * nseq;
+    const int twstep = L / (Ns * R);
+    double2 v[BM][R];
+    int outb[BM];
+#pragma unroll
+    for (int i = 0; i < BM; ++i) {
+        const int t = tid + i * LUCAS_THREADS;
+        if (t < total) {
+            int seq, j;
+            if (SEQ_FAST) {
+                seq = t % nseq;
+                j = t / nseq;
+            } else {
+                j = t % nb;
+                seq = t / nb;
+            }
+            const int k = j % Ns;
+            const int base = seq * sSeq;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                double2 x = lds[base + (j + r * nb) * sIdx];
+                if (r > 0 && Ns > 1) {
+                    const double2 w = tw[r * k * twstep];
+                    x = SIGN < 0 ? cmul(x, w) : cmulc(x, w);
+                }
+                v[i][r] = x;
+            }
+            Dft<R, SIGN>::run(v[i]);
+            outb[i] = base + ((j / Ns) * Ns * R + k) * sIdx;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < BM; ++i) {
+        const int t = tid + i * LUCAS_THREADS;
+        if (t < total) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) lds[outb[i] + r * Ns * sIdx] = v[i][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <int SIGN, bool SEQ_FAST>
+__device__ void fft_lds(double2* lds, int L, const int* rad, int nrad, int nseq, int sIdx, int sSeq,
+                        const double2* __restrict__ tw, int tid) {
+    int Ns = 1;
+    for (int s = 0; s < nrad; ++s) {
+        const int R = rad[s];
+        switch (R) {
+            case 2: fft_stage<2, SIGN, SEQ_FAST>(lds, L, Ns, nseq, sIdx, sSeq, tw, tid); break;
+            case 3: fft_stage<3, SIGN, SEQ_FAST>(lds, L, Ns, nseq, sIdx, sSeq, tw, tid); break;
+            case 4: fft_stage<4, SIGN, SEQ_FAST>(lds, L, Ns, nseq, sIdx, sSeq, tw, tid); break;
+            case 5: fft_stage<5, SIGN, SEQ_FAST>(lds, L, Ns, nseq, sIdx, sSeq, tw, tid); break;
+            case 7: fft_stage<7, SIGN, SEQ_FAST>(lds, L, Ns, nseq, sIdx, sSeq, tw, tid); break;
+            default: fft_stage<8, SIGN, SEQ_FAST>(lds, L, Ns, nseq, sIdx, sSeq, tw, tid); break;
+        }
+        Ns *= R;
+    }
+}
+
+// omega_M^e for 0 <= e < M, as omega_A^(e / B) * omega_M^(e mod B).
+__device__ __forceinline__ double2 outer_twiddle(const LucasJobDesc& J, int e) {
+    const int hi = e / J.B;
+    const int lo = e - hi * J.B;
+    return cmul(J.twA[hi], J.twMB[lo]);
+}
+
+__device__ __forceinline__ void block_max_err(float err, unsigned int* dst) {
+    // wave64 max reduction, then one atomic per wave.
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) err = fmaxf(err, __shfl_xor(err, off, 64));
+    if ((threadIdx.x & 63) == 0 && err > 0.0f) atomicMax(dst, __float_as_uint(err));
+}
+
+// Carry-save restoration of one digit (reference llintToIrrBal<n>,
+// IrrBaseBalanced.cu:52-110).  cv[0..] are the rounded products of digits
+// D-n .. D (cv[n] is digit D), w[] their widths.  Returns the balanced digit
+// and writes the carry into *carry.
+__device__ __forceinline__ int restore_digit(const long long* cv, const int* w, int n, int* carry) {
+    const int W = w[n];
+    const long long mask = (1LL << W) - 1;
+    int sum = 0;
+    int shift = 0;
+#pragma unroll
+    for (int i = 0; i <= LUCAS_NMAX; ++i) {
+        if (i <= n) {
+            const long long c = cv[n - i];
+            if (i > 0) shift += w[n - i];
+            const unsigned long long mag = c < 0 ? (unsigned long long)(-c) : (unsigned long long)c;
+            const int part = shift < 64 ? (int)((mag >> shift) & (unsigned long long)mask) : 0;
+            sum += c < 0 ? -part : part;
+        }
+    }
+    const int half = 1 << (W - 1);
+    int h = 0;
+    if (sum < -half)
+        h = -((-sum + half) >> W);
+    else if (sum >= half)
+        h = (sum + half) >> W;
+    *carry = h;
+    return sum - h * (1 << W);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Column kernel: one workgroup per C-column tile of one job.
+// ---------------------------------------------------------------------------
+extern "C" __global__ void __launch_bounds__(LUCAS_THREADS)
+lucas_col_kernel(const LucasJobDesc* __restrict__ jobs, const int2* __restrict__ wgmap, int mode) {
+    extern __shared__ double2 lds[];
+    const int2 jm = wgmap[blockIdx.x];
+    const LucasJobDesc& J = jobs[jm.x];
+    const int A = J.A, B = J.B, C = J.C, H = J.halo, N = J.N;
+    const int n = J.ncarry, q = J.q, bb = J.b;
+    const int c0 = jm.y * C;
+    const int tid = threadIdx.x;
+    const int ncl = C + H;  // local columns: [0,H) halo, [H, H+C) own
+    constexpr int EPT = LUCAS_EMAX;
+    int xe[EPT], xo[EPT];   // new digits 2m, 2m+1 of the thread's own elements
+    float err = 0.0f;
+
+    if (!(mode & LUCAS_IN_DIGITS)) {
+        // 1. load the tile (+ halo) with the inverse outer twiddle omega_M^(-c*r)
+        for (int t = tid; t < A * ncl; t += LUCAS_THREADS) {
+            const int r = t / ncl, lc = t - r * ncl;
+            int gc = c0 - H + lc;
+            if (gc < 0) gc += B;
+            const double2 v = J.spec2[(size_t)r * B + gc];
+            lds[t] = cmulc(v, outer_twiddle(J, gc * r));
+        }
+        __syncthreads();
+        // 2. inverse column FFTs
+        fft_lds<+1, true>(lds, A, J.radA, J.nradA, ncl, ncl, 1, J.twA, tid);
+        // 3. unweight, -2 on digit 0, round half-even, roundoff
+        for (int t = tid; t < A * ncl; t += LUCAS_THREADS) {
+            const int r = t / ncl, lc = t - r * ncl;
+            int gc = c0 - H + lc;
+            if (gc < 0) gc += B;
+            const double2 v = lds[t];
+            const int rh = J.rhi[r];
+            const double iwh = J.iwhi[r];
+            const int je = 2 * gc, jo = je + 1;
+            const double ie = J.iwlo[je] * iwh * (J.rlo[je] + rh >= N ? 2.0 : 1.0);
+            const double io = J.iwlo[jo] * iwh * (J.rlo[jo] + rh >= N ? 2.0 : 1.0);
+            double ve = v.x * ie;
+            const double vo = v.y * io;
+            if (gc == 0 && r == 0) ve -= 2.0;
+            const double re = rint(ve), ro = rint(vo);
+            err = fmaxf(err, (float)fmax(fabs(ve - re), fabs(vo - ro)));
+            lds[t] = make_double2(re, ro);
+        }
+        __syncthreads();
+        // 4. carry-save restoration + single ripple for the own columns
+#pragma unroll
+        for (int i = 0; i < EPT; ++i) {
+            const int t = tid + i * LUCAS_THREADS;
+            if (t < A * C) {
+                const int r = t / C, lo = t - r * C;
+                const int lc = lo + H, gc = c0 + lo;
+                // products and widths of digits 2m+d, d = -1-n .. 1  (index d+1+n)
+                long long cv[LUCAS_NMAX + 3];
+                int w[LUCAS_NMAX + 3];
+                int r2m = J.rlo[2 * gc] + J.rhi[r];
+                if (r2m >= N) r2m -= N;
+#pragma unroll
+                for (int k = 0; k < LUCAS_NMAX + 3; ++k) {
+                    if (k <= n + 2) {
+                        const int d = k - 1 - n;
+                        const int ie = (1 - d) >> 1;  // element offset back
+                        int rr = r;
+                        if (gc - ie < 0) rr = (r == 0) ? A - 1 : r - 1;
+                        const double2 v = lds[rr * ncl + (lc - ie)];
+                        cv[k] = (long long)((d & 1) ? v.y : v.x);
+                        // r_{2m+d} = (r_2m - d*b) mod N
+                        long long rd = (long long)r2m - (long long)d * bb;
+                        rd %= N;
+                        if (rd < 0) rd += N;
+                        w[k] = q + (rd < bb ? 1 : 0);
+                    }
+                }
+                int hm1, h0, h1;
+                (void)restore_digit(cv, w, n, &hm1);                // digit 2m-1: carry only
+                const int d0 = restore_digit(cv + 1, w + 1, n, &h0);  // digit 2m
+                const int d1 = restore_digit(cv + 2, w + 2, n, &h1);  // digit 2m+1
+                (void)h1;
+                xe[i] = d0 + hm1;
+                xo[i] = d1 + h0;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < EPT; ++i) {
+            const int t = tid + i * LUCAS_THREADS;
+            if (t < A * C) {
+                const int r = t / C, lo = t - r * C;
+                const size_t m = (size_t)(c0 + lo) + (size_t)B * r;
+                xe[i] = J.digits[2 * m];
+                xo[i] = J.digits[2 * m + 1];
+            }
+        }
+    }
+
+    if (mode & LUCAS_OUT_DIGITS) {
+#pragma unroll
+        for (int i = 0; i < EPT; ++i) {
+            const int t = tid + i * LUCAS_THREADS;
+            if (t < A * C) {
+                const int r = t / C, lo = t - r * C;
+                const size_t m = (size_t)(c0 + lo) + (size_t)B * r;
+                J.digits[2 * m] = xe[i];
+                J.digits[2 * m + 1] = xo[i];
+            }
+        }
+        block_max_err(err, J.maxerr);
+        return;
+    }
+
+    __syncthreads();  // every carry read of the rounded tile is done
+    // 5. weight: y_j = 2^(r_j/N) * x_j, into a compact C-column tile
+#pragma unroll
+    for (int i = 0; i < EPT; ++i) {
+        const int t = tid + i * LUCAS_THREADS;
+        if (t < A * C) {
+            const int r = t / C, lo = t - r * C;
+            const int gc = c0 + lo;
+            const int rh = J.rhi[r];
+            const double wh = J.whi[r];
+            const int je = 2 * gc, jo = je + 1;
+            const double ae = J.wlo[je] * wh * (J.rlo[je] + rh >= N ? 0.5 : 1.0);
+            const double ao = J.wlo[jo] * wh * (J.rlo[jo] + rh >= N ? 0.5 : 1.0);
+            lds[t] = make_double2(ae * (double)xe[i], ao * (double)xo[i]);
+        }
+    }
+    __syncthreads();
+    // 6. forward column FFTs
+    fft_lds<-1, true>(lds, A, J.radA, J.nradA, C, C, 1, J.twA, tid);
+    // 7. outer twiddle omega_M^(c*k) and store row-major
+    for (int t = tid; t < A * C; t += LUCAS_THREADS) {
+        const int k = t / C, lo = t - k * C;
+        const int gc = c0 + lo;
+        J.spec[(size_t)k * B + gc] = cmul(lds[t], outer_twiddle(J, gc * k));
+    }
+    block_max_err(err, J.maxerr);
+}
+
+// ---------------------------------------------------------------------------
+// Row kernel: one workgroup per row pair (k, A-k) of one job.  Rows 0 and A/2
+// pair with themselves.  Frequency K = k + A*kc of the half-length complex
+// transform Z pairs with M-K; the real-signal square is formed per pair as
+//   E = Z_K + conj(Z_{M-K}),  O = -i (Z_K - conj(Z_{M-K}))   (2x scaled)
+//   Z'_K     = E^2 + w O^2 + 2i E O,          w = omega_M^K
+//   Z'_{M-K} = conj(E^2 + w O^2) + 2i conj(E O)
+// which is 4x the packed spectrum of the squared real signal; the 1/4 is
+// folded into the unweight table.
+// ---------------------------------------------------------------------------
+extern "C" __global__ void __launch_bounds__(LUCAS_THREADS)
+lucas_row_kernel(const LucasJobDesc* __restrict__ jobs, const int2* __restrict__ wgmap) {
+    extern __shared__ double2 lds[];
+    const int2 jm = wgmap[blockIdx.x];
+    const LucasJobDesc& J = jobs[jm.x];
+    const int A = J.A, B = J.B;
+    const int tid = threadIdx.x;
+    const int r0 = jm.y;
+    const bool self = (r0 == 0) || (2 * r0 == A);
+    const int r1 = self ? r0 : A - r0;
+    const int nrows = self ? 1 : 2;
+
+    for (int t = tid; t < nrows * B; t += LUCAS_THREADS) {
+        const int rl = t / B, kc = t - rl * B;
+        lds[t] = J.spec[(size_t)(rl ? r1 : r0) * B + kc];
+    }
+    __syncthreads();
+    fft_lds<-1, false>(lds, B, J.radB, J.nradB, nrows, 1, B, J.twB, tid);
+
+    const double2 wr = J.twMA[r0];
+    if (!self) {
+        for (int kc = tid; kc < B; kc += LUCAS_THREADS) {
+            const int pc = B - 1 - kc;
+            const double2 zk = lds[kc], zm = lds[B + pc];
+            const double2 w = cmul(wr, J.twB[kc]);
+            const double2 E = cadd(zk, cconj(zm));
+            const double2 O = mul_si<-1>(csub(zk, cconj(zm)));
+            const double2 P = cadd(cmul(E, E), cmul(w, cmul(O, O)));
+            const double2 Q = cmul(E, O);
+            lds[kc] = cadd(P, make_double2(-2.0 * Q.y, 2.0 * Q.x));
+            lds[B + pc] = make_double2(P.x + 2.0 * Q.y, -P.y + 2.0 * Q.x);
+        }
+    } else {
+        const int last = (r0 == 0) ? B / 2 : (B - 1) / 2;
+        for (int kc = tid; kc <= last; kc += LUCAS_THREADS) {
+            int pc = (r0 == 0) ? B - kc : B - 1 - kc;
+            if (pc == B) pc = 0;
+            const double2 zk = lds[kc], zm = lds[pc];
+            const double2 w = cmul(wr, J.twB[kc]);
+            const double2 E = cadd(zk, cconj(zm));
+            const double2 O = mul_si<-1>(csub(zk, cconj(zm)));
+            const double2 P = cadd(cmul(E, E), cmul(w, cmul(O, O)));
+            const double2 Q = cmul(E, O);
+            lds[kc] = cadd(P, make_double2(-2.0 * Q.y, 2.0 * Q.x));
+            if (pc != kc) lds[pc] = make_double2(P.x + 2.0 * Q.y, -P.y + 2.0 * Q.x);
+        }
+    }
+    __syncthreads();
+    fft_lds<+1, false>(lds, B, J.radB, J.nradB, nrows, 1, B, J.twB, tid);
+    for (int t = tid; t < nrows * B; t += LUCAS_THREADS) {
+        const int rl = t / B, kc = t - rl * B;
+        J.spec2[(size_t)(rl ? r1 : r0) * B + kc] = lds[t];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launchers (the engine's only entry into device code).
+// ---------------------------------------------------------------------------
+namespace lucas {
+
+hipError_t launch_col_kernel(const LucasJobDesc* jobs, const int2* map, int nwg, size_t lds, int mode,
+                             hipStream_t stream) {
+    hipLaunchKernelGGL(lucas_col_kernel, dim3(nwg), dim3(LUCAS_THREADS), lds, stream, jobs, map, mode);
+    return hipGetLastError();
+}
+
+hipError_t launch_row_kernel(const LucasJobDesc* jobs, const int2* map, int nwg, size_t lds, hipStream_t stream) {
+    hipLaunchKernelGGL(lucas_row_kernel, dim3(nwg), dim3(LUCAS_THREADS), lds, stream, jobs, map);
+    return hipGetLastError();
+}
+
+}  // namespace lucas

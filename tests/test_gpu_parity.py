@@ -1,0 +1,154 @@
+"""GPU parity: the HIP engine (through the C ABI) against the CPU oracles.
+
+* digit vectors after k iterations are bit-identical to the oracle's
+  restatement of the reference (same carry-save restoration, same ripple);
+* Res64 equals GMP at the iterations the workloads name; Mersenne-prime
+  exponents end at residue 0;
+* max roundoff stays below 0.4 on every case.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import fixture_res64
+from gpulucas_amd import lucas
+
+pytestmark = pytest.mark.gpu
+
+ROUNDOFF_LIMIT = 0.4
+
+
+def random_balanced(p, N, seed):
+    rng = np.random.default_rng(seed)
+    w, _, _ = lucas.digit_tables(p, N)
+    half = (1 << (w - 1)).astype(np.int64)
+    return (rng.integers(0, 2 * half) - half).astype(np.int32)
+
+
+# (p, N, A, C): radix 2/3/5/7 mixes, odd A, C=1..16, halo wider than C
+SHAPES = [
+    (1279, 64, 0, 0), (1279, 64, 4, 1), (1279, 64, 8, 4),
+    (2203, 120, 0, 0), (4423, 240, 0, 0), (4423, 240, 15, 8),
+    (9941, 560, 0, 0), (9941, 560, 7, 4), (19937, 1080, 0, 0),
+    (21701, 1176, 0, 0), (44497, 2400, 0, 0), (86243, 4608, 0, 0),
+    (86243, 4608, 48, 16), (86243, 4608, 72, 8), (110503, 6048, 0, 0),
+]
+
+
+@pytest.mark.parametrize("p,N,A,C", SHAPES)
+def test_digits_bit_exact_vs_oracle_from_s0(p, N, A, C):
+    print(f"case p={p} N={N} A={A} C={C}", flush=True)
+    with lucas.Context(p, N, A=A, C_=C) as ctx:
+        x = oracle.s0_digits(N)
+        assert np.array_equal(ctx.digits(), x)
+        done = 0
+        for k in (1, 2, 3, 7, 12):
+            ctx.step(k - done)
+            x, _ = oracle.ibdwt_iterate(p, N, x, k - done)
+            done = k
+            assert np.array_equal(ctx.digits(), x), f"digits differ after {k} iterations"
+        assert ctx.max_roundoff() < ROUNDOFF_LIMIT
+
+
+@pytest.mark.parametrize("p,N,A,C", SHAPES[::2])
+def test_digits_bit_exact_vs_oracle_steady_state(p, N, A, C):
+    print(f"case p={p} N={N} A={A} C={C}", flush=True)
+    x = random_balanced(p, N, 11)
+    with lucas.Context(p, N, A=A, C_=C) as ctx:
+        ctx.set_digits(x, 0)
+        ctx.step(3)
+        y, _ = oracle.ibdwt_iterate(p, N, x, 3)
+        assert np.array_equal(ctx.digits(), y)
+        assert ctx.max_roundoff() < ROUNDOFF_LIMIT
+
+
+@pytest.mark.parametrize("p,N", [(1279, 64), (2203, 120), (4423, 240), (9941, 560), (19937, 1080),
+                                 (44497, 2400), (86243, 4608), (110503, 6048), (132049, 7168)])
+def test_full_run_mersenne_primes(p, N):
+    print(f"case p={p} N={N}", flush=True)
+    r = lucas.mersenneTest(p, N)
+    assert r["status"] == lucas.OK
+    assert r["is_prime"] == 1 and r["res64"] == 0
+    assert r["iterations"] == p - 2
+    assert r["max_roundoff"] < ROUNDOFF_LIMIT
+
+
+@pytest.mark.parametrize("p", [1009, 2003, 4001, 9973, 10007, 20011, 30011, 44501, 50021])
+def test_full_run_composites_vs_gmp(p, gmp_fixtures):
+    print(f"case p={p}", flush=True)
+    r = lucas.mersenneTest(p, 0)
+    want, zero = fixture_res64(gmp_fixtures, "small_composite_full", p, p - 2)
+    assert r["res64"] == want and not zero and r["is_prime"] == 0
+    assert r["max_roundoff"] < ROUNDOFF_LIMIT
+
+
+def test_config1_p86243_n4608(gmp_fixtures):
+    p, N = 86243, 4608
+    with lucas.Context(p, N) as ctx:
+        for k in (1000, 10000, 86241):
+            ctx.step(k - ctx.iteration)
+            want, zero = fixture_res64(gmp_fixtures, "c1_p86243", p, k)
+            assert ctx.res64() == (want, zero), k
+        assert ctx.res64() == (0, True)
+        assert ctx.max_roundoff() < ROUNDOFF_LIMIT
+
+
+def test_config2_p1257787_n65536_full(gmp_fixtures):
+    p, N = 1257787, 65536
+    with lucas.Context(p, N) as ctx:
+        ks = [1000, 10000] + [100000 * m for m in range(1, 13)] + [1257785]
+        for k in ks:
+            ctx.step(k - ctx.iteration)
+            want, zero = fixture_res64(gmp_fixtures, "c2_p1257787", p, k)
+            assert ctx.res64() == (want, zero), k
+            print(f"k={k} roundoff={ctx.max_roundoff():.4f}", flush=True)
+        assert ctx.res64() == (0, True)
+        assert ctx.max_roundoff() < ROUNDOFF_LIMIT
+
+
+def test_config3_p2976221_n163840(gmp_fixtures):
+    p, N = 2976221, 163840
+    with lucas.Context(p, N) as ctx:
+        for k in (1000, 5000, 20000, 100000):
+            ctx.step(k - ctx.iteration)
+            want, zero = fixture_res64(gmp_fixtures, "c3_p2976221", p, k)
+            assert ctx.res64() == (want, zero), k
+        assert ctx.max_roundoff() < ROUNDOFF_LIMIT
+
+
+def test_config3_near3m_primes(gmp_fixtures):
+    from gpulucas_amd import workloads as wl
+    for p in wl.near3m_primes():
+        print(f"case p={p}", flush=True)
+        with lucas.Context(p, 163840) as ctx:
+            ctx.step(20000)
+            want, zero = fixture_res64(gmp_fixtures, "c3_near3m", p, 20000)
+            assert ctx.res64() == (want, zero)
+            assert ctx.max_roundoff() < ROUNDOFF_LIMIT
+
+
+def test_error_test_contract():
+    ms = lucas.errorTest(86243, 4608, 200)
+    assert ms > 0
+    # q = 20 bits per word at a short length: roundoff must trip the 0.30 gate
+    bad = lucas.error_test(1310719, 65536, 200)
+    assert bad["status"] == lucas.ERR_ROUNDOFF or bad["max_roundoff"] < 0.30
+    if bad["status"] == lucas.ERR_ROUNDOFF:
+        assert lucas.errorTest(1310719, 65536, 200) == -1.0
+
+
+def test_checkpoint_resume_identical(tmp_path):
+    p, N = 86243, 4608
+    path = str(tmp_path / "m86243.chk")
+    with lucas.Context(p, N) as ctx:
+        ctx.step(3000)
+        ctx.save(path)
+        ctx.step(2000)
+        want = ctx.res64()
+    with lucas.Context.load(path) as ctx2:
+        assert ctx2.iteration == 3000 and ctx2.N == N
+        ctx2.step(2000)
+        assert ctx2.res64() == want
+    # full test with periodic checkpoints then resume from the file
+    r1 = lucas.mersenne_test(p, N, checkpoint_every=20000, checkpoint_path=path)
+    assert r1["is_prime"] == 1

@@ -1,0 +1,127 @@
+"""Host-side logic of libgpulucas (no GPU needed): exports, planning, the
+per-digit tables the kernels regenerate, and residue normalisation."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+from gpulucas_amd import lucas, workloads as wl
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    src = open(os.path.join(ROOT, "include", "gpulucas.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(lucas_[A-Za-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_function():
+    import ctypes
+    lib = ctypes.CDLL(lucas.LIB_PATH)
+    names = header_functions()
+    assert len(names) >= 30
+    for n in names:
+        assert hasattr(lib, n), n
+    # the ctypes mirror binds exactly the declared set
+    assert lucas.exported_symbols() == names
+
+
+def test_version_and_status_strings():
+    assert "gfx950" in lucas.version()
+    assert lucas._lib().lucas_status_string(lucas.ERR_ROUNDOFF) == b"roundoff limit exceeded"
+
+
+@pytest.mark.parametrize("p,N,q,n", [(86243, 4608, 18, 2), (1257787, 65536, 19, 2), (2976221, 163840, 18, 2),
+                                     (859433, 49152, 17, 3), (1279, 64, 19, 2), (4423, 240, 18, 2)])
+def test_plan_config_shapes(p, N, q, n):
+    info = lucas.plan(p, N)
+    assert info["N"] == N and info["q"] == q and info["carry_depth"] == n
+    assert info["A"] * info["B"] == N // 2
+    assert info["n_wide"] == p % N
+    assert info["halo"] == (n + 2) // 2
+    assert info["B"] % info["C"] == 0
+    assert info["col_wgs"] == info["B"] // info["C"]
+    assert info["row_wgs"] == info["A"] // 2 + 1
+    assert info["device_bytes"] >= 20 * N
+
+
+def test_plan_rejects_bad_inputs():
+    for p, N in [(2976221, 163841), (2976221, 11 * 16384), (2976221, 131072), (5, 4), (86243, 0 - 2)]:
+        with pytest.raises(lucas.LucasError):
+            lucas.plan(p, N)
+
+
+@pytest.mark.parametrize("A", [40, 64, 80, 128, 160, 256, 320, 512, 640, 1024])
+def test_plan_forced_shapes(A):
+    info = lucas.plan(2976221, 163840, A=A)
+    assert info["A"] == A and A * info["B"] == 81920
+
+
+@pytest.mark.parametrize("p", [86243, 1257787, 2976221, 4990753, 1062497, 39999983, 110503])
+def test_default_length(p):
+    N = lucas.default_N(p)
+    info = lucas.plan(p, N)
+    assert p / N <= 20.2
+    assert info["q"] <= 20
+    # smallest admissible: the next smaller smooth length exceeds the rule
+    assert N >= p / 20.2
+
+
+@pytest.mark.parametrize("p,N", [(86243, 4608), (2976221, 163840), (1257787, 65536), (4423, 240)])
+def test_digit_tables_match_oracle(p, N):
+    w, a, ai = lucas.digit_tables(p, N)
+    ow, _, oa, oai = oracle.tables(p, N)
+    assert np.array_equal(w, ow)
+    assert np.array_equal(a, oa)
+    assert np.max(np.abs(ai - oai) / oai) < 2.3e-16
+
+
+@pytest.mark.parametrize("p,N", [(1279, 64), (4423, 240), (86243, 4608)])
+def test_digits_res64_matches_gmp(p, N):
+    rng = np.random.default_rng(p)
+    e = oracle.offsets(p, N)
+    w, _, _ = lucas.digit_tables(p, N)
+    for trial in range(5):
+        lim = (1 << (w + 1)).astype(np.int64)
+        d = (rng.integers(0, 2 * lim) - lim).astype(np.int32)
+        r, z, norm = lucas.digits_res64(p, N, d)
+        assert (r, z) == oracle.gmp_digits_res64(p, d, e)
+        assert np.all(norm >= 0) and np.all(norm < (1 << w))
+        assert oracle.gmp_digits_res64(p, norm, e) == (r, z)
+
+
+@pytest.mark.parametrize("p,N", [(1279, 64), (4423, 240)])
+def test_digits_res64_zero_forms(p, N):
+    w, _, _ = lucas.digit_tables(p, N)
+    zero = np.zeros(N, dtype=np.int32)
+    assert lucas.digits_res64(p, N, zero)[:2] == (0, True)
+    ones = ((1 << w) - 1).astype(np.int32)  # = M_p
+    assert lucas.digits_res64(p, N, ones)[:2] == (0, True)
+    neg = zero.copy()
+    neg[0] = -1  # -1 == M_p - 1: res64 all ones in the low word
+    r, z, _ = lucas.digits_res64(p, N, neg)
+    assert not z and r == (2 ** p - 2) & (2 ** 64 - 1)
+
+
+def test_workload_lists_are_deterministic_primes():
+    a = wl.batch_primes()
+    assert a == wl.batch_primes() and len(set(a)) == 64
+    assert all(wl.is_prime(p) and 1_000_000 <= p <= 5_000_000 for p in a)
+    b = wl.near3m_primes()
+    assert all(2_900_000 <= p <= 3_000_000 and wl.is_prime(p) for p in b)
+    jobs = wl.serving_jobs(count=32)
+    assert all(t2 >= t1 for (t1, _, _), (t2, _, _) in zip(jobs, jobs[1:]))
+    assert all(1e5 <= p <= 4.1e7 and 5000 <= k <= 50000 for _, p, k in jobs)
+
+
+def test_miller_rabin_against_sieve():
+    n = 20000
+    sieve = np.ones(n, dtype=bool)
+    sieve[:2] = False
+    for i in range(2, int(n ** 0.5) + 1):
+        if sieve[i]:
+            sieve[i * i::i] = False
+    assert [i for i in range(n) if wl.is_prime(i)] == list(np.nonzero(sieve)[0])
