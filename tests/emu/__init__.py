@@ -1,0 +1,43 @@
+"""CPU emulation of the engine's kernels (tests only)."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+LIB = os.path.join(HERE, "libemu.so")
+SOURCES = [os.path.join(ROOT, "gpulucas_amd", "csrc", "lucas_kernels.hip"),
+           os.path.join(HERE, "emu_driver.cpp"),
+           os.path.join(ROOT, "gpulucas_amd", "csrc", "lucas_host.cpp")]
+_lib = None
+
+
+def build():
+    cmd = ["g++", "-std=c++20", "-O2", "-fPIC", "-shared", "-pthread", "-I" + HERE,
+           "-I" + os.path.join(ROOT, "gpulucas_amd", "csrc"), "-I" + os.path.join(ROOT, "include"),
+           "-x", "c++", SOURCES[0], "-x", "c++", SOURCES[1], SOURCES[2], "-o", LIB]
+    subprocess.run(cmd, check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        newest = max(os.path.getmtime(s) for s in SOURCES + [os.path.join(HERE, "hip", "hip_runtime.h")])
+        if not os.path.exists(LIB) or os.path.getmtime(LIB) < newest:
+            build()
+        _lib = C.CDLL(LIB)
+        _lib.emu_run.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_longlong, C.c_void_p,
+                                 C.POINTER(C.c_double)]
+    return _lib
+
+
+def run(p, N, digits, iters, A=0, C_=0):
+    x = np.ascontiguousarray(digits, dtype=np.int32)
+    out = np.zeros(N, dtype=np.int32)
+    err = C.c_double(0)
+    rc = lib().emu_run(p, N, A, C_, x.ctypes.data_as(C.c_void_p), iters, out.ctypes.data_as(C.c_void_p),
+                       C.byref(err))
+    assert rc == 0, "plan refused"
+    return out, err.value

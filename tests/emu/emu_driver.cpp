@@ -1,0 +1,81 @@
+// CPU emulation driver: runs the engine's column/row kernels (compiled from
+// gpulucas_amd/csrc/lucas_kernels.hip against tests/emu/hip/hip_runtime.h)
+// on host memory for one job.  Tests only.
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "lucas_desc.h"
+#include "lucas_host.h"
+
+thread_local emu_idx threadIdx, blockIdx;
+emu_idx blockDim;
+double2 lds[LUCAS_EMAX * LUCAS_THREADS];
+
+namespace emu {
+std::barrier<>* bar = nullptr;
+std::vector<float> shfl;
+std::mutex atom_mu;
+
+void run_grid(unsigned nblocks, unsigned nthreads, const std::function<void()>& body) {
+    std::barrier<> b(nthreads);
+    bar = &b;
+    shfl.assign(nthreads, 0.f);
+    blockDim = {nthreads, 1, 1};
+    std::vector<std::thread> ts;
+    for (unsigned t = 0; t < nthreads; ++t)
+        ts.emplace_back([&, t] {
+            threadIdx = {t, 0, 0};
+            for (unsigned blk = 0; blk < nblocks; ++blk) {
+                blockIdx = {blk, 0, 0};
+                body();
+                bar->arrive_and_wait();  // the workgroup's LDS is reused by the next one
+            }
+        });
+    for (auto& th : ts) th.join();
+    bar = nullptr;
+}
+}  // namespace emu
+
+namespace lucas {
+hipError_t launch_col_kernel(const LucasJobDesc*, const int2*, int, size_t, int, hipStream_t);
+hipError_t launch_row_kernel(const LucasJobDesc*, const int2*, int, size_t, hipStream_t);
+}
+
+extern "C" int emu_run(int p, int N, int A, int C, const int32_t* digits_in, long long iters, int32_t* digits_out,
+                       double* maxerr) {
+    using namespace lucas;
+    Plan pl;
+    if (!make_plan(p, N, A, C, &pl).empty()) return -1;
+    Tables t;
+    make_tables(pl, &t);
+    std::vector<double2> spec(pl.M), spec2(pl.M);
+    std::vector<int> dig(digits_in, digits_in + N);
+    unsigned err = 0;
+    LucasJobDesc d{};
+    d.p = pl.p; d.N = pl.N; d.M = pl.M; d.A = pl.A; d.B = pl.B; d.q = pl.q; d.b = pl.b;
+    d.ncarry = pl.ncarry; d.halo = pl.halo; d.C = pl.C; d.col_wgs = pl.col_wgs; d.row_wgs = pl.row_wgs;
+    d.nradA = (int)pl.radA.size(); d.nradB = (int)pl.radB.size();
+    for (int i = 0; i < d.nradA; ++i) d.radA[i] = pl.radA[i];
+    for (int i = 0; i < d.nradB; ++i) d.radB[i] = pl.radB[i];
+    d.twA = (const double2*)t.twA.data(); d.twB = (const double2*)t.twB.data();
+    d.twMA = (const double2*)t.twMA.data(); d.twMB = (const double2*)t.twMB.data();
+    d.rlo = t.rlo.data(); d.rhi = t.rhi.data();
+    d.wlo = t.wlo.data(); d.whi = t.whi.data(); d.iwlo = t.iwlo.data(); d.iwhi = t.iwhi.data();
+    d.spec = spec.data(); d.spec2 = spec2.data(); d.digits = dig.data(); d.maxerr = &err;
+    std::vector<int2> cmap, rmap;
+    for (int w = 0; w < pl.col_wgs; ++w) cmap.push_back({0, w});
+    for (int w = 0; w < pl.row_wgs; ++w) rmap.push_back({0, w});
+    if (iters > 0) {
+        launch_col_kernel(&d, cmap.data(), pl.col_wgs, 0, LUCAS_IN_DIGITS, nullptr);
+        for (long long i = 0; i < iters; ++i) {
+            launch_row_kernel(&d, rmap.data(), pl.row_wgs, 0, nullptr);
+            launch_col_kernel(&d, cmap.data(), pl.col_wgs, 0, i + 1 == iters ? LUCAS_OUT_DIGITS : 0, nullptr);
+        }
+    }
+    std::memcpy(digits_out, dig.data(), sizeof(int32_t) * N);
+    float f;
+    std::memcpy(&f, &err, 4);
+    *maxerr = f;
+    return 0;
+}

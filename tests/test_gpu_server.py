@@ -1,0 +1,93 @@
+"""Serving: continuous batching of (p, iterations) requests with HBM-footprint
+admission.  Each result must be identical alone and in the mix, and must
+equal GMP."""
+import threading
+import time
+
+import pytest
+
+from conftest import fixture_res64
+from gpulucas_amd import lucas, workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+SMALL_PRIMES = [9689, 9941, 11213, 19937, 21701, 23209, 44497, 86243]
+SMALL_COMPOSITES = [9973, 10007, 20011, 30011, 44501, 50021]
+
+
+def test_server_full_runs_match_gmp(gmp_fixtures):
+    with lucas.Server(max_jobs=8, chunk_iters=128) as s:
+        ids = {}
+        for p in SMALL_PRIMES + SMALL_COMPOSITES:
+            ids[s.submit(p, p - 2)] = p
+        s.drain(timeout_s=300)
+        for jid, p in ids.items():
+            r = s.poll(jid)
+            assert r["done"] == 1 and r["status"] == lucas.OK
+            if p in SMALL_PRIMES:
+                assert r["is_zero"] == 1 and r["res64"] == 0, p
+            else:
+                want, _ = fixture_res64(gmp_fixtures, "small_composite_full", p, p - 2)
+                assert r["res64"] == want, p
+            assert r["max_roundoff"] < 0.4
+        st = s.stats()
+        assert st["peak_jobs"] <= 8 and st["iterations"] >= sum(p - 2 for p in ids.values())
+
+
+def test_server_admission_by_footprint():
+    ps = [86243, 110503, 132049, 216091]
+    biggest = max(lucas.plan(p)["device_bytes"] for p in ps)
+    # room for about two jobs at a time
+    with lucas.Server(hbm_budget_bytes=int(2.2 * biggest), chunk_iters=256) as s:
+        ids = [s.submit(p, 3000) for p in ps]
+        s.drain(timeout_s=300)
+        st = s.stats()
+        assert 1 <= st["peak_jobs"] <= 2
+        res = [s.poll(i) for i in ids]
+    for p, r in zip(ps, res):
+        with lucas.Context(p, 0) as c:
+            c.step(3000)
+            assert (r["res64"], bool(r["is_zero"])) == c.res64()
+
+
+def test_server_poisson_arrivals_alone_equals_mixed():
+    # jobs arrive while others are running; each result must match its solo run
+    jobs = [(t * 0.2, p, k // 10) for t, p, k in wl.serving_jobs(seed=5, count=12, p_hi=3e6)]
+    out = {}
+    with lucas.Server(chunk_iters=200) as s:
+        t0 = time.time()
+
+        def feed():
+            for t, p, k in jobs:
+                dt = t - (time.time() - t0)
+                if dt > 0:
+                    time.sleep(dt)
+                out[s.submit(p, k)] = (p, k)
+
+        th = threading.Thread(target=feed)
+        th.start()
+        th.join()
+        s.drain(timeout_s=600)
+        results = {jid: s.poll(jid) for jid in out}
+        assert s.stats()["peak_jobs"] >= 2  # work overlapped on the device
+    lat = sorted(r["finish_s"] - r["submit_s"] for r in results.values())
+    print(f"latency p50={lat[len(lat) // 2]:.3f}s max={lat[-1]:.3f}s", flush=True)
+    for jid, (p, k) in out.items():
+        r = results[jid]
+        assert r["status"] == lucas.OK
+        with lucas.Context(p, 0) as c:
+            c.step(k)
+            assert (r["res64"], bool(r["is_zero"])) == c.res64(), (p, k)
+
+
+def test_server_checkpoints(tmp_path):
+    with lucas.Server(chunk_iters=500, checkpoint_dir=str(tmp_path), checkpoint_every=1000) as s:
+        jid = s.submit(86243, 5000)
+        s.drain(timeout_s=120)
+        r = s.poll(jid)
+    path = tmp_path / "86243.chk"
+    assert path.exists()
+    with lucas.Context.load(str(path)) as c:
+        assert c.iteration % 1000 == 0 and 0 < c.iteration < 5000
+        c.step(5000 - c.iteration)
+        assert c.res64() == (r["res64"], bool(r["is_zero"]))
