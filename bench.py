@@ -68,9 +68,9 @@ def cpu_baseline(seconds_target):
     import oracle
 
     # calibrate on a short run, then time a bounded sample (~seconds_target s)
-    ms = oracle.gmp_ll_time(P_HEAD, 64, 20)
+    ms = oracle.gmp_time_ms(P_HEAD, 64, 20)
     iters = max(50, int(seconds_target * 1e3 / ms))
-    ms = oracle.gmp_ll_time(P_HEAD, 0, iters)
+    ms = oracle.gmp_time_ms(P_HEAD, 0, iters)
     return {"value": round(ms, 4), "unit": "ms/LL-iter", "cores": 1, "kind": "port",
             "sample": f"GMP {oracle.gmp_version()} LL squaring+fold at p={P_HEAD}, {iters} iterations "
                       f"after 64 warm-up, 1 thread"}

@@ -53,6 +53,7 @@ void fill_info(const Plan& pl, lucas_plan_info* out) {
     out->halo = pl.halo;
     out->col_wgs = pl.col_wgs;
     out->row_wgs = pl.row_wgs;
+    out->threads = pl.threads;
     out->device_bytes = job_bytes(pl);
 }
 
@@ -200,6 +201,15 @@ int lucas_plan_shape(int p, int N, int A, int C, lucas_plan_info* out) {
     int rc = plan_for(p, N, A, C, &pl);
     if (rc != LUCAS_OK) return rc;
     fill_info(pl, out);
+    return LUCAS_OK;
+}
+
+int lucas_plan_list(int p, int N, lucas_plan_info* out, int cap, int* count) {
+    if (!count || cap < 0 || (cap > 0 && !out)) return LUCAS_ERR_INVALID_ARG;
+    std::vector<Plan> c = plan_candidates(p, N);
+    if (c.empty()) return LUCAS_ERR_INVALID_ARG;
+    *count = (int)c.size();
+    for (int i = 0; i < cap && i < (int)c.size(); ++i) fill_info(c[i], &out[i]);
     return LUCAS_OK;
 }
 
@@ -460,6 +470,40 @@ int lucas_autotune_N(int p, int iters, const lucas_options* opt, lucas_plan_info
     }
     if (best.N == 0) return last_rc;
     fill_info(best, out);
+    return LUCAS_OK;
+}
+
+int lucas_autotune_shape(int p, int N, int iters, int max_candidates, const lucas_options* opt,
+                         lucas_plan_info* out, double* best_ms) {
+    if (!out || p < 7) return LUCAS_ERR_INVALID_ARG;
+    if (N == 0) N = default_length(p);
+    std::vector<Plan> cands = plan_candidates(p, N);
+    if (cands.empty()) return LUCAS_ERR_INVALID_ARG;
+    if (max_candidates > 0 && (int)cands.size() > max_candidates) cands.resize(max_candidates);
+    if (iters < 10) iters = 10;
+    const int device = opt ? opt->device : 0;
+    double best = 1e30;
+    Plan bestp;
+    int last_rc = LUCAS_OK;
+    for (const Plan& pl : cands) {
+        lucas_ctx* ctx = nullptr;
+        int rc = ctx_new(pl, device, &ctx);
+        double ms = 0;
+        if (rc == LUCAS_OK) rc = lucas_ctx_time_iterations(ctx, 40, &ms);  // warm-up, graph build
+        if (rc == LUCAS_OK) rc = lucas_ctx_time_iterations(ctx, iters, &ms);
+        if (ctx) lucas_ctx_destroy(ctx);
+        if (rc != LUCAS_OK) {
+            last_rc = rc;
+            continue;
+        }
+        if (ms < best) {
+            best = ms;
+            bestp = pl;
+        }
+    }
+    if (bestp.N == 0) return last_rc != LUCAS_OK ? last_rc : LUCAS_ERR_DEVICE;
+    fill_info(bestp, out);
+    if (best_ms) *best_ms = best;
     return LUCAS_OK;
 }
 

@@ -16,10 +16,12 @@
 #include <hip/hip_runtime.h>
 
 #define LUCAS_MAX_RADICES 16
-// Elements (complex words) each thread may hold while an FFT stage is in flight.
-#define LUCAS_EMAX 16
-// Threads per workgroup for both kernels.
-#define LUCAS_THREADS 256
+// Elements (complex words) each work-item may hold while an FFT stage is in
+// flight; a tile of a workgroup with T work-items holds at most EMAX*T words.
+#define LUCAS_EMAX 8
+// Work-items per workgroup: kernels are built for 256 and 512.
+#define LUCAS_THREADS_SMALL 256
+#define LUCAS_THREADS_LARGE 512
 // Deepest carry-save restoration supported (reference carry depth n <= 6).
 #define LUCAS_NMAX 6
 
@@ -34,6 +36,7 @@ struct LucasJobDesc {
     int halo;               // halo columns needed by the carry: ceil((n+1)/2)
     int C;                  // columns per column workgroup (B % C == 0)
     int col_wgs, row_wgs;   // workgroups per launch of each kernel
+    int threads;            // work-items per workgroup (256 or 512)
     int nradA, nradB;
     int radA[LUCAS_MAX_RADICES];
     int radB[LUCAS_MAX_RADICES];

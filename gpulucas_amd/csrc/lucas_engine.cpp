@@ -15,9 +15,10 @@
 
 namespace lucas {
 
-hipError_t launch_col_kernel(const LucasJobDesc* jobs, const int2* map, int nwg, size_t lds, int mode,
+hipError_t launch_col_kernel(int threads, const LucasJobDesc* jobs, const int2* map, int nwg, size_t lds, int mode,
                              hipStream_t stream);
-hipError_t launch_row_kernel(const LucasJobDesc* jobs, const int2* map, int nwg, size_t lds, hipStream_t stream);
+hipError_t launch_row_kernel(int threads, const LucasJobDesc* jobs, const int2* map, int nwg, size_t lds,
+                             hipStream_t stream);
 
 static const int kGraphIters = 32;
 
@@ -97,6 +98,7 @@ int job_create(const Plan& plan, int device, hipStream_t stream, Job** out) {
     d.C = plan.C;
     d.col_wgs = plan.col_wgs;
     d.row_wgs = plan.row_wgs;
+    d.threads = plan.threads;
     d.nradA = (int)plan.radA.size();
     d.nradB = (int)plan.radB.size();
     for (int i = 0; i < d.nradA; ++i) d.radA[i] = plan.radA[i];
@@ -166,8 +168,10 @@ Engine::~Engine() {
     if (gexec_) hipGraphExecDestroy(gexec_);
     if (graph_) hipGraphDestroy(graph_);
     if (d_desc_) hipFree(d_desc_);
-    if (d_colmap_) hipFree(d_colmap_);
-    if (d_rowmap_) hipFree(d_rowmap_);
+    for (Group& g : groups_) {
+        if (g.d_colmap) hipFree(g.d_colmap);
+        if (g.d_rowmap) hipFree(g.d_rowmap);
+    }
     if (ev0_) hipEventDestroy(ev0_);
     if (ev1_) hipEventDestroy(ev1_);
     if (stream_) hipStreamDestroy(stream_);
@@ -203,15 +207,19 @@ int Engine::prepare() {
         graph_ = nullptr;
     }
     std::vector<LucasJobDesc> desc;
-    std::vector<int2> cmap, rmap;
-    lds_col_ = lds_row_ = 0;
+    std::vector<int2> cmap[2], rmap[2];
+    groups_[0].threads = LUCAS_THREADS_SMALL;
+    groups_[1].threads = LUCAS_THREADS_LARGE;
+    for (Group& g : groups_) g.lds_col = g.lds_row = 0;
     for (size_t i = 0; i < jobs_.size(); ++i) {
         const Job* j = jobs_[i];
+        const int gi = j->plan.threads == LUCAS_THREADS_LARGE ? 1 : 0;
+        Group& g = groups_[gi];
         desc.push_back(j->desc);
-        for (int w = 0; w < j->plan.col_wgs; ++w) cmap.push_back(make_int2((int)i, w));
-        for (int w = 0; w < j->plan.row_wgs; ++w) rmap.push_back(make_int2((int)i, w));
-        if (j->plan.lds_col > lds_col_) lds_col_ = j->plan.lds_col;
-        if (j->plan.lds_row > lds_row_) lds_row_ = j->plan.lds_row;
+        for (int w = 0; w < j->plan.col_wgs; ++w) cmap[gi].push_back(make_int2((int)i, w));
+        for (int w = 0; w < j->plan.row_wgs; ++w) rmap[gi].push_back(make_int2((int)i, w));
+        if (j->plan.lds_col > g.lds_col) g.lds_col = j->plan.lds_col;
+        if (j->plan.lds_row > g.lds_row) g.lds_row = j->plan.lds_row;
     }
     auto grow = [&](void** ptr, size_t* cap, size_t need) -> bool {
         if (need <= *cap) return true;
@@ -223,34 +231,44 @@ int Engine::prepare() {
         *cap = n;
         return true;
     };
-    if (!grow((void**)&d_desc_, &cap_desc_, sizeof(LucasJobDesc) * (desc.size() + 1)) ||
-        !grow((void**)&d_colmap_, &cap_col_, sizeof(int2) * (cmap.size() + 1)) ||
-        !grow((void**)&d_rowmap_, &cap_row_, sizeof(int2) * (rmap.size() + 1)))
-        return LUCAS_ERR_OOM;
+    if (!grow((void**)&d_desc_, &cap_desc_, sizeof(LucasJobDesc) * (desc.size() + 1))) return LUCAS_ERR_OOM;
     hipError_t e = hipSuccess;
-    if (!desc.empty()) {
+    if (!desc.empty())
         e = hipMemcpyAsync(d_desc_, desc.data(), sizeof(LucasJobDesc) * desc.size(), hipMemcpyHostToDevice, stream_);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(d_colmap_, cmap.data(), sizeof(int2) * cmap.size(), hipMemcpyHostToDevice, stream_);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(d_rowmap_, rmap.data(), sizeof(int2) * rmap.size(), hipMemcpyHostToDevice, stream_);
-        if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+    for (int gi = 0; gi < 2 && e == hipSuccess; ++gi) {
+        Group& g = groups_[gi];
+        if (!grow((void**)&g.d_colmap, &g.cap_col, sizeof(int2) * (cmap[gi].size() + 1)) ||
+            !grow((void**)&g.d_rowmap, &g.cap_row, sizeof(int2) * (rmap[gi].size() + 1)))
+            return LUCAS_ERR_OOM;
+        g.ncol = (int)cmap[gi].size();
+        g.nrow = (int)rmap[gi].size();
+        if (g.ncol)
+            e = hipMemcpyAsync(g.d_colmap, cmap[gi].data(), sizeof(int2) * g.ncol, hipMemcpyHostToDevice, stream_);
+        if (e == hipSuccess && g.nrow)
+            e = hipMemcpyAsync(g.d_rowmap, rmap[gi].data(), sizeof(int2) * g.nrow, hipMemcpyHostToDevice, stream_);
     }
+    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
     if (e != hipSuccess) return LUCAS_ERR_DEVICE;
-    ncol_ = (int)cmap.size();
-    nrow_ = (int)rmap.size();
     dirty_ = false;
     return LUCAS_OK;
 }
 
 int Engine::launch_col(int mode) {
-    return launch_col_kernel(d_desc_, d_colmap_, ncol_, lds_col_, mode, stream_) == hipSuccess ? LUCAS_OK
-                                                                                               : LUCAS_ERR_DEVICE;
+    for (const Group& g : groups_) {
+        if (!g.ncol) continue;
+        if (launch_col_kernel(g.threads, d_desc_, g.d_colmap, g.ncol, g.lds_col, mode, stream_) != hipSuccess)
+            return LUCAS_ERR_DEVICE;
+    }
+    return LUCAS_OK;
 }
 
 int Engine::launch_row() {
-    return launch_row_kernel(d_desc_, d_rowmap_, nrow_, lds_row_, stream_) == hipSuccess ? LUCAS_OK
-                                                                                       : LUCAS_ERR_DEVICE;
+    for (const Group& g : groups_) {
+        if (!g.nrow) continue;
+        if (launch_row_kernel(g.threads, d_desc_, g.d_rowmap, g.nrow, g.lds_row, stream_) != hipSuccess)
+            return LUCAS_ERR_DEVICE;
+    }
+    return LUCAS_OK;
 }
 
 int Engine::build_graph() {

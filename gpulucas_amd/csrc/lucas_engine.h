@@ -55,17 +55,24 @@ private:
     int launch_row();
     int build_graph();
 
+    // Jobs whose tiles need the same workgroup size share one launch.
+    struct Group {
+        int threads = 0;
+        int2* d_colmap = nullptr;
+        int2* d_rowmap = nullptr;
+        size_t cap_col = 0, cap_row = 0;
+        int ncol = 0, nrow = 0;
+        size_t lds_col = 0, lds_row = 0;
+    };
+
     int device_;
     hipStream_t stream_ = nullptr;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     std::vector<Job*> jobs_;
     bool dirty_ = true;
     LucasJobDesc* d_desc_ = nullptr;
-    int2* d_colmap_ = nullptr;
-    int2* d_rowmap_ = nullptr;
-    size_t cap_desc_ = 0, cap_col_ = 0, cap_row_ = 0;
-    int ncol_ = 0, nrow_ = 0;
-    size_t lds_col_ = 0, lds_row_ = 0;
+    size_t cap_desc_ = 0;
+    Group groups_[2];
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t gexec_ = nullptr;
     int graph_iters_ = 0;

@@ -23,6 +23,8 @@
 namespace lucas {
 
 static const long double kPi = 3.141592653589793238462643383279502884L;
+// LDS one workgroup may allocate (column tile + digit buffer, or two rows).
+static const size_t kMaxLds = 65536;
 
 bool valid_length(long long N) {
     if (N < 4 || (N & 1)) return false;
@@ -57,76 +59,99 @@ std::vector<int> radix_plan(int L) {
     return r;
 }
 
-static long long stage_cost(const std::vector<int>& rad, long long elems) {
-    // per-thread element passes through LDS for one FFT of the tile
-    long long per = (elems + LUCAS_THREADS - 1) / LUCAS_THREADS;
+static long long stage_cost(const std::vector<int>& rad, long long elems, int threads) {
+    // per-work-item element passes through LDS for one FFT of the tile
+    long long per = (elems + threads - 1) / threads;
     return (long long)rad.size() * per;
 }
 
-std::string make_plan(int p, int N, int forceA, int forceC, Plan* out) {
+static std::string base_plan(int p, int N, Plan* pl) {
     if (p < 7) return "p must be >= 7";
     if (!valid_length(N)) return "N must be even and of the form 2^a 3^b 5^c 7^d";
     if ((long long)N > (long long)p) return "N must not exceed p";
-    Plan pl;
-    pl.p = p;
-    pl.N = N;
-    pl.M = N / 2;
-    pl.q = p / N;
-    pl.b = p % N;
-    pl.ncarry = carry_depth(pl.q);
-    if (pl.ncarry < 0) return "p/N >= 21 bits per word is out of range";
-    pl.halo = (pl.ncarry + 2) / 2;  // ceil((n+1)/2)
+    pl->p = p;
+    pl->N = N;
+    pl->M = N / 2;
+    pl->q = p / N;
+    pl->b = p % N;
+    pl->ncarry = carry_depth(pl->q);
+    if (pl->ncarry < 0) return "p/N >= 21 bits per word is out of range";
+    pl->halo = (pl->ncarry + 2) / 2;  // ceil((n+1)/2)
     // The restoration keeps product bits below the sum of n+1 digit widths;
     // a product has at most 2(q+1) + log2(N) + 1 bits, and only products
     // below 2^53 are exact FP64 integers in the first place.
-    {
-        int coverage = (pl.ncarry + 1) * pl.q;
-        int need = std::min(53, 2 * (pl.q + 1) + (int)ceil(log2((double)N)) + 1);
-        if (coverage < need) return "word size too small for the carry-save restoration";
-    }
-    const long long cap = (long long)LUCAS_EMAX * LUCAS_THREADS;
-    long long best = std::numeric_limits<long long>::max();
-    Plan bestp;
+    int coverage = (pl->ncarry + 1) * pl->q;
+    int need = std::min(53, 2 * (pl->q + 1) + (int)ceil(log2((double)N)) + 1);
+    if (coverage < need) return "word size too small for the carry-save restoration";
+    return "";
+}
+
+// Modelled cost of a tiling (lower is better): per-work-item LDS passes on
+// the critical path of each kernel, serialised when the grid exceeds about
+// two workgroups per CU, plus a small term for redundant halo work.
+static long long model_cost(const Plan& pl) {
+    const int T = pl.threads;
+    long long col_t = stage_cost(pl.radA, (long long)pl.A * (pl.C + pl.halo), T) +
+                      stage_cost(pl.radA, (long long)pl.A * pl.C, T) + 8;
+    long long row_t = 2 * stage_cost(pl.radB, 2LL * pl.B, T) + 8;
+    col_t *= (pl.col_wgs + 511) / 512;
+    row_t *= (pl.row_wgs + 511) / 512;
+    // narrow tiles read short row segments: penalise C < 4 (uncoalesced)
+    long long coal = pl.C >= 4 ? 0 : (4 / pl.C) * 4;
+    long long total = (long long)pl.A * (2 * pl.C + pl.halo) * pl.col_wgs / 256 + 4LL * pl.B * pl.row_wgs / 256;
+    return 64 * (col_t + row_t + coal) + total / 64;
+}
+
+std::vector<Plan> plan_candidates(int p, int N) {
+    std::vector<Plan> out;
+    Plan pl;
+    if (!base_plan(p, N, &pl).empty()) return out;
     for (int A = 2; A <= pl.M; ++A) {
         if (pl.M % A) continue;
-        if (forceA && A != forceA) continue;
         const int B = pl.M / A;
         if (B < pl.halo) continue;
-        if (2LL * B > cap) continue;
         std::vector<int> ra = radix_plan(A), rb = radix_plan(B);
         if (ra.empty() || rb.empty()) continue;
         if ((int)ra.size() > LUCAS_MAX_RADICES || (int)rb.size() > LUCAS_MAX_RADICES) continue;
         for (int C : {1, 2, 4, 8, 16, 32, 64}) {
-            if (forceC && C != forceC) continue;
             if (B % C) continue;
-            if ((long long)A * (C + pl.halo) > cap) continue;
-            const long long colw = B / C, roww = A / 2 + 1;
-            long long col_t = stage_cost(ra, (long long)A * (C + pl.halo)) + stage_cost(ra, (long long)A * C) + 6;
-            long long row_t = 2 * stage_cost(rb, 2LL * B) + 6;
-            // waves of workgroups beyond ~2 per CU serialise
-            col_t *= (colw + 511) / 512;
-            row_t *= (roww + 511) / 512;
-            // redundant halo work and total work both cost bandwidth
-            long long total = (long long)A * (2 * C + pl.halo) * colw / 256 + 4LL * B * roww / 256;
-            long long cost = 64 * (col_t + row_t) + total / 64;
-            if (cost < best) {
-                best = cost;
-                bestp = pl;
-                bestp.A = A;
-                bestp.B = B;
-                bestp.C = C;
-                bestp.radA = ra;
-                bestp.radB = rb;
-                bestp.col_wgs = (int)colw;
-                bestp.row_wgs = (int)roww;
-            }
+            const long long tile = std::max<long long>((long long)A * (C + pl.halo), 2LL * B);
+            int T = 0;
+            if (tile <= (long long)LUCAS_EMAX * LUCAS_THREADS_SMALL) T = LUCAS_THREADS_SMALL;
+            else if (tile <= (long long)LUCAS_EMAX * LUCAS_THREADS_LARGE) T = LUCAS_THREADS_LARGE;
+            else continue;
+            const size_t lds_col = sizeof(double) * 2 * (size_t)A * (C + pl.halo) + sizeof(int) * 2 * (size_t)A * C;
+            const size_t lds_row = sizeof(double) * 2 * 2 * (size_t)B;
+            if (lds_col > kMaxLds || lds_row > kMaxLds) continue;
+            Plan c = pl;
+            c.A = A;
+            c.B = B;
+            c.C = C;
+            c.threads = T;
+            c.radA = ra;
+            c.radB = rb;
+            c.col_wgs = B / C;
+            c.row_wgs = A / 2 + 1;
+            c.lds_col = lds_col;
+            c.lds_row = lds_row;
+            out.push_back(c);
         }
     }
-    if (bestp.A == 0) return "no admissible (A, B, C) tiling for this N";
-    bestp.lds_col = sizeof(double) * 2 * (size_t)bestp.A * (bestp.C + bestp.halo);
-    bestp.lds_row = sizeof(double) * 2 * 2 * (size_t)bestp.B;
-    *out = bestp;
-    return "";
+    std::stable_sort(out.begin(), out.end(), [](const Plan& a, const Plan& b) { return model_cost(a) < model_cost(b); });
+    return out;
+}
+
+std::string make_plan(int p, int N, int forceA, int forceC, Plan* out) {
+    Plan pl;
+    std::string why = base_plan(p, N, &pl);
+    if (!why.empty()) return why;
+    for (const Plan& c : plan_candidates(p, N)) {
+        if (forceA && c.A != forceA) continue;
+        if (forceC && c.C != forceC) continue;
+        *out = c;
+        return "";
+    }
+    return "no admissible (A, B, C) tiling for this N";
 }
 
 double max_bits_per_word(int N) {

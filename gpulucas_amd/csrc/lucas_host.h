@@ -17,6 +17,7 @@ struct Plan {
     int ncarry = 0;         // reference carry depth n
     int halo = 0;           // ceil((n+1)/2)
     int col_wgs = 0, row_wgs = 0;
+    int threads = 0;        // work-items per workgroup (256 or 512)
     std::vector<int> radA, radB;
     size_t lds_col = 0, lds_row = 0;  // bytes of LDS per workgroup
 };
@@ -30,6 +31,8 @@ std::vector<int> radix_plan(int L);
 // Full plan for (p, N) with optional forced A / C (0 = choose).  Returns an
 // empty string on success or the reason for refusal.
 std::string make_plan(int p, int N, int A, int C, Plan* out);
+// Every admissible (A, C) tiling of (p, N), cheapest modelled first.
+std::vector<Plan> plan_candidates(int p, int N);
 // Default N for p: smallest admissible length under the bits-per-word rule.
 int default_length(int p);
 // Bits per word allowed at length N by the default rule.

@@ -24,6 +24,12 @@
 
 #include "lucas_desc.h"
 
+// Dynamic LDS of the running workgroup (sized per launch).
+__device__ __forceinline__ double2* lucas_dynamic_lds() {
+    extern __shared__ double2 lucas_lds[];
+    return lucas_lds;
+}
+
 namespace {
 
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
@@ -41,6 +47,22 @@ __device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -
 template <int SIGN>
 __device__ __forceinline__ double2 mul_si(double2 a) {
     return SIGN < 0 ? make_double2(a.y, -a.x) : make_double2(-a.y, a.x);
+}
+
+// Quotient t / d for 0 <= t < 2^22, 0 < d < 2^13: float reciprocal estimate
+// (error below one) and one integer correction.  Avoids the long integer
+// division sequence for every index split.
+struct FDiv {
+    int d;
+    float r;
+};
+__device__ __forceinline__ FDiv fdiv_make(int d) { return FDiv{d, 1.0f / (float)d}; }
+__device__ __forceinline__ int fdiv(int t, FDiv f) {
+    int q = (int)((float)t * f.r);
+    const int rem = t - q * f.d;
+    if (rem >= f.d) ++q;
+    else if (rem < 0) --q;
+    return q;
 }
 
 // ---------------------------------------------------------------------------
@@ -154,28 +176,32 @@ template <int SIGN> struct Dft<8, SIGN> {
 // so one LDS buffer serves all stages.  SEQ_FAST maps consecutive threads to
 // consecutive sequences (column tiles) instead of consecutive butterflies.
 // ---------------------------------------------------------------------------
-template <int R, int SIGN, bool SEQ_FAST>
+template <int T, int R, int SIGN, bool SEQ_FAST>
 __device__ __forceinline__ void fft_stage(double2* lds, int L, int Ns, int nseq, int sIdx, int sSeq,
                                           const double2* __restrict__ tw, int tid) {
     constexpr int BM = (LUCAS_EMAX + R - 1) / R;
     const int nb = L / R;
     const int total = nb * nseq;
     const int twstep = L / (Ns * R);
+    const FDiv fsplit = fdiv_make(SEQ_FAST ? nseq : nb);
+    const FDiv fns = fdiv_make(Ns);
     double2 v[BM][R];
     int outb[BM];
 #pragma unroll
     for (int i = 0; i < BM; ++i) {
-        const int t = tid + i * LUCAS_THREADS;
+        const int t = tid + i * T;
         if (t < total) {
             int seq, j;
+            const int qq = fdiv(t, fsplit);
             if (SEQ_FAST) {
-                seq = t % nseq;
-                j = t / nseq;
+                j = qq;
+                seq = t - qq * nseq;
             } else {
-                j = t % nb;
-                seq = t / nb;
+                seq = qq;
+                j = t - qq * nb;
             }
-            const int k = j % Ns;
+            const int jq = fdiv(j, fns);
+            const int k = j - jq * Ns;
             const int base = seq * sSeq;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
@@ -187,13 +213,13 @@ __device__ __forceinline__ void fft_stage(double2* lds, int L, int Ns, int nseq,
                 v[i][r] = x;
             }
             Dft<R, SIGN>::run(v[i]);
-            outb[i] = base + ((j / Ns) * Ns * R + k) * sIdx;
+            outb[i] = base + (jq * Ns * R + k) * sIdx;
         }
     }
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < BM; ++i) {
-        const int t = tid + i * LUCAS_THREADS;
+        const int t = tid + i * T;
         if (t < total) {
 #pragma unroll
             for (int r = 0; r < R; ++r) lds[outb[i] + r * Ns * sIdx] = v[i][r];
@@ -202,19 +228,22 @@ __device__ __forceinline__ void fft_stage(double2* lds, int L, int Ns, int nseq,
     __syncthreads();
 }
 
-template <int SIGN, bool SEQ_FAST>
-__device__ void fft_lds(double2* lds, int L, const int* rad, int nrad, int nseq, int sIdx, int sSeq,
-                        const double2* __restrict__ tw, int tid) {
+// Forward transforms only: an inverse transform is conj(DFT(conj(x))), so the
+// kernels conjugate on the way in and out and reach this code from one call
+// site each (one inlined copy per kernel).
+template <int T, int SIGN, bool SEQ_FAST>
+__device__ __forceinline__ void fft_lds(double2* lds, int L, const int* rad, int nrad, int nseq, int sIdx, int sSeq,
+                                     const double2* __restrict__ tw, int tid) {
     int Ns = 1;
     for (int s = 0; s < nrad; ++s) {
         const int R = rad[s];
         switch (R) {
-            case 2: fft_stage<2, SIGN, SEQ_FAST>(lds, L, Ns, nseq, sIdx, sSeq, tw, tid); break;
-            case 3: fft_stage<3, SIGN, SEQ_FAST>(lds, L, Ns, nseq, sIdx, sSeq, tw, tid); break;
-            case 4: fft_stage<4, SIGN, SEQ_FAST>(lds, L, Ns, nseq, sIdx, sSeq, tw, tid); break;
-            case 5: fft_stage<5, SIGN, SEQ_FAST>(lds, L, Ns, nseq, sIdx, sSeq, tw, tid); break;
-            case 7: fft_stage<7, SIGN, SEQ_FAST>(lds, L, Ns, nseq, sIdx, sSeq, tw, tid); break;
-            default: fft_stage<8, SIGN, SEQ_FAST>(lds, L, Ns, nseq, sIdx, sSeq, tw, tid); break;
+            case 2: fft_stage<T, 2, SIGN, SEQ_FAST>(lds, L, Ns, nseq, sIdx, sSeq, tw, tid); break;
+            case 3: fft_stage<T, 3, SIGN, SEQ_FAST>(lds, L, Ns, nseq, sIdx, sSeq, tw, tid); break;
+            case 4: fft_stage<T, 4, SIGN, SEQ_FAST>(lds, L, Ns, nseq, sIdx, sSeq, tw, tid); break;
+            case 5: fft_stage<T, 5, SIGN, SEQ_FAST>(lds, L, Ns, nseq, sIdx, sSeq, tw, tid); break;
+            case 7: fft_stage<T, 7, SIGN, SEQ_FAST>(lds, L, Ns, nseq, sIdx, sSeq, tw, tid); break;
+            default: fft_stage<T, 8, SIGN, SEQ_FAST>(lds, L, Ns, nseq, sIdx, sSeq, tw, tid); break;
         }
         Ns *= R;
     }
@@ -235,19 +264,20 @@ __device__ __forceinline__ void block_max_err(float err, unsigned int* dst) {
 }
 
 // Carry-save restoration of one digit (reference llintToIrrBal<n>,
-// IrrBaseBalanced.cu:52-110).  cv[0..] are the rounded products of digits
-// D-n .. D (cv[n] is digit D), w[] their widths.  Returns the balanced digit
-// and writes the carry into *carry.
+// IrrBaseBalanced.cu:52-110).  The window is stored top-down: cv[i] is the
+// rounded product of digit D-i and w[i] its width, i = 0..n, so every index
+// is a compile-time constant once the loops unroll.  Returns the balanced
+// digit and writes the carry into *carry.
 __device__ __forceinline__ int restore_digit(const long long* cv, const int* w, int n, int* carry) {
-    const int W = w[n];
+    const int W = w[0];
     const long long mask = (1LL << W) - 1;
     int sum = 0;
     int shift = 0;
 #pragma unroll
     for (int i = 0; i <= LUCAS_NMAX; ++i) {
         if (i <= n) {
-            const long long c = cv[n - i];
-            if (i > 0) shift += w[n - i];
+            const long long c = cv[i];
+            if (i > 0) shift += w[i];
             const unsigned long long mag = c < 0 ? (unsigned long long)(-c) : (unsigned long long)c;
             const int part = shift < 64 ? (int)((mag >> shift) & (unsigned long long)mask) : 0;
             sum += c < 0 ? -part : part;
@@ -263,14 +293,14 @@ __device__ __forceinline__ int restore_digit(const long long* cv, const int* w, 
     return sum - h * (1 << W);
 }
 
-}  // namespace
-
 // ---------------------------------------------------------------------------
-// Column kernel: one workgroup per C-column tile of one job.
+// Column kernel: one workgroup of T work-items per C-column tile of one job.
+// LDS: the (C+H) x A complex tile, then C x A int2 new digits.
 // ---------------------------------------------------------------------------
-extern "C" __global__ void __launch_bounds__(LUCAS_THREADS)
-lucas_col_kernel(const LucasJobDesc* __restrict__ jobs, const int2* __restrict__ wgmap, int mode) {
-    extern __shared__ double2 lds[];
+template <int T>
+__device__ __forceinline__ void col_kernel_body(const LucasJobDesc* __restrict__ jobs, const int2* __restrict__ wgmap,
+                                                int mode) {
+    double2* const lds = lucas_dynamic_lds();
     const int2 jm = wgmap[blockIdx.x];
     const LucasJobDesc& J = jobs[jm.x];
     const int A = J.A, B = J.B, C = J.C, H = J.halo, N = J.N;
@@ -278,28 +308,56 @@ lucas_col_kernel(const LucasJobDesc* __restrict__ jobs, const int2* __restrict__
     const int c0 = jm.y * C;
     const int tid = threadIdx.x;
     const int ncl = C + H;  // local columns: [0,H) halo, [H, H+C) own
-    constexpr int EPT = LUCAS_EMAX;
-    int xe[EPT], xo[EPT];   // new digits 2m, 2m+1 of the thread's own elements
+    const FDiv fncl = fdiv_make(ncl), fC = fdiv_make(C);
+    int2* const xdig = reinterpret_cast<int2*>(lds + A * ncl);  // new digits (2m, 2m+1)
     float err = 0.0f;
 
-    if (!(mode & LUCAS_IN_DIGITS)) {
-        // 1. load the tile (+ halo) with the inverse outer twiddle omega_M^(-c*r)
-        for (int t = tid; t < A * ncl; t += LUCAS_THREADS) {
-            const int r = t / ncl, lc = t - r * ncl;
+    // Pass 0 (inverse column FFT of the tile + halo, unweight, round, carry)
+    // runs unless the digits come from memory; pass 1 weights the new digits
+    // and runs the forward column FFT.  One FFT call site for both passes.
+    int pass = (mode & LUCAS_IN_DIGITS) ? 1 : 0;
+    if (pass == 1) {
+        for (int t = tid; t < A * C; t += T) {
+            const int r = fdiv(t, fC), lo = t - r * C;
+            xdig[t] = reinterpret_cast<const int2*>(J.digits)[(size_t)(c0 + lo) + (size_t)B * r];
+        }
+    } else {
+        // load the tile with the inverse outer twiddle omega_M^(-c*r), conjugated
+        for (int t = tid; t < A * ncl; t += T) {
+            const int r = fdiv(t, fncl), lc = t - r * ncl;
             int gc = c0 - H + lc;
             if (gc < 0) gc += B;
             const double2 v = J.spec2[(size_t)r * B + gc];
-            lds[t] = cmulc(v, outer_twiddle(J, gc * r));
+            lds[t] = cconj(cmulc(v, outer_twiddle(J, gc * r)));
+        }
+    }
+    for (;;) {
+        if (pass == 1) {
+            __syncthreads();
+            // weight: y_j = 2^(r_j/N) x_j into a compact C-column tile
+            for (int t = tid; t < A * C; t += T) {
+                const int r = fdiv(t, fC), lo = t - r * C;
+                const int gc = c0 + lo;
+                const int rh = J.rhi[r];
+                const double wh = J.whi[r];
+                const int je = 2 * gc, jo = je + 1;
+                const double ae = J.wlo[je] * wh * (J.rlo[je] + rh >= N ? 0.5 : 1.0);
+                const double ao = J.wlo[jo] * wh * (J.rlo[jo] + rh >= N ? 0.5 : 1.0);
+                const int2 x = xdig[t];
+                lds[t] = make_double2(ae * (double)x.x, ao * (double)x.y);
+            }
         }
         __syncthreads();
-        // 2. inverse column FFTs
-        fft_lds<+1, true>(lds, A, J.radA, J.nradA, ncl, ncl, 1, J.twA, tid);
-        // 3. unweight, -2 on digit 0, round half-even, roundoff
-        for (int t = tid; t < A * ncl; t += LUCAS_THREADS) {
-            const int r = t / ncl, lc = t - r * ncl;
+        const int nseq = pass ? C : ncl;
+        fft_lds<T, -1, true>(lds, A, J.radA, J.nradA, nseq, nseq, 1, J.twA, tid);
+        if (pass == 1) break;
+
+        // unweight, -2 on digit 0, round half-even, roundoff
+        for (int t = tid; t < A * ncl; t += T) {
+            const int r = fdiv(t, fncl), lc = t - r * ncl;
             int gc = c0 - H + lc;
             if (gc < 0) gc += B;
-            const double2 v = lds[t];
+            const double2 v = cconj(lds[t]);
             const int rh = J.rhi[r];
             const double iwh = J.iwhi[r];
             const int je = 2 * gc, jo = je + 1;
@@ -313,93 +371,61 @@ lucas_col_kernel(const LucasJobDesc* __restrict__ jobs, const int2* __restrict__
             lds[t] = make_double2(re, ro);
         }
         __syncthreads();
-        // 4. carry-save restoration + single ripple for the own columns
+        // carry-save restoration + single ripple for the own columns
+        for (int t = tid; t < A * C; t += T) {
+            const int r = fdiv(t, fC), lo = t - r * C;
+            const int lc = lo + H, gc = c0 + lo;
+            // products and widths of digits 2m+1-k, k = 0 .. n+2 (top-down)
+            long long cv[LUCAS_NMAX + 3];
+            int w[LUCAS_NMAX + 3];
 #pragma unroll
-        for (int i = 0; i < EPT; ++i) {
-            const int t = tid + i * LUCAS_THREADS;
-            if (t < A * C) {
-                const int r = t / C, lo = t - r * C;
-                const int lc = lo + H, gc = c0 + lo;
-                // products and widths of digits 2m+d, d = -1-n .. 1  (index d+1+n)
-                long long cv[LUCAS_NMAX + 3];
-                int w[LUCAS_NMAX + 3];
-                int r2m = J.rlo[2 * gc] + J.rhi[r];
-                if (r2m >= N) r2m -= N;
-#pragma unroll
-                for (int k = 0; k < LUCAS_NMAX + 3; ++k) {
-                    if (k <= n + 2) {
-                        const int d = k - 1 - n;
-                        const int ie = (1 - d) >> 1;  // element offset back
-                        int rr = r;
-                        if (gc - ie < 0) rr = (r == 0) ? A - 1 : r - 1;
-                        const double2 v = lds[rr * ncl + (lc - ie)];
-                        cv[k] = (long long)((d & 1) ? v.y : v.x);
-                        // r_{2m+d} = (r_2m - d*b) mod N
-                        long long rd = (long long)r2m - (long long)d * bb;
-                        rd %= N;
-                        if (rd < 0) rd += N;
-                        w[k] = q + (rd < bb ? 1 : 0);
-                    }
+            for (int k = 0; k < LUCAS_NMAX + 3; ++k) {
+                if (k <= n + 2) {
+                    const int ie = k >> 1;  // element offset back
+                    int rr = r;
+                    if (gc - ie < 0) rr = (r == 0) ? A - 1 : r - 1;
+                    const double2 v = lds[rr * ncl + (lc - ie)];
+                    cv[k] = (long long)((k & 1) ? v.x : v.y);
                 }
-                int hm1, h0, h1;
-                (void)restore_digit(cv, w, n, &hm1);                // digit 2m-1: carry only
-                const int d0 = restore_digit(cv + 1, w + 1, n, &h0);  // digit 2m
-                const int d1 = restore_digit(cv + 2, w + 2, n, &h1);  // digit 2m+1
-                (void)h1;
-                xe[i] = d0 + hm1;
-                xo[i] = d1 + h0;
             }
-        }
-    } else {
+            // digit j is wide iff r_j < b, r_{j-1} = r_j + b (mod N)
+            int r2m = J.rlo[2 * gc] + J.rhi[r];
+            if (r2m >= N) r2m -= N;
+            int rv = r2m - bb;
+            if (rv < 0) rv += N;
+            w[0] = q + (rv < bb ? 1 : 0);  // digit 2m+1
+            rv = r2m;
 #pragma unroll
-        for (int i = 0; i < EPT; ++i) {
-            const int t = tid + i * LUCAS_THREADS;
-            if (t < A * C) {
-                const int r = t / C, lo = t - r * C;
-                const size_t m = (size_t)(c0 + lo) + (size_t)B * r;
-                xe[i] = J.digits[2 * m];
-                xo[i] = J.digits[2 * m + 1];
+            for (int k = 1; k < LUCAS_NMAX + 3; ++k) {
+                if (k <= n + 2) {
+                    if (k > 1) {
+                        rv += bb;
+                        if (rv >= N) rv -= N;
+                    }
+                    w[k] = q + (rv < bb ? 1 : 0);  // digit 2m+1-k
+                }
             }
+            int hm1, h0, h1;
+            const int d1 = restore_digit(cv, w, n, &h1);          // digit 2m+1
+            const int d0 = restore_digit(cv + 1, w + 1, n, &h0);  // digit 2m
+            (void)restore_digit(cv + 2, w + 2, n, &hm1);          // digit 2m-1: carry only
+            (void)h1;
+            xdig[t] = make_int2(d0 + hm1, d1 + h0);
         }
-    }
-
-    if (mode & LUCAS_OUT_DIGITS) {
-#pragma unroll
-        for (int i = 0; i < EPT; ++i) {
-            const int t = tid + i * LUCAS_THREADS;
-            if (t < A * C) {
-                const int r = t / C, lo = t - r * C;
-                const size_t m = (size_t)(c0 + lo) + (size_t)B * r;
-                J.digits[2 * m] = xe[i];
-                J.digits[2 * m + 1] = xo[i];
+        if (mode & LUCAS_OUT_DIGITS) {
+            __syncthreads();
+            for (int t = tid; t < A * C; t += T) {
+                const int r = fdiv(t, fC), lo = t - r * C;
+                reinterpret_cast<int2*>(J.digits)[(size_t)(c0 + lo) + (size_t)B * r] = xdig[t];
             }
+            block_max_err(err, J.maxerr);
+            return;
         }
-        block_max_err(err, J.maxerr);
-        return;
+        pass = 1;
     }
-
-    __syncthreads();  // every carry read of the rounded tile is done
-    // 5. weight: y_j = 2^(r_j/N) * x_j, into a compact C-column tile
-#pragma unroll
-    for (int i = 0; i < EPT; ++i) {
-        const int t = tid + i * LUCAS_THREADS;
-        if (t < A * C) {
-            const int r = t / C, lo = t - r * C;
-            const int gc = c0 + lo;
-            const int rh = J.rhi[r];
-            const double wh = J.whi[r];
-            const int je = 2 * gc, jo = je + 1;
-            const double ae = J.wlo[je] * wh * (J.rlo[je] + rh >= N ? 0.5 : 1.0);
-            const double ao = J.wlo[jo] * wh * (J.rlo[jo] + rh >= N ? 0.5 : 1.0);
-            lds[t] = make_double2(ae * (double)xe[i], ao * (double)xo[i]);
-        }
-    }
-    __syncthreads();
-    // 6. forward column FFTs
-    fft_lds<-1, true>(lds, A, J.radA, J.nradA, C, C, 1, J.twA, tid);
-    // 7. outer twiddle omega_M^(c*k) and store row-major
-    for (int t = tid; t < A * C; t += LUCAS_THREADS) {
-        const int k = t / C, lo = t - k * C;
+    // outer twiddle omega_M^(c*k) and store row-major
+    for (int t = tid; t < A * C; t += T) {
+        const int k = fdiv(t, fC), lo = t - k * C;
         const int gc = c0 + lo;
         J.spec[(size_t)k * B + gc] = cmul(lds[t], outer_twiddle(J, gc * k));
     }
@@ -416,9 +442,10 @@ lucas_col_kernel(const LucasJobDesc* __restrict__ jobs, const int2* __restrict__
 // which is 4x the packed spectrum of the squared real signal; the 1/4 is
 // folded into the unweight table.
 // ---------------------------------------------------------------------------
-extern "C" __global__ void __launch_bounds__(LUCAS_THREADS)
-lucas_row_kernel(const LucasJobDesc* __restrict__ jobs, const int2* __restrict__ wgmap) {
-    extern __shared__ double2 lds[];
+template <int T>
+__device__ __forceinline__ void row_kernel_body(const LucasJobDesc* __restrict__ jobs,
+                                                const int2* __restrict__ wgmap) {
+    double2* const lds = lucas_dynamic_lds();
     const int2 jm = wgmap[blockIdx.x];
     const LucasJobDesc& J = jobs[jm.x];
     const int A = J.A, B = J.B;
@@ -428,47 +455,57 @@ lucas_row_kernel(const LucasJobDesc* __restrict__ jobs, const int2* __restrict__
     const int r1 = self ? r0 : A - r0;
     const int nrows = self ? 1 : 2;
 
-    for (int t = tid; t < nrows * B; t += LUCAS_THREADS) {
-        const int rl = t / B, kc = t - rl * B;
+    for (int t = tid; t < nrows * B; t += T) {
+        const int rl = t >= B ? 1 : 0, kc = t - rl * B;
         lds[t] = J.spec[(size_t)(rl ? r1 : r0) * B + kc];
     }
-    __syncthreads();
-    fft_lds<-1, false>(lds, B, J.radB, J.nradB, nrows, 1, B, J.twB, tid);
-
-    const double2 wr = J.twMA[r0];
-    if (!self) {
-        for (int kc = tid; kc < B; kc += LUCAS_THREADS) {
-            const int pc = B - 1 - kc;
-            const double2 zk = lds[kc], zm = lds[B + pc];
-            const double2 w = cmul(wr, J.twB[kc]);
-            const double2 E = cadd(zk, cconj(zm));
-            const double2 O = mul_si<-1>(csub(zk, cconj(zm)));
-            const double2 P = cadd(cmul(E, E), cmul(w, cmul(O, O)));
-            const double2 Q = cmul(E, O);
-            lds[kc] = cadd(P, make_double2(-2.0 * Q.y, 2.0 * Q.x));
-            lds[B + pc] = make_double2(P.x + 2.0 * Q.y, -P.y + 2.0 * Q.x);
-        }
-    } else {
-        const int last = (r0 == 0) ? B / 2 : (B - 1) / 2;
-        for (int kc = tid; kc <= last; kc += LUCAS_THREADS) {
-            int pc = (r0 == 0) ? B - kc : B - 1 - kc;
+    // pass 0: forward row FFT; pass 1: inverse row FFT (conjugated input)
+    for (int pass = 0;; ++pass) {
+        __syncthreads();
+        fft_lds<T, -1, false>(lds, B, J.radB, J.nradB, nrows, 1, B, J.twB, tid);
+        if (pass == 1) break;
+        const double2 wr = J.twMA[r0];
+        // pairs (kc in row r0) <-> (pc in row r1): rows differ, or row r0 pairs
+        // with itself (row 0: pc = -kc mod B; row A/2: pc = B-1-kc)
+        const int last = !self ? B - 1 : (r0 == 0 ? B / 2 : (B - 1) / 2);
+        const int poff = self ? 0 : B;
+        for (int kc = tid; kc <= last; kc += T) {
+            int pc = (self && r0 == 0) ? B - kc : B - 1 - kc;
             if (pc == B) pc = 0;
-            const double2 zk = lds[kc], zm = lds[pc];
+            const double2 zk = lds[kc], zm = lds[poff + pc];
             const double2 w = cmul(wr, J.twB[kc]);
             const double2 E = cadd(zk, cconj(zm));
             const double2 O = mul_si<-1>(csub(zk, cconj(zm)));
             const double2 P = cadd(cmul(E, E), cmul(w, cmul(O, O)));
             const double2 Q = cmul(E, O);
-            lds[kc] = cadd(P, make_double2(-2.0 * Q.y, 2.0 * Q.x));
-            if (pc != kc) lds[pc] = make_double2(P.x + 2.0 * Q.y, -P.y + 2.0 * Q.x);
+            // stored conjugated for the inverse transform
+            lds[kc] = make_double2(P.x - 2.0 * Q.y, -(P.y + 2.0 * Q.x));
+            if (poff + pc != kc) lds[poff + pc] = make_double2(P.x + 2.0 * Q.y, P.y - 2.0 * Q.x);
         }
     }
-    __syncthreads();
-    fft_lds<+1, false>(lds, B, J.radB, J.nradB, nrows, 1, B, J.twB, tid);
-    for (int t = tid; t < nrows * B; t += LUCAS_THREADS) {
-        const int rl = t / B, kc = t - rl * B;
-        J.spec2[(size_t)(rl ? r1 : r0) * B + kc] = lds[t];
+    for (int t = tid; t < nrows * B; t += T) {
+        const int rl = t >= B ? 1 : 0, kc = t - rl * B;
+        J.spec2[(size_t)(rl ? r1 : r0) * B + kc] = cconj(lds[t]);
     }
+}
+
+}  // namespace
+
+extern "C" __global__ void __launch_bounds__(256)
+lucas_col_kernel_256(const LucasJobDesc* __restrict__ jobs, const int2* __restrict__ wgmap, int mode) {
+    col_kernel_body<256>(jobs, wgmap, mode);
+}
+extern "C" __global__ void __launch_bounds__(512)
+lucas_col_kernel_512(const LucasJobDesc* __restrict__ jobs, const int2* __restrict__ wgmap, int mode) {
+    col_kernel_body<512>(jobs, wgmap, mode);
+}
+extern "C" __global__ void __launch_bounds__(256)
+lucas_row_kernel_256(const LucasJobDesc* __restrict__ jobs, const int2* __restrict__ wgmap) {
+    row_kernel_body<256>(jobs, wgmap);
+}
+extern "C" __global__ void __launch_bounds__(512)
+lucas_row_kernel_512(const LucasJobDesc* __restrict__ jobs, const int2* __restrict__ wgmap) {
+    row_kernel_body<512>(jobs, wgmap);
 }
 
 // ---------------------------------------------------------------------------
@@ -476,14 +513,21 @@ lucas_row_kernel(const LucasJobDesc* __restrict__ jobs, const int2* __restrict__
 // ---------------------------------------------------------------------------
 namespace lucas {
 
-hipError_t launch_col_kernel(const LucasJobDesc* jobs, const int2* map, int nwg, size_t lds, int mode,
+hipError_t launch_col_kernel(int threads, const LucasJobDesc* jobs, const int2* map, int nwg, size_t lds, int mode,
                              hipStream_t stream) {
-    hipLaunchKernelGGL(lucas_col_kernel, dim3(nwg), dim3(LUCAS_THREADS), lds, stream, jobs, map, mode);
+    if (threads == 512)
+        hipLaunchKernelGGL(lucas_col_kernel_512, dim3(nwg), dim3(512), lds, stream, jobs, map, mode);
+    else
+        hipLaunchKernelGGL(lucas_col_kernel_256, dim3(nwg), dim3(256), lds, stream, jobs, map, mode);
     return hipGetLastError();
 }
 
-hipError_t launch_row_kernel(const LucasJobDesc* jobs, const int2* map, int nwg, size_t lds, hipStream_t stream) {
-    hipLaunchKernelGGL(lucas_row_kernel, dim3(nwg), dim3(LUCAS_THREADS), lds, stream, jobs, map);
+hipError_t launch_row_kernel(int threads, const LucasJobDesc* jobs, const int2* map, int nwg, size_t lds,
+                             hipStream_t stream) {
+    if (threads == 512)
+        hipLaunchKernelGGL(lucas_row_kernel_512, dim3(nwg), dim3(512), lds, stream, jobs, map);
+    else
+        hipLaunchKernelGGL(lucas_row_kernel_256, dim3(nwg), dim3(256), lds, stream, jobs, map);
     return hipGetLastError();
 }
 

@@ -46,7 +46,7 @@ class PlanInfo(C.Structure):
     _fields_ = [("p", C.c_int), ("N", C.c_int), ("q", C.c_int), ("n_wide", C.c_int),
                 ("carry_depth", C.c_int), ("A", C.c_int), ("B", C.c_int), ("C", C.c_int),
                 ("halo", C.c_int), ("col_wgs", C.c_int), ("row_wgs", C.c_int),
-                ("device_bytes", C.c_size_t)]
+                ("threads", C.c_int), ("device_bytes", C.c_size_t)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -93,6 +93,9 @@ _SIGS = {
     "lucas_default_N": (C.c_int, [C.c_int]),
     "lucas_plan": (C.c_int, [C.c_int, C.c_int, _P(PlanInfo)]),
     "lucas_plan_shape": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _P(PlanInfo)]),
+    "lucas_plan_list": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_int, _P(C.c_int)]),
+    "lucas_autotune_shape": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _P(Options), _P(PlanInfo),
+                                       _P(C.c_double)]),
     "lucas_digit_tables": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "lucas_digits_res64": (C.c_int, [C.c_int, C.c_int, C.c_void_p, _P(C.c_uint64), _P(C.c_int), C.c_void_p]),
     "lucas_ctx_create": (C.c_int, [C.c_int, C.c_int, C.c_int, _P(C.c_void_p)]),
@@ -175,6 +178,23 @@ def plan(p: int, N: int = 0, A: int = 0, C_: int = 0) -> dict:
     info = PlanInfo()
     _check(_lib().lucas_plan_shape(p, N, A, C_, C.byref(info)), f"plan(p={p}, N={N})")
     return info.as_dict()
+
+
+def plan_list(p: int, N: int) -> list[dict]:
+    n = C.c_int(0)
+    _check(_lib().lucas_plan_list(p, N, None, 0, C.byref(n)), f"plan_list(p={p}, N={N})")
+    arr = (PlanInfo * n.value)()
+    _check(_lib().lucas_plan_list(p, N, arr, n.value, C.byref(n)), "plan_list")
+    return [a.as_dict() for a in arr]
+
+
+def autotune_shape(p: int, N: int = 0, iters: int = 200, max_candidates: int = 0, device: int = 0) -> tuple[dict, float]:
+    info = PlanInfo()
+    ms = C.c_double(0)
+    opt = _options(device)
+    _check(_lib().lucas_autotune_shape(p, N, iters, max_candidates, C.byref(opt), C.byref(info), C.byref(ms)),
+           f"autotune_shape(p={p}, N={N})")
+    return info.as_dict(), ms.value
 
 
 def digit_tables(p: int, N: int):

@@ -1,0 +1,88 @@
+"""Measure the fastest tiling per signal length and write the shape table.
+
+For each length N the engine's admissible tilings (A x B radix split, tile
+width C, workgroup size) are timed on the GPU with `lucas_autotune_shape`;
+the winners go to gpulucas_amd/csrc/tuned_shapes.inc, which the planner reads
+before its cost model (rebuild the library afterwards).
+
+    python -m gpulucas_amd.tune --out tune.json            # measure
+    python -m gpulucas_amd.tune --emit tune.json [...]     # write the .inc
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+INC = os.path.join(HERE, "csrc", "tuned_shapes.inc")
+
+
+def standard_lengths(serving: bool = True):
+    """(p, N) pairs of the standard workloads: config shapes, batch lengths and
+    (optionally) the serving workload's lengths."""
+    from gpulucas_amd import lucas, workloads as wl
+
+    pairs = {4608: 86243, 65536: 1257787, 163840: 2976221, 49152: 859433}
+    for p in wl.batch_primes() + wl.near3m_primes():
+        pairs.setdefault(lucas.default_N(p), p)
+    if serving:
+        for _, p, _ in wl.serving_jobs(count=64):
+            pairs.setdefault(lucas.default_N(p), p)
+    return sorted((p, N) for N, p in pairs.items())
+
+
+def measure(pairs, iters, max_candidates, out):
+    from gpulucas_amd import lucas
+
+    results = {}
+    for p, N in pairs:
+        t0 = time.time()
+        info, ms = lucas.autotune_shape(p, N, iters=iters, max_candidates=max_candidates)
+        base = lucas.plan(p, N)
+        results[str(N)] = {"p": p, "N": N, "A": info["A"], "C": info["C"], "threads": info["threads"],
+                           "ms_per_iter": ms, "model_A": base["A"], "model_C": base["C"]}
+        print(f"N={N} p={p}: A={info['A']} C={info['C']} T={info['threads']} {ms * 1e3:.2f} us/iter "
+              f"({time.time() - t0:.1f}s)", flush=True)
+        with open(out, "w") as f:
+            json.dump(results, f, indent=1)
+    return results
+
+
+def emit(paths):
+    table = {}
+    for path in paths:
+        with open(path) as f:
+            table.update(json.load(f))
+    rows = sorted((int(v["N"]), int(v["A"]), int(v["C"]), float(v["ms_per_iter"])) for v in table.values())
+    with open(INC, "w") as f:
+        f.write("// Generated by gpulucas_amd/tune.py from GPU measurements (MI355X):\n")
+        f.write("// signal length N -> fastest column length A and tile width C.\n")
+        for N, A, C, ms in rows:
+            f.write(f"{{{N}, {A}, {C}}},  // {ms * 1e3:.2f} us/iteration\n")
+    print(f"wrote {len(rows)} entries to {INC}")
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=None)
+    ap.add_argument("--emit", nargs="*", default=None)
+    ap.add_argument("--iters", type=int, default=300)
+    ap.add_argument("--max-candidates", type=int, default=60)
+    ap.add_argument("--lengths", default=None, help="comma list of p:N pairs")
+    ap.add_argument("--no-serving", action="store_true")
+    a = ap.parse_args(argv)
+    if a.emit is not None:
+        emit(a.emit)
+        return
+    if a.lengths:
+        pairs = [tuple(int(x) for x in s.split(":")) for s in a.lengths.split(",")]
+    else:
+        pairs = standard_lengths(serving=not a.no_serving)
+    measure(pairs, a.iters, a.max_candidates, a.out or "tune.json")
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -55,6 +55,7 @@ typedef struct {
     int C;             /* columns per column workgroup */
     int halo;          /* halo columns recomputed per column workgroup */
     int col_wgs, row_wgs;
+    int threads;       /* work-items per workgroup */
     size_t device_bytes; /* device footprint of one job */
 } lucas_plan_info;
 
@@ -89,6 +90,9 @@ int lucas_default_N(int p);
 int lucas_plan(int p, int N, lucas_plan_info* out);
 /* Same with an explicit column length A and tile width C (0 = automatic). */
 int lucas_plan_shape(int p, int N, int A, int C, lucas_plan_info* out);
+/* Every admissible (A, C) tiling of (p, N), cheapest modelled first; fills up
+ * to `cap` entries and writes the total into *count. */
+int lucas_plan_list(int p, int N, lucas_plan_info* out, int cap, int* count);
 /* Per-digit widths and weights as the reference computes them
  * (computeBitsPerWord gpuLucas.cu:688-694, computeWeightVectors 673-686):
  * widths[N], weights[N] = 2^(ceil(jp/N) - jp/N); any pointer may be NULL. */
@@ -144,6 +148,13 @@ int lucas_error_test(int p, int N, int iterations, const lucas_options* opt, luc
  * `iters` iterations per candidate and keeps the fastest with roundoff below
  * the limit.  Writes the chosen N and its plan. */
 int lucas_autotune_N(int p, int iters, const lucas_options* opt, lucas_plan_info* out);
+
+/* Measured choice of tiling (radix split A x B, tile width C, workgroup
+ * size) for (p, N): times `iters` iterations of up to `max_candidates`
+ * tilings and keeps the fastest (replaces the hand-set signal-size and
+ * slice constants, gpuLucas.cu:159-173, 323-343).  *best_ms = ms/iteration. */
+int lucas_autotune_shape(int p, int N, int iters, int max_candidates, const lucas_options* opt,
+                         lucas_plan_info* out, double* best_ms);
 
 /* ---- batched exponents (no reference counterpart: the reference runs one
  * exponent per process).  All jobs advance together, one launch of each

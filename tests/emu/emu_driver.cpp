@@ -10,7 +10,7 @@
 
 thread_local emu_idx threadIdx, blockIdx;
 emu_idx blockDim;
-double2 lds[LUCAS_EMAX * LUCAS_THREADS];
+double2 lucas_lds[65536 / sizeof(double2)];
 
 namespace emu {
 std::barrier<>* bar = nullptr;
@@ -38,8 +38,8 @@ void run_grid(unsigned nblocks, unsigned nthreads, const std::function<void()>& 
 }  // namespace emu
 
 namespace lucas {
-hipError_t launch_col_kernel(const LucasJobDesc*, const int2*, int, size_t, int, hipStream_t);
-hipError_t launch_row_kernel(const LucasJobDesc*, const int2*, int, size_t, hipStream_t);
+hipError_t launch_col_kernel(int, const LucasJobDesc*, const int2*, int, size_t, int, hipStream_t);
+hipError_t launch_row_kernel(int, const LucasJobDesc*, const int2*, int, size_t, hipStream_t);
 }
 
 extern "C" int emu_run(int p, int N, int A, int C, const int32_t* digits_in, long long iters, int32_t* digits_out,
@@ -55,6 +55,7 @@ extern "C" int emu_run(int p, int N, int A, int C, const int32_t* digits_in, lon
     LucasJobDesc d{};
     d.p = pl.p; d.N = pl.N; d.M = pl.M; d.A = pl.A; d.B = pl.B; d.q = pl.q; d.b = pl.b;
     d.ncarry = pl.ncarry; d.halo = pl.halo; d.C = pl.C; d.col_wgs = pl.col_wgs; d.row_wgs = pl.row_wgs;
+    d.threads = pl.threads;
     d.nradA = (int)pl.radA.size(); d.nradB = (int)pl.radB.size();
     for (int i = 0; i < d.nradA; ++i) d.radA[i] = pl.radA[i];
     for (int i = 0; i < d.nradB; ++i) d.radB[i] = pl.radB[i];
@@ -67,10 +68,10 @@ extern "C" int emu_run(int p, int N, int A, int C, const int32_t* digits_in, lon
     for (int w = 0; w < pl.col_wgs; ++w) cmap.push_back({0, w});
     for (int w = 0; w < pl.row_wgs; ++w) rmap.push_back({0, w});
     if (iters > 0) {
-        launch_col_kernel(&d, cmap.data(), pl.col_wgs, 0, LUCAS_IN_DIGITS, nullptr);
+        launch_col_kernel(pl.threads, &d, cmap.data(), pl.col_wgs, 0, LUCAS_IN_DIGITS, nullptr);
         for (long long i = 0; i < iters; ++i) {
-            launch_row_kernel(&d, rmap.data(), pl.row_wgs, 0, nullptr);
-            launch_col_kernel(&d, cmap.data(), pl.col_wgs, 0, i + 1 == iters ? LUCAS_OUT_DIGITS : 0, nullptr);
+            launch_row_kernel(pl.threads, &d, rmap.data(), pl.row_wgs, 0, nullptr);
+            launch_col_kernel(pl.threads, &d, cmap.data(), pl.col_wgs, 0, i + 1 == iters ? LUCAS_OUT_DIGITS : 0, nullptr);
         }
     }
     std::memcpy(digits_out, dig.data(), sizeof(int32_t) * N);
