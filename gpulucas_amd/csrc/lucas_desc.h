@@ -19,15 +19,26 @@
 // Elements (complex words) each work-item may hold while an FFT stage is in
 // flight; a tile of a workgroup with T work-items holds at most EMAX*T words.
 #define LUCAS_EMAX 8
-// Work-items per workgroup: kernels are built for 256 and 512.
+// Work-items per workgroup: kernels are built for 256, 512 and 1024.
 #define LUCAS_THREADS_SMALL 256
 #define LUCAS_THREADS_LARGE 512
+#define LUCAS_THREADS_MAX 1024
 // Deepest carry-save restoration supported (reference carry depth n <= 6).
 #define LUCAS_NMAX 6
 
 // Column-kernel modes.
 #define LUCAS_IN_DIGITS 1   // start from int32 digits instead of spec2
 #define LUCAS_OUT_DIGITS 2  // stop after the carry and write int32 digits
+
+// One Stockham stage of a length-L transform (L = product of the radices):
+// radix R after Ns = product of the previous radices; nb = L/R butterflies
+// per sequence; twiddle index step L/(Ns*R).  mNs and mnb are multiply-high
+// reciprocals: q = umulhi(t, m) = t / d for t, d < 2^16 (m = ceil(2^32/d),
+// unused when d == 1).
+struct LucasStage {
+    int R, Ns, nb, twstep;
+    unsigned int mNs, mnb;
+};
 
 struct LucasJobDesc {
     int p, N, M, A, B;      // exponent, signal length, complex length, M = A*B
@@ -36,10 +47,12 @@ struct LucasJobDesc {
     int halo;               // halo columns needed by the carry: ceil((n+1)/2)
     int C;                  // columns per column workgroup (B % C == 0)
     int col_wgs, row_wgs;   // workgroups per launch of each kernel
-    int threads;            // work-items per workgroup (256 or 512)
+    int threads;            // work-items per workgroup (256, 512 or 1024)
     int nradA, nradB;
-    int radA[LUCAS_MAX_RADICES];
-    int radB[LUCAS_MAX_RADICES];
+    LucasStage stA[LUCAS_MAX_RADICES];  // column transform stages (length A)
+    LucasStage stB[LUCAS_MAX_RADICES];  // row transform stages (length B)
+    unsigned int m_ncl, m_C;             // reciprocals of C+halo and C
+    double invB;                         // 1/B
     // Twiddle tables (forward sign, exp(-2*pi*i*t/L)), rounded from long double.
     const double2* twA;     // omega_A^t, t < A
     const double2* twB;     // omega_B^t, t < B

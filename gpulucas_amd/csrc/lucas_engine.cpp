@@ -86,23 +86,7 @@ int job_create(const Plan& plan, int device, hipStream_t stream, Job** out) {
         return LUCAS_ERR_DEVICE;
     }
     LucasJobDesc& d = job->desc;
-    d.p = plan.p;
-    d.N = plan.N;
-    d.M = plan.M;
-    d.A = plan.A;
-    d.B = plan.B;
-    d.q = plan.q;
-    d.b = plan.b;
-    d.ncarry = plan.ncarry;
-    d.halo = plan.halo;
-    d.C = plan.C;
-    d.col_wgs = plan.col_wgs;
-    d.row_wgs = plan.row_wgs;
-    d.threads = plan.threads;
-    d.nradA = (int)plan.radA.size();
-    d.nradB = (int)plan.radB.size();
-    for (int i = 0; i < d.nradA; ++i) d.radA[i] = plan.radA[i];
-    for (int i = 0; i < d.nradB; ++i) d.radB[i] = plan.radB[i];
+    fill_desc_shape(plan, &d);
     char* m = job->mem;
     d.spec = (double2*)(m + o_spec);
     d.spec2 = (double2*)(m + o_spec2);
@@ -207,13 +191,14 @@ int Engine::prepare() {
         graph_ = nullptr;
     }
     std::vector<LucasJobDesc> desc;
-    std::vector<int2> cmap[2], rmap[2];
+    std::vector<int2> cmap[3], rmap[3];
     groups_[0].threads = LUCAS_THREADS_SMALL;
     groups_[1].threads = LUCAS_THREADS_LARGE;
+    groups_[2].threads = LUCAS_THREADS_MAX;
     for (Group& g : groups_) g.lds_col = g.lds_row = 0;
     for (size_t i = 0; i < jobs_.size(); ++i) {
         const Job* j = jobs_[i];
-        const int gi = j->plan.threads == LUCAS_THREADS_LARGE ? 1 : 0;
+        const int gi = j->plan.threads == LUCAS_THREADS_MAX ? 2 : (j->plan.threads == LUCAS_THREADS_LARGE ? 1 : 0);
         Group& g = groups_[gi];
         desc.push_back(j->desc);
         for (int w = 0; w < j->plan.col_wgs; ++w) cmap[gi].push_back(make_int2((int)i, w));
@@ -235,7 +220,7 @@ int Engine::prepare() {
     hipError_t e = hipSuccess;
     if (!desc.empty())
         e = hipMemcpyAsync(d_desc_, desc.data(), sizeof(LucasJobDesc) * desc.size(), hipMemcpyHostToDevice, stream_);
-    for (int gi = 0; gi < 2 && e == hipSuccess; ++gi) {
+    for (int gi = 0; gi < 3 && e == hipSuccess; ++gi) {
         Group& g = groups_[gi];
         if (!grow((void**)&g.d_colmap, &g.cap_col, sizeof(int2) * (cmap[gi].size() + 1)) ||
             !grow((void**)&g.d_rowmap, &g.cap_row, sizeof(int2) * (rmap[gi].size() + 1)))

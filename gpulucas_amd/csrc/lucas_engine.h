@@ -72,7 +72,7 @@ private:
     bool dirty_ = true;
     LucasJobDesc* d_desc_ = nullptr;
     size_t cap_desc_ = 0;
-    Group groups_[2];
+    Group groups_[3];
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t gexec_ = nullptr;
     int graph_iters_ = 0;

@@ -116,13 +116,13 @@ std::vector<Plan> plan_candidates(int p, int N) {
         for (int C : {1, 2, 4, 8, 16, 32, 64}) {
             if (B % C) continue;
             const long long tile = std::max<long long>((long long)A * (C + pl.halo), 2LL * B);
-            int T = 0;
-            if (tile <= (long long)LUCAS_EMAX * LUCAS_THREADS_SMALL) T = LUCAS_THREADS_SMALL;
-            else if (tile <= (long long)LUCAS_EMAX * LUCAS_THREADS_LARGE) T = LUCAS_THREADS_LARGE;
-            else continue;
             const size_t lds_col = sizeof(double) * 2 * (size_t)A * (C + pl.halo) + sizeof(int) * 2 * (size_t)A * C;
             const size_t lds_row = sizeof(double) * 2 * 2 * (size_t)B;
             if (lds_col > kMaxLds || lds_row > kMaxLds) continue;
+            for (int T : {LUCAS_THREADS_SMALL, LUCAS_THREADS_LARGE, LUCAS_THREADS_MAX}) {
+            if (tile > (long long)LUCAS_EMAX * T) continue;
+            // a workgroup more than twice the tile would leave most work-items idle
+            if (T > LUCAS_THREADS_SMALL && tile < T / 2) continue;
             Plan c = pl;
             c.A = A;
             c.B = B;
@@ -135,6 +135,7 @@ std::vector<Plan> plan_candidates(int p, int N) {
             c.lds_col = lds_col;
             c.lds_row = lds_row;
             out.push_back(c);
+            }
         }
     }
     std::stable_sort(out.begin(), out.end(), [](const Plan& a, const Plan& b) { return model_cost(a) < model_cost(b); });
@@ -168,6 +169,48 @@ int default_length(int p) {
         if (make_plan(p, (int)N, 0, 0, &pl).empty()) return (int)N;
     }
     return 0;
+}
+
+unsigned int magic_u32(unsigned int d) {
+    if (d <= 1) return 0;
+    return (unsigned int)(((1ULL << 32) + d - 1) / d);
+}
+
+void stage_table(int L, const std::vector<int>& rad, LucasStage* out) {
+    int Ns = 1;
+    for (size_t s = 0; s < rad.size(); ++s) {
+        LucasStage& st = out[s];
+        st.R = rad[s];
+        st.Ns = Ns;
+        st.nb = L / rad[s];
+        st.twstep = L / (Ns * rad[s]);
+        st.mNs = magic_u32((unsigned)Ns);
+        st.mnb = magic_u32((unsigned)st.nb);
+        Ns *= rad[s];
+    }
+}
+
+void fill_desc_shape(const Plan& plan, LucasJobDesc* d) {
+    d->p = plan.p;
+    d->N = plan.N;
+    d->M = plan.M;
+    d->A = plan.A;
+    d->B = plan.B;
+    d->q = plan.q;
+    d->b = plan.b;
+    d->ncarry = plan.ncarry;
+    d->halo = plan.halo;
+    d->C = plan.C;
+    d->col_wgs = plan.col_wgs;
+    d->row_wgs = plan.row_wgs;
+    d->threads = plan.threads;
+    d->nradA = (int)plan.radA.size();
+    d->nradB = (int)plan.radB.size();
+    stage_table(plan.A, plan.radA, d->stA);
+    stage_table(plan.B, plan.radB, d->stB);
+    d->m_ncl = magic_u32((unsigned)(plan.C + plan.halo));
+    d->m_C = magic_u32((unsigned)plan.C);
+    d->invB = 1.0 / (double)plan.B;
 }
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }

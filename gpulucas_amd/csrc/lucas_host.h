@@ -8,6 +8,8 @@
 #include <string>
 #include <vector>
 
+#include "lucas_desc.h"
+
 namespace lucas {
 
 struct Plan {
@@ -37,6 +39,13 @@ std::vector<Plan> plan_candidates(int p, int N);
 int default_length(int p);
 // Bits per word allowed at length N by the default rule.
 double max_bits_per_word(int N);
+
+// Multiply-high reciprocal of d: umulhi(t, m) == t / d for t, d < 2^16.
+unsigned int magic_u32(unsigned int d);
+// Stage table of a length-L transform with radix sequence `rad`.
+void stage_table(int L, const std::vector<int>& rad, LucasStage* out);
+// Fill every descriptor field that depends only on the plan (not on memory).
+void fill_desc_shape(const Plan& plan, LucasJobDesc* d);
 
 // Device footprint of one job under `plan` (state + tables), bytes.
 size_t job_bytes(const Plan& plan);

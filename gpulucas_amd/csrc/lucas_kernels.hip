@@ -24,6 +24,35 @@
 
 #include "lucas_desc.h"
 
+// Pointers read from a job descriptor are generic; loading through a global
+// address-space cast lets the compiler emit global (not flat) accesses, which
+// do not share the LDS wait counter.  Complex words move as 16-byte vectors.
+#ifndef LUCAS_GLOBAL
+#define LUCAS_GLOBAL __attribute__((address_space(1)))
+#endif
+typedef double lucas_v2d __attribute__((vector_size(16)));
+typedef int lucas_v2i __attribute__((vector_size(8)));
+template <typename T>
+__device__ __forceinline__ T gload(const T* p, size_t i) {
+    return ((const LUCAS_GLOBAL T*)p)[i];
+}
+__device__ __forceinline__ double2 gload2(const double2* p, size_t i) {
+    const lucas_v2d v = ((const LUCAS_GLOBAL lucas_v2d*)p)[i];
+    return make_double2(v[0], v[1]);
+}
+__device__ __forceinline__ void gstore2(double2* p, size_t i, double2 x) {
+    const lucas_v2d v = {x.x, x.y};
+    ((LUCAS_GLOBAL lucas_v2d*)p)[i] = v;
+}
+__device__ __forceinline__ int2 gload2i(const int* p, size_t i) {
+    const lucas_v2i v = ((const LUCAS_GLOBAL lucas_v2i*)p)[i];
+    return make_int2(v[0], v[1]);
+}
+__device__ __forceinline__ void gstore2i(int* p, size_t i, int2 x) {
+    const lucas_v2i v = {x.x, x.y};
+    ((LUCAS_GLOBAL lucas_v2i*)p)[i] = v;
+}
+
 // Dynamic LDS of the running workgroup (sized per launch).
 __device__ __forceinline__ double2* lucas_dynamic_lds() {
     extern __shared__ double2 lucas_lds[];
@@ -49,20 +78,10 @@ __device__ __forceinline__ double2 mul_si(double2 a) {
     return SIGN < 0 ? make_double2(a.y, -a.x) : make_double2(-a.y, a.x);
 }
 
-// Quotient t / d for 0 <= t < 2^22, 0 < d < 2^13: float reciprocal estimate
-// (error below one) and one integer correction.  Avoids the long integer
-// division sequence for every index split.
-struct FDiv {
-    int d;
-    float r;
-};
-__device__ __forceinline__ FDiv fdiv_make(int d) { return FDiv{d, 1.0f / (float)d}; }
-__device__ __forceinline__ int fdiv(int t, FDiv f) {
-    int q = (int)((float)t * f.r);
-    const int rem = t - q * f.d;
-    if (rem >= f.d) ++q;
-    else if (rem < 0) --q;
-    return q;
+// Quotient t / d for 0 <= t, d < 2^16 from a host-computed multiply-high
+// reciprocal m = ceil(2^32 / d) (d == 1 has no 32-bit reciprocal).
+__device__ __forceinline__ int udiv(int t, int d, unsigned int m) {
+    return d == 1 ? t : (int)__umulhi((unsigned int)t, m);
 }
 
 // ---------------------------------------------------------------------------
@@ -177,14 +196,11 @@ template <int SIGN> struct Dft<8, SIGN> {
 // consecutive sequences (column tiles) instead of consecutive butterflies.
 // ---------------------------------------------------------------------------
 template <int T, int R, int SIGN, bool SEQ_FAST>
-__device__ __forceinline__ void fft_stage(double2* lds, int L, int Ns, int nseq, int sIdx, int sSeq,
-                                          const double2* __restrict__ tw, int tid) {
+__device__ __forceinline__ void fft_stage(double2* lds, const LucasStage& st, int nseq, unsigned int mseq, int sIdx,
+                                          int sSeq, const double2* __restrict__ tw, int tid) {
     constexpr int BM = (LUCAS_EMAX + R - 1) / R;
-    const int nb = L / R;
+    const int Ns = st.Ns, nb = st.nb, twstep = st.twstep;
     const int total = nb * nseq;
-    const int twstep = L / (Ns * R);
-    const FDiv fsplit = fdiv_make(SEQ_FAST ? nseq : nb);
-    const FDiv fns = fdiv_make(Ns);
     double2 v[BM][R];
     int outb[BM];
 #pragma unroll
@@ -192,22 +208,22 @@ __device__ __forceinline__ void fft_stage(double2* lds, int L, int Ns, int nseq,
         const int t = tid + i * T;
         if (t < total) {
             int seq, j;
-            const int qq = fdiv(t, fsplit);
             if (SEQ_FAST) {
-                j = qq;
-                seq = t - qq * nseq;
+                j = udiv(t, nseq, mseq);
+                seq = t - j * nseq;
             } else {
-                seq = qq;
-                j = t - qq * nb;
+                seq = udiv(t, nb, st.mnb);
+                j = t - seq * nb;
             }
-            const int jq = fdiv(j, fns);
+            const int jq = udiv(j, Ns, st.mNs);
             const int k = j - jq * Ns;
             const int base = seq * sSeq;
+            const int kt = k * twstep;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 double2 x = lds[base + (j + r * nb) * sIdx];
                 if (r > 0 && Ns > 1) {
-                    const double2 w = tw[r * k * twstep];
+                    const double2 w = gload2(tw, r * kt);
                     x = SIGN < 0 ? cmul(x, w) : cmulc(x, w);
                 }
                 v[i][r] = x;
@@ -232,28 +248,35 @@ __device__ __forceinline__ void fft_stage(double2* lds, int L, int Ns, int nseq,
 // kernels conjugate on the way in and out and reach this code from one call
 // site each (one inlined copy per kernel).
 template <int T, int SIGN, bool SEQ_FAST>
-__device__ __forceinline__ void fft_lds(double2* lds, int L, const int* rad, int nrad, int nseq, int sIdx, int sSeq,
-                                     const double2* __restrict__ tw, int tid) {
-    int Ns = 1;
-    for (int s = 0; s < nrad; ++s) {
-        const int R = rad[s];
-        switch (R) {
-            case 2: fft_stage<T, 2, SIGN, SEQ_FAST>(lds, L, Ns, nseq, sIdx, sSeq, tw, tid); break;
-            case 3: fft_stage<T, 3, SIGN, SEQ_FAST>(lds, L, Ns, nseq, sIdx, sSeq, tw, tid); break;
-            case 4: fft_stage<T, 4, SIGN, SEQ_FAST>(lds, L, Ns, nseq, sIdx, sSeq, tw, tid); break;
-            case 5: fft_stage<T, 5, SIGN, SEQ_FAST>(lds, L, Ns, nseq, sIdx, sSeq, tw, tid); break;
-            case 7: fft_stage<T, 7, SIGN, SEQ_FAST>(lds, L, Ns, nseq, sIdx, sSeq, tw, tid); break;
-            default: fft_stage<T, 8, SIGN, SEQ_FAST>(lds, L, Ns, nseq, sIdx, sSeq, tw, tid); break;
+__device__ __forceinline__ void fft_lds(double2* lds, const LucasStage* stages, int nst, int nseq, unsigned int mseq,
+                                        int sIdx, int sSeq, const double2* __restrict__ tw, int tid) {
+    for (int s = 0; s < nst; ++s) {
+        const LucasStage st = stages[s];
+        switch (st.R) {
+            case 2: fft_stage<T, 2, SIGN, SEQ_FAST>(lds, st, nseq, mseq, sIdx, sSeq, tw, tid); break;
+            case 3: fft_stage<T, 3, SIGN, SEQ_FAST>(lds, st, nseq, mseq, sIdx, sSeq, tw, tid); break;
+            case 4: fft_stage<T, 4, SIGN, SEQ_FAST>(lds, st, nseq, mseq, sIdx, sSeq, tw, tid); break;
+            case 5: fft_stage<T, 5, SIGN, SEQ_FAST>(lds, st, nseq, mseq, sIdx, sSeq, tw, tid); break;
+            case 7: fft_stage<T, 7, SIGN, SEQ_FAST>(lds, st, nseq, mseq, sIdx, sSeq, tw, tid); break;
+            default: fft_stage<T, 8, SIGN, SEQ_FAST>(lds, st, nseq, mseq, sIdx, sSeq, tw, tid); break;
         }
-        Ns *= R;
     }
 }
 
-// omega_M^e for 0 <= e < M, as omega_A^(e / B) * omega_M^(e mod B).
-__device__ __forceinline__ double2 outer_twiddle(const LucasJobDesc& J, int e) {
-    const int hi = e / J.B;
-    const int lo = e - hi * J.B;
-    return cmul(J.twA[hi], J.twMB[lo]);
+// omega_M^e for 0 <= e < M, as omega_A^(e / B) * omega_M^(e mod B); the
+// quotient comes from a double reciprocal of B plus one correction.
+__device__ __forceinline__ double2 outer_twiddle(const LucasJobDesc& J, const double2* twA, const double2* twMB,
+                                                 int e) {
+    int hi = (int)((double)e * J.invB);
+    int lo = e - hi * J.B;
+    if (lo < 0) {
+        --hi;
+        lo += J.B;
+    } else if (lo >= J.B) {
+        ++hi;
+        lo -= J.B;
+    }
+    return cmul(gload2(twA, hi), gload2(twMB, lo));
 }
 
 __device__ __forceinline__ void block_max_err(float err, unsigned int* dst) {
@@ -264,24 +287,23 @@ __device__ __forceinline__ void block_max_err(float err, unsigned int* dst) {
 }
 
 // Carry-save restoration of one digit (reference llintToIrrBal<n>,
-// IrrBaseBalanced.cu:52-110).  The window is stored top-down: cv[i] is the
-// rounded product of digit D-i and w[i] its width, i = 0..n, so every index
-// is a compile-time constant once the loops unroll.  Returns the balanced
-// digit and writes the carry into *carry.
-__device__ __forceinline__ int restore_digit(const long long* cv, const int* w, int n, int* carry) {
+// IrrBaseBalanced.cu:52-110) at compile-time depth NC.  The window is stored
+// top-down: mag[i], neg[i] are |c| and sign of the rounded product of digit
+// D-i, w[i] its width, i = 0..NC.  Returns the balanced digit and writes the
+// carry into *carry.
+template <int NC>
+__device__ __forceinline__ int restore_digit(const unsigned long long* mag, const bool* neg, const int* w,
+                                             int* carry) {
     const int W = w[0];
-    const long long mask = (1LL << W) - 1;
+    const unsigned int mask = (1u << W) - 1u;
     int sum = 0;
     int shift = 0;
 #pragma unroll
-    for (int i = 0; i <= LUCAS_NMAX; ++i) {
-        if (i <= n) {
-            const long long c = cv[i];
-            if (i > 0) shift += w[i];
-            const unsigned long long mag = c < 0 ? (unsigned long long)(-c) : (unsigned long long)c;
-            const int part = shift < 64 ? (int)((mag >> shift) & (unsigned long long)mask) : 0;
-            sum += c < 0 ? -part : part;
-        }
+    for (int i = 0; i <= NC; ++i) {
+        if (i > 0) shift += w[i];
+        // fields at or above bit 64 are zero (the reference's shift is undefined there)
+        const int part = shift < 64 ? (int)((unsigned int)(mag[i] >> shift) & mask) : 0;
+        sum += neg[i] ? -part : part;
     }
     const int half = 1 << (W - 1);
     int h = 0;
@@ -291,6 +313,51 @@ __device__ __forceinline__ int restore_digit(const long long* cv, const int* w, 
         h = (sum + half) >> W;
     *carry = h;
     return sum - h * (1 << W);
+}
+
+// New digits (2m, 2m+1) of element m = (row r, local column lc, global
+// column gc) from the rounded products in the tile: restoration of digits
+// 2m+1, 2m, 2m-1 and the single ripple x_j = d_j + h_{j-1}
+// (loadIntToDoubleIBDWT, gpuLucas.cu:730-766).
+template <int NC>
+__device__ __forceinline__ int2 carry_element(const double2* lds, const LucasJobDesc& J, int r, int lc, int gc,
+                                              int ncl) {
+    const int A = J.A, N = J.N, q = J.q, bb = J.b;
+    unsigned long long mag[NC + 3];
+    bool neg[NC + 3];
+    int w[NC + 3];
+    // products and widths of digits 2m+1-k, k = 0 .. NC+2 (top-down)
+#pragma unroll
+    for (int k = 0; k < NC + 3; ++k) {
+        const int ie = k >> 1;  // element offset back
+        int rr = r;
+        if (gc - ie < 0) rr = (r == 0) ? A - 1 : r - 1;
+        const double2 v = lds[rr * ncl + (lc - ie)];
+        const long long c = (long long)((k & 1) ? v.x : v.y);
+        neg[k] = c < 0;
+        mag[k] = c < 0 ? (unsigned long long)(-c) : (unsigned long long)c;
+    }
+    // digit j is wide iff r_j < b, r_{j-1} = r_j + b (mod N)
+    int r2m = gload(J.rlo, 2 * gc) + gload(J.rhi, r);
+    if (r2m >= N) r2m -= N;
+    int rv = r2m - bb;
+    if (rv < 0) rv += N;
+    w[0] = q + (rv < bb ? 1 : 0);  // digit 2m+1
+    rv = r2m;
+#pragma unroll
+    for (int k = 1; k < NC + 3; ++k) {
+        if (k > 1) {
+            rv += bb;
+            if (rv >= N) rv -= N;
+        }
+        w[k] = q + (rv < bb ? 1 : 0);  // digit 2m+1-k
+    }
+    int hm1, h0, h1;
+    const int d1 = restore_digit<NC>(mag, neg, w, &h1);              // digit 2m+1
+    const int d0 = restore_digit<NC>(mag + 1, neg + 1, w + 1, &h0);  // digit 2m
+    (void)restore_digit<NC>(mag + 2, neg + 2, w + 2, &hm1);          // digit 2m-1: carry only
+    (void)h1;
+    return make_int2(d0 + hm1, d1 + h0);
 }
 
 // ---------------------------------------------------------------------------
@@ -304,12 +371,14 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc* __restrict__
     const int2 jm = wgmap[blockIdx.x];
     const LucasJobDesc& J = jobs[jm.x];
     const int A = J.A, B = J.B, C = J.C, H = J.halo, N = J.N;
-    const int n = J.ncarry, q = J.q, bb = J.b;
+    const int n = J.ncarry;
     const int c0 = jm.y * C;
     const int tid = threadIdx.x;
     const int ncl = C + H;  // local columns: [0,H) halo, [H, H+C) own
-    const FDiv fncl = fdiv_make(ncl), fC = fdiv_make(C);
+    const unsigned int mncl = J.m_ncl, mC = J.m_C;
     int2* const xdig = reinterpret_cast<int2*>(lds + A * ncl);  // new digits (2m, 2m+1)
+    const double2* const twA = J.twA;
+    const double2* const twMB = J.twMB;
     float err = 0.0f;
 
     // Pass 0 (inverse column FFT of the tile + halo, unweight, round, carry)
@@ -318,17 +387,17 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc* __restrict__
     int pass = (mode & LUCAS_IN_DIGITS) ? 1 : 0;
     if (pass == 1) {
         for (int t = tid; t < A * C; t += T) {
-            const int r = fdiv(t, fC), lo = t - r * C;
-            xdig[t] = reinterpret_cast<const int2*>(J.digits)[(size_t)(c0 + lo) + (size_t)B * r];
+            const int r = udiv(t, C, mC), lo = t - r * C;
+            xdig[t] = gload2i(J.digits, (size_t)(c0 + lo) + (size_t)B * r);
         }
     } else {
         // load the tile with the inverse outer twiddle omega_M^(-c*r), conjugated
         for (int t = tid; t < A * ncl; t += T) {
-            const int r = fdiv(t, fncl), lc = t - r * ncl;
+            const int r = udiv(t, ncl, mncl), lc = t - r * ncl;
             int gc = c0 - H + lc;
             if (gc < 0) gc += B;
-            const double2 v = J.spec2[(size_t)r * B + gc];
-            lds[t] = cconj(cmulc(v, outer_twiddle(J, gc * r)));
+            const double2 v = gload2(J.spec2, (size_t)r * B + gc);
+            lds[t] = cconj(cmulc(v, outer_twiddle(J, twA, twMB, gc * r)));
         }
     }
     for (;;) {
@@ -336,33 +405,33 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc* __restrict__
             __syncthreads();
             // weight: y_j = 2^(r_j/N) x_j into a compact C-column tile
             for (int t = tid; t < A * C; t += T) {
-                const int r = fdiv(t, fC), lo = t - r * C;
+                const int r = udiv(t, C, mC), lo = t - r * C;
                 const int gc = c0 + lo;
-                const int rh = J.rhi[r];
-                const double wh = J.whi[r];
+                const int rh = gload(J.rhi, r);
+                const double wh = gload(J.whi, r);
                 const int je = 2 * gc, jo = je + 1;
-                const double ae = J.wlo[je] * wh * (J.rlo[je] + rh >= N ? 0.5 : 1.0);
-                const double ao = J.wlo[jo] * wh * (J.rlo[jo] + rh >= N ? 0.5 : 1.0);
+                const double ae = gload(J.wlo, je) * wh * (gload(J.rlo, je) + rh >= N ? 0.5 : 1.0);
+                const double ao = gload(J.wlo, jo) * wh * (gload(J.rlo, jo) + rh >= N ? 0.5 : 1.0);
                 const int2 x = xdig[t];
                 lds[t] = make_double2(ae * (double)x.x, ao * (double)x.y);
             }
         }
         __syncthreads();
         const int nseq = pass ? C : ncl;
-        fft_lds<T, -1, true>(lds, A, J.radA, J.nradA, nseq, nseq, 1, J.twA, tid);
+        fft_lds<T, -1, true>(lds, J.stA, J.nradA, nseq, pass ? mC : mncl, nseq, 1, twA, tid);
         if (pass == 1) break;
 
         // unweight, -2 on digit 0, round half-even, roundoff
         for (int t = tid; t < A * ncl; t += T) {
-            const int r = fdiv(t, fncl), lc = t - r * ncl;
+            const int r = udiv(t, ncl, mncl), lc = t - r * ncl;
             int gc = c0 - H + lc;
             if (gc < 0) gc += B;
             const double2 v = cconj(lds[t]);
-            const int rh = J.rhi[r];
-            const double iwh = J.iwhi[r];
+            const int rh = gload(J.rhi, r);
+            const double iwh = gload(J.iwhi, r);
             const int je = 2 * gc, jo = je + 1;
-            const double ie = J.iwlo[je] * iwh * (J.rlo[je] + rh >= N ? 2.0 : 1.0);
-            const double io = J.iwlo[jo] * iwh * (J.rlo[jo] + rh >= N ? 2.0 : 1.0);
+            const double ie = gload(J.iwlo, je) * iwh * (gload(J.rlo, je) + rh >= N ? 2.0 : 1.0);
+            const double io = gload(J.iwlo, jo) * iwh * (gload(J.rlo, jo) + rh >= N ? 2.0 : 1.0);
             double ve = v.x * ie;
             const double vo = v.y * io;
             if (gc == 0 && r == 0) ve -= 2.0;
@@ -373,50 +442,23 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc* __restrict__
         __syncthreads();
         // carry-save restoration + single ripple for the own columns
         for (int t = tid; t < A * C; t += T) {
-            const int r = fdiv(t, fC), lo = t - r * C;
+            const int r = udiv(t, C, mC), lo = t - r * C;
             const int lc = lo + H, gc = c0 + lo;
-            // products and widths of digits 2m+1-k, k = 0 .. n+2 (top-down)
-            long long cv[LUCAS_NMAX + 3];
-            int w[LUCAS_NMAX + 3];
-#pragma unroll
-            for (int k = 0; k < LUCAS_NMAX + 3; ++k) {
-                if (k <= n + 2) {
-                    const int ie = k >> 1;  // element offset back
-                    int rr = r;
-                    if (gc - ie < 0) rr = (r == 0) ? A - 1 : r - 1;
-                    const double2 v = lds[rr * ncl + (lc - ie)];
-                    cv[k] = (long long)((k & 1) ? v.x : v.y);
-                }
+            int2 x;
+            switch (n) {
+                case 2: x = carry_element<2>(lds, J, r, lc, gc, ncl); break;
+                case 3: x = carry_element<3>(lds, J, r, lc, gc, ncl); break;
+                case 4: x = carry_element<4>(lds, J, r, lc, gc, ncl); break;
+                case 5: x = carry_element<5>(lds, J, r, lc, gc, ncl); break;
+                default: x = carry_element<6>(lds, J, r, lc, gc, ncl); break;
             }
-            // digit j is wide iff r_j < b, r_{j-1} = r_j + b (mod N)
-            int r2m = J.rlo[2 * gc] + J.rhi[r];
-            if (r2m >= N) r2m -= N;
-            int rv = r2m - bb;
-            if (rv < 0) rv += N;
-            w[0] = q + (rv < bb ? 1 : 0);  // digit 2m+1
-            rv = r2m;
-#pragma unroll
-            for (int k = 1; k < LUCAS_NMAX + 3; ++k) {
-                if (k <= n + 2) {
-                    if (k > 1) {
-                        rv += bb;
-                        if (rv >= N) rv -= N;
-                    }
-                    w[k] = q + (rv < bb ? 1 : 0);  // digit 2m+1-k
-                }
-            }
-            int hm1, h0, h1;
-            const int d1 = restore_digit(cv, w, n, &h1);          // digit 2m+1
-            const int d0 = restore_digit(cv + 1, w + 1, n, &h0);  // digit 2m
-            (void)restore_digit(cv + 2, w + 2, n, &hm1);          // digit 2m-1: carry only
-            (void)h1;
-            xdig[t] = make_int2(d0 + hm1, d1 + h0);
+            xdig[t] = x;
         }
         if (mode & LUCAS_OUT_DIGITS) {
             __syncthreads();
             for (int t = tid; t < A * C; t += T) {
-                const int r = fdiv(t, fC), lo = t - r * C;
-                reinterpret_cast<int2*>(J.digits)[(size_t)(c0 + lo) + (size_t)B * r] = xdig[t];
+                const int r = udiv(t, C, mC), lo = t - r * C;
+                gstore2i(J.digits, (size_t)(c0 + lo) + (size_t)B * r, xdig[t]);
             }
             block_max_err(err, J.maxerr);
             return;
@@ -425,9 +467,9 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc* __restrict__
     }
     // outer twiddle omega_M^(c*k) and store row-major
     for (int t = tid; t < A * C; t += T) {
-        const int k = fdiv(t, fC), lo = t - k * C;
+        const int k = udiv(t, C, mC), lo = t - k * C;
         const int gc = c0 + lo;
-        J.spec[(size_t)k * B + gc] = cmul(lds[t], outer_twiddle(J, gc * k));
+        gstore2(J.spec, (size_t)k * B + gc, cmul(lds[t], outer_twiddle(J, twA, twMB, gc * k)));
     }
     block_max_err(err, J.maxerr);
 }
@@ -454,17 +496,18 @@ __device__ __forceinline__ void row_kernel_body(const LucasJobDesc* __restrict__
     const bool self = (r0 == 0) || (2 * r0 == A);
     const int r1 = self ? r0 : A - r0;
     const int nrows = self ? 1 : 2;
+    const double2* const twB = J.twB;
 
     for (int t = tid; t < nrows * B; t += T) {
         const int rl = t >= B ? 1 : 0, kc = t - rl * B;
-        lds[t] = J.spec[(size_t)(rl ? r1 : r0) * B + kc];
+        lds[t] = gload2(J.spec, (size_t)(rl ? r1 : r0) * B + kc);
     }
     // pass 0: forward row FFT; pass 1: inverse row FFT (conjugated input)
     for (int pass = 0;; ++pass) {
         __syncthreads();
-        fft_lds<T, -1, false>(lds, B, J.radB, J.nradB, nrows, 1, B, J.twB, tid);
+        fft_lds<T, -1, false>(lds, J.stB, J.nradB, nrows, 0u, 1, B, twB, tid);
         if (pass == 1) break;
-        const double2 wr = J.twMA[r0];
+        const double2 wr = gload2(J.twMA, r0);
         // pairs (kc in row r0) <-> (pc in row r1): rows differ, or row r0 pairs
         // with itself (row 0: pc = -kc mod B; row A/2: pc = B-1-kc)
         const int last = !self ? B - 1 : (r0 == 0 ? B / 2 : (B - 1) / 2);
@@ -473,7 +516,7 @@ __device__ __forceinline__ void row_kernel_body(const LucasJobDesc* __restrict__
             int pc = (self && r0 == 0) ? B - kc : B - 1 - kc;
             if (pc == B) pc = 0;
             const double2 zk = lds[kc], zm = lds[poff + pc];
-            const double2 w = cmul(wr, J.twB[kc]);
+            const double2 w = cmul(wr, gload2(twB, kc));
             const double2 E = cadd(zk, cconj(zm));
             const double2 O = mul_si<-1>(csub(zk, cconj(zm)));
             const double2 P = cadd(cmul(E, E), cmul(w, cmul(O, O)));
@@ -485,7 +528,7 @@ __device__ __forceinline__ void row_kernel_body(const LucasJobDesc* __restrict__
     }
     for (int t = tid; t < nrows * B; t += T) {
         const int rl = t >= B ? 1 : 0, kc = t - rl * B;
-        J.spec2[(size_t)(rl ? r1 : r0) * B + kc] = cconj(lds[t]);
+        gstore2(J.spec2, (size_t)(rl ? r1 : r0) * B + kc, cconj(lds[t]));
     }
 }
 
@@ -499,6 +542,10 @@ extern "C" __global__ void __launch_bounds__(512)
 lucas_col_kernel_512(const LucasJobDesc* __restrict__ jobs, const int2* __restrict__ wgmap, int mode) {
     col_kernel_body<512>(jobs, wgmap, mode);
 }
+extern "C" __global__ void __launch_bounds__(1024)
+lucas_col_kernel_1024(const LucasJobDesc* __restrict__ jobs, const int2* __restrict__ wgmap, int mode) {
+    col_kernel_body<1024>(jobs, wgmap, mode);
+}
 extern "C" __global__ void __launch_bounds__(256)
 lucas_row_kernel_256(const LucasJobDesc* __restrict__ jobs, const int2* __restrict__ wgmap) {
     row_kernel_body<256>(jobs, wgmap);
@@ -506,6 +553,10 @@ lucas_row_kernel_256(const LucasJobDesc* __restrict__ jobs, const int2* __restri
 extern "C" __global__ void __launch_bounds__(512)
 lucas_row_kernel_512(const LucasJobDesc* __restrict__ jobs, const int2* __restrict__ wgmap) {
     row_kernel_body<512>(jobs, wgmap);
+}
+extern "C" __global__ void __launch_bounds__(1024)
+lucas_row_kernel_1024(const LucasJobDesc* __restrict__ jobs, const int2* __restrict__ wgmap) {
+    row_kernel_body<1024>(jobs, wgmap);
 }
 
 // ---------------------------------------------------------------------------
@@ -515,7 +566,9 @@ namespace lucas {
 
 hipError_t launch_col_kernel(int threads, const LucasJobDesc* jobs, const int2* map, int nwg, size_t lds, int mode,
                              hipStream_t stream) {
-    if (threads == 512)
+    if (threads == 1024)
+        hipLaunchKernelGGL(lucas_col_kernel_1024, dim3(nwg), dim3(1024), lds, stream, jobs, map, mode);
+    else if (threads == 512)
         hipLaunchKernelGGL(lucas_col_kernel_512, dim3(nwg), dim3(512), lds, stream, jobs, map, mode);
     else
         hipLaunchKernelGGL(lucas_col_kernel_256, dim3(nwg), dim3(256), lds, stream, jobs, map, mode);
@@ -524,7 +577,9 @@ hipError_t launch_col_kernel(int threads, const LucasJobDesc* jobs, const int2* 
 
 hipError_t launch_row_kernel(int threads, const LucasJobDesc* jobs, const int2* map, int nwg, size_t lds,
                              hipStream_t stream) {
-    if (threads == 512)
+    if (threads == 1024)
+        hipLaunchKernelGGL(lucas_row_kernel_1024, dim3(nwg), dim3(1024), lds, stream, jobs, map);
+    else if (threads == 512)
         hipLaunchKernelGGL(lucas_row_kernel_512, dim3(nwg), dim3(512), lds, stream, jobs, map);
     else
         hipLaunchKernelGGL(lucas_row_kernel_256, dim3(nwg), dim3(256), lds, stream, jobs, map);

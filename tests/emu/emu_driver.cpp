@@ -53,12 +53,7 @@ extern "C" int emu_run(int p, int N, int A, int C, const int32_t* digits_in, lon
     std::vector<int> dig(digits_in, digits_in + N);
     unsigned err = 0;
     LucasJobDesc d{};
-    d.p = pl.p; d.N = pl.N; d.M = pl.M; d.A = pl.A; d.B = pl.B; d.q = pl.q; d.b = pl.b;
-    d.ncarry = pl.ncarry; d.halo = pl.halo; d.C = pl.C; d.col_wgs = pl.col_wgs; d.row_wgs = pl.row_wgs;
-    d.threads = pl.threads;
-    d.nradA = (int)pl.radA.size(); d.nradB = (int)pl.radB.size();
-    for (int i = 0; i < d.nradA; ++i) d.radA[i] = pl.radA[i];
-    for (int i = 0; i < d.nradB; ++i) d.radB[i] = pl.radB[i];
+    fill_desc_shape(pl, &d);
     d.twA = (const double2*)t.twA.data(); d.twB = (const double2*)t.twB.data();
     d.twMA = (const double2*)t.twMA.data(); d.twMB = (const double2*)t.twMB.data();
     d.rlo = t.rlo.data(); d.rhi = t.rhi.data();

@@ -18,6 +18,7 @@
 #define __forceinline__ inline
 #define __launch_bounds__(x)
 #define __shared__
+#define LUCAS_GLOBAL
 
 struct double2 { double x, y; };
 struct int2 { int x, y; };
@@ -50,6 +51,7 @@ inline float __shfl_xor(float v, int mask, int /*width*/) {
     __syncthreads();
     return r;
 }
+inline unsigned __umulhi(unsigned a, unsigned b) { return (unsigned)(((unsigned long long)a * b) >> 32); }
 inline unsigned __float_as_uint(float f) { unsigned u; std::memcpy(&u, &f, 4); return u; }
 inline unsigned atomicMax(unsigned* p, unsigned v) {
     std::lock_guard<std::mutex> lk(emu::atom_mu);
