@@ -54,6 +54,7 @@ void fill_info(const Plan& pl, lucas_plan_info* out) {
     out->col_wgs = pl.col_wgs;
     out->row_wgs = pl.row_wgs;
     out->threads = pl.threads;
+    out->radix_pref = pl.radix_pref;
     out->device_bytes = job_bytes(pl);
 }
 
@@ -240,6 +241,14 @@ int lucas_ctx_create_shape(int p, int N, int A, int C, int device, lucas_ctx** o
     Plan pl;
     int rc = plan_for(p, N, A, C, &pl);
     if (rc != LUCAS_OK) return rc;
+    return ctx_new(pl, device, out);
+}
+
+int lucas_ctx_create_plan(const lucas_plan_info* plan, int device, lucas_ctx** out) {
+    if (!plan || !out) return LUCAS_ERR_INVALID_ARG;
+    Plan pl;
+    if (!make_plan_exact(plan->p, plan->N, plan->A, plan->C, plan->threads, plan->radix_pref, &pl).empty())
+        return LUCAS_ERR_INVALID_ARG;
     return ctx_new(pl, device, out);
 }
 

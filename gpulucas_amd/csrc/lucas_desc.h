@@ -38,6 +38,8 @@
 struct LucasStage {
     int R, Ns, nb, twstep;
     unsigned int mNs, mnb;
+    int twoff;  // this stage's block in the stage twiddle table: [k][r-1], k < Ns
+    int pad;
 };
 
 struct LucasJobDesc {
@@ -58,6 +60,8 @@ struct LucasJobDesc {
     const double2* twB;     // omega_B^t, t < B
     const double2* twMA;    // omega_M^t, t < A
     const double2* twMB;    // omega_M^t, t < B
+    const double2* twsA;    // stage twiddles of the length-A transform (see LucasStage)
+    const double2* twsB;    // stage twiddles of the length-B transform
     // Weight tables: j = Jlo + 2B*r;  r_j = (rlo[Jlo] + rhi[r]) mod N.
     const int* rlo;         // (-Jlo*p) mod N, Jlo < 2B
     const int* rhi;         // (-2B*r*p) mod N, r < A

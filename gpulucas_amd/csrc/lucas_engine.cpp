@@ -59,6 +59,8 @@ int job_create(const Plan& plan, int device, hipStream_t stream, Job** out) {
     const size_t o_iwlo = take(sizeof(double) * 2 * plan.B);
     const size_t o_whi = take(sizeof(double) * plan.A);
     const size_t o_iwhi = take(sizeof(double) * plan.A);
+    const size_t o_twsA = take(sizeof(double) * t.twsA.size());
+    const size_t o_twsB = take(sizeof(double) * t.twsB.size());
     const size_t o_err = take(sizeof(unsigned int));
     if (off > job->bytes) {
         hipFree(job->mem);
@@ -78,6 +80,8 @@ int job_create(const Plan& plan, int device, hipStream_t stream, Job** out) {
     memcpy(host.data() + o_iwlo, t.iwlo.data(), sizeof(double) * t.iwlo.size());
     memcpy(host.data() + o_whi, t.whi.data(), sizeof(double) * t.whi.size());
     memcpy(host.data() + o_iwhi, t.iwhi.data(), sizeof(double) * t.iwhi.size());
+    memcpy(host.data() + o_twsA, t.twsA.data(), sizeof(double) * t.twsA.size());
+    memcpy(host.data() + o_twsB, t.twsB.data(), sizeof(double) * t.twsB.size());
     e = hipMemcpyAsync(job->mem + o_dig, host.data() + o_dig, job->bytes - o_dig, hipMemcpyHostToDevice, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e != hipSuccess) {
@@ -101,6 +105,8 @@ int job_create(const Plan& plan, int device, hipStream_t stream, Job** out) {
     d.iwlo = (const double*)(m + o_iwlo);
     d.whi = (const double*)(m + o_whi);
     d.iwhi = (const double*)(m + o_iwhi);
+    d.twsA = (const double2*)(m + o_twsA);
+    d.twsB = (const double2*)(m + o_twsB);
     d.maxerr = (unsigned int*)(m + o_err);
     job->iter = 0;
     *out = job;

@@ -42,21 +42,43 @@ int carry_depth(int q) {
     return 6;
 }
 
-std::vector<int> radix_plan(int L) {
+std::vector<int> radix_plan(int L, int pref) {
     std::vector<int> r;
     int twos = 0;
     while (L % 2 == 0) { L /= 2; ++twos; }
-    // 2^k as 8s, with a 4x4 instead of 8x2 when k mod 3 == 1.
-    int eights = twos / 3, rem = twos % 3;
-    if (rem == 1 && eights > 0) { --eights; rem = 4; }
-    for (int i = 0; i < eights; ++i) r.push_back(8);
-    if (rem == 1) r.push_back(2);
-    if (rem == 2) r.push_back(4);
-    if (rem == 4) { r.push_back(4); r.push_back(4); }
+    if (pref == 4) {
+        // 2^k as 4s; an odd power takes one 8 (or a 2 when k == 1)
+        int fours = twos / 2;
+        if (twos % 2 && twos >= 3) {
+            r.push_back(8);
+            fours = (twos - 3) / 2;
+        } else if (twos % 2) {
+            r.push_back(2);
+        }
+        for (int i = 0; i < fours; ++i) r.push_back(4);
+    } else {
+        // 2^k as 8s, with a 4x4 instead of 8x2 when k mod 3 == 1.
+        int eights = twos / 3, rem = twos % 3;
+        if (rem == 1 && eights > 0) { --eights; rem = 4; }
+        for (int i = 0; i < eights; ++i) r.push_back(8);
+        if (rem == 1) r.push_back(2);
+        if (rem == 2) r.push_back(4);
+        if (rem == 4) { r.push_back(4); r.push_back(4); }
+    }
     for (int f : {7, 5, 3})
         while (L % f == 0) { L /= f; r.push_back(f); }
     if (L != 1) r.clear();
     return r;
+}
+
+int stage_twiddle_count(int L, const std::vector<int>& rad) {
+    (void)L;
+    int n = 0, Ns = 1;
+    for (int R : rad) {
+        n += Ns * (R - 1);
+        Ns *= R;
+    }
+    return n;
 }
 
 static long long stage_cost(const std::vector<int>& rad, long long elems, int threads) {
@@ -110,13 +132,17 @@ std::vector<Plan> plan_candidates(int p, int N) {
         if (pl.M % A) continue;
         const int B = pl.M / A;
         if (B < pl.halo) continue;
-        std::vector<int> ra = radix_plan(A), rb = radix_plan(B);
+        for (int pref : {8, 4}) {
+        std::vector<int> ra = radix_plan(A, pref), rb = radix_plan(B, pref);
+        if (pref == 4 && ra == radix_plan(A, 8) && rb == radix_plan(B, 8)) continue;
         if (ra.empty() || rb.empty()) continue;
         if ((int)ra.size() > LUCAS_MAX_RADICES || (int)rb.size() > LUCAS_MAX_RADICES) continue;
         for (int C : {1, 2, 4, 8, 16, 32, 64}) {
             if (B % C) continue;
             const long long tile = std::max<long long>((long long)A * (C + pl.halo), 2LL * B);
-            const size_t lds_col = sizeof(double) * 2 * (size_t)A * (C + pl.halo) + sizeof(int) * 2 * (size_t)A * C;
+            // column tile, new-digit buffer, per-wave roundoff slots (up to 16 waves)
+            const size_t lds_col =
+                sizeof(double) * 2 * (size_t)A * (C + pl.halo) + sizeof(int) * 2 * (size_t)A * C + 16 * sizeof(float);
             const size_t lds_row = sizeof(double) * 2 * 2 * (size_t)B;
             if (lds_col > kMaxLds || lds_row > kMaxLds) continue;
             for (int T : {LUCAS_THREADS_SMALL, LUCAS_THREADS_LARGE, LUCAS_THREADS_MAX}) {
@@ -134,25 +160,45 @@ std::vector<Plan> plan_candidates(int p, int N) {
             c.row_wgs = A / 2 + 1;
             c.lds_col = lds_col;
             c.lds_row = lds_row;
+            c.radix_pref = pref;
             out.push_back(c);
             }
+        }
         }
     }
     std::stable_sort(out.begin(), out.end(), [](const Plan& a, const Plan& b) { return model_cost(a) < model_cost(b); });
     return out;
 }
 
-std::string make_plan(int p, int N, int forceA, int forceC, Plan* out) {
+// Measured fastest tilings per signal length (gpulucas_amd/tune.py).
+struct TunedShape {
+    int N, A, C, threads, pref;
+};
+static const TunedShape kTuned[] = {
+#include "tuned_shapes.inc"
+    {0, 0, 0, 0, 0}};
+
+std::string make_plan_exact(int p, int N, int A, int C, int threads, int pref, Plan* out) {
     Plan pl;
     std::string why = base_plan(p, N, &pl);
     if (!why.empty()) return why;
     for (const Plan& c : plan_candidates(p, N)) {
-        if (forceA && c.A != forceA) continue;
-        if (forceC && c.C != forceC) continue;
+        if (A && c.A != A) continue;
+        if (C && c.C != C) continue;
+        if (threads && c.threads != threads) continue;
+        if (pref && c.radix_pref != pref) continue;
         *out = c;
         return "";
     }
     return "no admissible (A, B, C) tiling for this N";
+}
+
+std::string make_plan(int p, int N, int forceA, int forceC, Plan* out) {
+    if (!forceA && !forceC) {
+        for (const TunedShape* t = kTuned; t->N; ++t)
+            if (t->N == N && make_plan_exact(p, N, t->A, t->C, t->threads, t->pref, out).empty()) return "";
+    }
+    return make_plan_exact(p, N, forceA, forceC, 0, 0, out);
 }
 
 double max_bits_per_word(int N) {
@@ -177,9 +223,12 @@ unsigned int magic_u32(unsigned int d) {
 }
 
 void stage_table(int L, const std::vector<int>& rad, LucasStage* out) {
-    int Ns = 1;
+    int Ns = 1, off = 0;
     for (size_t s = 0; s < rad.size(); ++s) {
         LucasStage& st = out[s];
+        st.twoff = off;
+        st.pad = 0;
+        off += Ns * (rad[s] - 1);
         st.R = rad[s];
         st.Ns = Ns;
         st.nb = L / rad[s];
@@ -224,6 +273,8 @@ size_t job_bytes(const Plan& pl) {
     s += align256(sizeof(int) * 2 * (size_t)pl.B) + align256(sizeof(int) * (size_t)pl.A);
     s += align256(sizeof(double) * 2 * (size_t)pl.B) * 2;  // wlo, iwlo
     s += align256(sizeof(double) * (size_t)pl.A) * 2;      // whi, iwhi
+    s += align256(sizeof(double) * 2 * (size_t)std::max(1, stage_twiddle_count(pl.A, pl.radA)));
+    s += align256(sizeof(double) * 2 * (size_t)std::max(1, stage_twiddle_count(pl.B, pl.radB)));
     s += 256;                                               // maxerr
     return s;
 }
@@ -237,7 +288,29 @@ static void root_table(long long count, long long denom, std::vector<double>* ou
     }
 }
 
+// Stage twiddles of a length-L Stockham transform: stage s (radix R after
+// Ns) holds w_L^(r*k*L/(Ns*R)) at [twoff + k*(R-1) + r-1], k < Ns, 1 <= r < R.
+static void stage_twiddles(int L, const std::vector<int>& rad, std::vector<double>* out) {
+    out->assign(2 * (size_t)std::max(1, stage_twiddle_count(L, rad)), 0.0);
+    int Ns = 1;
+    size_t off = 0;
+    for (int R : rad) {
+        const long long twstep = L / (Ns * R);
+        for (int k = 0; k < Ns; ++k)
+            for (int r = 1; r < R; ++r) {
+                const long long e = ((long long)r * k * twstep) % L;
+                long double ang = 2.0L * kPi * (long double)e / (long double)L;
+                (*out)[2 * off] = (double)cosl(ang);
+                (*out)[2 * off + 1] = (double)(-sinl(ang));
+                ++off;
+            }
+        Ns *= R;
+    }
+}
+
 void make_tables(const Plan& pl, Tables* t) {
+    stage_twiddles(pl.A, pl.radA, &t->twsA);
+    stage_twiddles(pl.B, pl.radB, &t->twsB);
     root_table(pl.A, pl.A, &t->twA);
     root_table(pl.B, pl.B, &t->twB);
     root_table(pl.A, pl.M, &t->twMA);

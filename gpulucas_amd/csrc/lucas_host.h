@@ -19,7 +19,8 @@ struct Plan {
     int ncarry = 0;         // reference carry depth n
     int halo = 0;           // ceil((n+1)/2)
     int col_wgs = 0, row_wgs = 0;
-    int threads = 0;        // work-items per workgroup (256 or 512)
+    int threads = 0;        // work-items per workgroup (256, 512 or 1024)
+    int radix_pref = 8;     // preferred power-of-two radix (8 or 4)
     std::vector<int> radA, radB;
     size_t lds_col = 0, lds_row = 0;  // bytes of LDS per workgroup
 };
@@ -28,13 +29,18 @@ struct Plan {
 bool valid_length(long long N);
 // Reference carry depth rule (setSliceAndDice, gpuLucas.cu:323-343); -1 if q >= 21.
 int carry_depth(int q);
-// Radix sequence for a Stockham FFT of length L (radices 8, 4, 2, 3, 5, 7).
-std::vector<int> radix_plan(int L);
+// Radix sequence for a Stockham FFT of length L (radices 8, 4, 2, 3, 5, 7),
+// powers of two as 8s (pref 8) or 4s (pref 4).
+std::vector<int> radix_plan(int L, int pref = 8);
+// Entries of the stage twiddle table of a length-L transform.
+int stage_twiddle_count(int L, const std::vector<int>& rad);
 // Full plan for (p, N) with optional forced A / C (0 = choose).  Returns an
 // empty string on success or the reason for refusal.
 std::string make_plan(int p, int N, int A, int C, Plan* out);
 // Every admissible (A, C) tiling of (p, N), cheapest modelled first.
 std::vector<Plan> plan_candidates(int p, int N);
+// Plan matching a full description (threads / pref 0 = any); "" on success.
+std::string make_plan_exact(int p, int N, int A, int C, int threads, int pref, Plan* out);
 // Default N for p: smallest admissible length under the bits-per-word rule.
 int default_length(int p);
 // Bits per word allowed at length N by the default rule.
@@ -53,6 +59,7 @@ size_t job_bytes(const Plan& plan);
 // Tables (host copies).
 struct Tables {
     std::vector<double> twA, twB, twMA, twMB;  // interleaved re,im
+    std::vector<double> twsA, twsB;            // stage twiddles, interleaved
     std::vector<int> rlo, rhi;
     std::vector<double> wlo, whi, iwlo, iwhi;
 };

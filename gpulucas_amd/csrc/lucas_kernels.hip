@@ -199,7 +199,7 @@ template <int T, int R, int SIGN, bool SEQ_FAST>
 __device__ __forceinline__ void fft_stage(double2* lds, const LucasStage& st, int nseq, unsigned int mseq, int sIdx,
                                           int sSeq, const double2* __restrict__ tw, int tid) {
     constexpr int BM = (LUCAS_EMAX + R - 1) / R;
-    const int Ns = st.Ns, nb = st.nb, twstep = st.twstep;
+    const int Ns = st.Ns, nb = st.nb;
     const int total = nb * nseq;
     double2 v[BM][R];
     int outb[BM];
@@ -218,12 +218,12 @@ __device__ __forceinline__ void fft_stage(double2* lds, const LucasStage& st, in
             const int jq = udiv(j, Ns, st.mNs);
             const int k = j - jq * Ns;
             const int base = seq * sSeq;
-            const int kt = k * twstep;
+            const double2* const twk = tw + st.twoff + k * (R - 1);  // this butterfly's R-1 twiddles
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 double2 x = lds[base + (j + r * nb) * sIdx];
                 if (r > 0 && Ns > 1) {
-                    const double2 w = gload2(tw, r * kt);
+                    const double2 w = gload2(twk, r - 1);
                     x = SIGN < 0 ? cmul(x, w) : cmulc(x, w);
                 }
                 v[i][r] = x;
@@ -279,11 +279,24 @@ __device__ __forceinline__ double2 outer_twiddle(const LucasJobDesc& J, const do
     return cmul(gload2(twA, hi), gload2(twMB, lo));
 }
 
-__device__ __forceinline__ void block_max_err(float err, unsigned int* dst) {
-    // wave64 max reduction, then one atomic per wave.
+// Workgroup max of the roundoff: wave64 shuffle reduction, one LDS slot per
+// wave, then a single atomic per workgroup, issued only when the value beats
+// `seen` (the job's max as read at kernel start).  One atomic per wave on a
+// single address serialises across the chip once every wave has a nonzero
+// roundoff.
+template <int T>
+__device__ __forceinline__ void block_max_err(float err, float* red, unsigned int seen, unsigned int* dst) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) err = fmaxf(err, __shfl_xor(err, off, 64));
-    if ((threadIdx.x & 63) == 0 && err > 0.0f) atomicMax(dst, __float_as_uint(err));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = err;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = red[0];
+#pragma unroll
+        for (int w = 1; w < T / 64; ++w) m = fmaxf(m, red[w]);
+        const unsigned int bits = __float_as_uint(m);
+        if (bits > seen) atomicMax(dst, bits);
+    }
 }
 
 // Carry-save restoration of one digit (reference llintToIrrBal<n>,
@@ -377,6 +390,8 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc* __restrict__
     const int ncl = C + H;  // local columns: [0,H) halo, [H, H+C) own
     const unsigned int mncl = J.m_ncl, mC = J.m_C;
     int2* const xdig = reinterpret_cast<int2*>(lds + A * ncl);  // new digits (2m, 2m+1)
+    float* const red = reinterpret_cast<float*>(xdig + A * C);   // per-wave roundoff slots
+    const unsigned int seen = *reinterpret_cast<volatile unsigned int*>(J.maxerr);
     const double2* const twA = J.twA;
     const double2* const twMB = J.twMB;
     float err = 0.0f;
@@ -418,7 +433,7 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc* __restrict__
         }
         __syncthreads();
         const int nseq = pass ? C : ncl;
-        fft_lds<T, -1, true>(lds, J.stA, J.nradA, nseq, pass ? mC : mncl, nseq, 1, twA, tid);
+        fft_lds<T, -1, true>(lds, J.stA, J.nradA, nseq, pass ? mC : mncl, nseq, 1, J.twsA, tid);
         if (pass == 1) break;
 
         // unweight, -2 on digit 0, round half-even, roundoff
@@ -460,7 +475,7 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc* __restrict__
                 const int r = udiv(t, C, mC), lo = t - r * C;
                 gstore2i(J.digits, (size_t)(c0 + lo) + (size_t)B * r, xdig[t]);
             }
-            block_max_err(err, J.maxerr);
+            block_max_err<T>(err, red, seen, J.maxerr);
             return;
         }
         pass = 1;
@@ -471,7 +486,7 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc* __restrict__
         const int gc = c0 + lo;
         gstore2(J.spec, (size_t)k * B + gc, cmul(lds[t], outer_twiddle(J, twA, twMB, gc * k)));
     }
-    block_max_err(err, J.maxerr);
+    block_max_err<T>(err, red, seen, J.maxerr);
 }
 
 // ---------------------------------------------------------------------------
@@ -505,7 +520,7 @@ __device__ __forceinline__ void row_kernel_body(const LucasJobDesc* __restrict__
     // pass 0: forward row FFT; pass 1: inverse row FFT (conjugated input)
     for (int pass = 0;; ++pass) {
         __syncthreads();
-        fft_lds<T, -1, false>(lds, J.stB, J.nradB, nrows, 0u, 1, B, twB, tid);
+        fft_lds<T, -1, false>(lds, J.stB, J.nradB, nrows, 0u, 1, B, J.twsB, tid);
         if (pass == 1) break;
         const double2 wr = gload2(J.twMA, r0);
         // pairs (kc in row r0) <-> (pc in row r1): rows differ, or row r0 pairs

@@ -46,7 +46,7 @@ class PlanInfo(C.Structure):
     _fields_ = [("p", C.c_int), ("N", C.c_int), ("q", C.c_int), ("n_wide", C.c_int),
                 ("carry_depth", C.c_int), ("A", C.c_int), ("B", C.c_int), ("C", C.c_int),
                 ("halo", C.c_int), ("col_wgs", C.c_int), ("row_wgs", C.c_int),
-                ("threads", C.c_int), ("device_bytes", C.c_size_t)]
+                ("threads", C.c_int), ("radix_pref", C.c_int), ("device_bytes", C.c_size_t)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -100,6 +100,7 @@ _SIGS = {
     "lucas_digits_res64": (C.c_int, [C.c_int, C.c_int, C.c_void_p, _P(C.c_uint64), _P(C.c_int), C.c_void_p]),
     "lucas_ctx_create": (C.c_int, [C.c_int, C.c_int, C.c_int, _P(C.c_void_p)]),
     "lucas_ctx_create_shape": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P(C.c_void_p)]),
+    "lucas_ctx_create_plan": (C.c_int, [_P(PlanInfo), C.c_int, _P(C.c_void_p)]),
     "lucas_ctx_destroy": (C.c_int, [C.c_void_p]),
     "lucas_ctx_info": (C.c_int, [C.c_void_p, _P(PlanInfo)]),
     "lucas_ctx_step": (C.c_int, [C.c_void_p, C.c_longlong]),
@@ -219,10 +220,14 @@ def digits_res64(p: int, N: int, digits) -> tuple[int, bool, np.ndarray]:
 class Context:
     """Device-resident state of one exponent (s_0 = 4 at creation)."""
 
-    def __init__(self, p: int, N: int = 0, device: int = 0, A: int = 0, C_: int = 0, _handle=None):
+    def __init__(self, p: int, N: int = 0, device: int = 0, A: int = 0, C_: int = 0, _handle=None,
+                 plan: dict | None = None):
         self._h = C.c_void_p()
         if _handle is not None:
             self._h = _handle
+        elif plan is not None:
+            info = PlanInfo(**{k: plan[k] for k, _ in PlanInfo._fields_ if k in plan})
+            _check(_lib().lucas_ctx_create_plan(C.byref(info), device, C.byref(self._h)), f"ctx_create_plan({plan})")
         else:
             _check(_lib().lucas_ctx_create_shape(p, N, A, C_, device, C.byref(self._h)),
                    f"ctx_create(p={p}, N={N})")

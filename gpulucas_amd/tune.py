@@ -43,8 +43,10 @@ def measure(pairs, iters, max_candidates, out):
         info, ms = lucas.autotune_shape(p, N, iters=iters, max_candidates=max_candidates)
         base = lucas.plan(p, N)
         results[str(N)] = {"p": p, "N": N, "A": info["A"], "C": info["C"], "threads": info["threads"],
-                           "ms_per_iter": ms, "model_A": base["A"], "model_C": base["C"]}
-        print(f"N={N} p={p}: A={info['A']} C={info['C']} T={info['threads']} {ms * 1e3:.2f} us/iter "
+                           "radix_pref": info["radix_pref"], "ms_per_iter": ms,
+                           "model_A": base["A"], "model_C": base["C"]}
+        print(f"N={N} p={p}: A={info['A']} C={info['C']} T={info['threads']} R{info['radix_pref']} "
+              f"{ms * 1e3:.2f} us/iter "
               f"({time.time() - t0:.1f}s)", flush=True)
         with open(out, "w") as f:
             json.dump(results, f, indent=1)
@@ -56,12 +58,13 @@ def emit(paths):
     for path in paths:
         with open(path) as f:
             table.update(json.load(f))
-    rows = sorted((int(v["N"]), int(v["A"]), int(v["C"]), float(v["ms_per_iter"])) for v in table.values())
+    rows = sorted((int(v["N"]), int(v["A"]), int(v["C"]), int(v["threads"]), int(v.get("radix_pref", 8)),
+                   float(v["ms_per_iter"])) for v in table.values())
     with open(INC, "w") as f:
         f.write("// Generated by gpulucas_amd/tune.py from GPU measurements (MI355X):\n")
-        f.write("// signal length N -> fastest column length A and tile width C.\n")
-        for N, A, C, ms in rows:
-            f.write(f"{{{N}, {A}, {C}}},  // {ms * 1e3:.2f} us/iteration\n")
+        f.write("// {N, A, C, threads, radix preference}\n")
+        for N, A, C, T, pref, ms in rows:
+            f.write(f"{{{N}, {A}, {C}, {T}, {pref}}},  // {ms * 1e3:.2f} us/iteration\n")
     print(f"wrote {len(rows)} entries to {INC}")
 
 

@@ -56,6 +56,7 @@ typedef struct {
     int halo;          /* halo columns recomputed per column workgroup */
     int col_wgs, row_wgs;
     int threads;       /* work-items per workgroup */
+    int radix_pref;    /* power-of-two radix preference of the FFT stages (8 or 4) */
     size_t device_bytes; /* device footprint of one job */
 } lucas_plan_info;
 
@@ -109,6 +110,9 @@ int lucas_digits_res64(int p, int N, const int32_t* digits, uint64_t* res64, int
  * gpuLucas.cu:718-726; mallocArrays gpuLucas.cu:948-1029). */
 int lucas_ctx_create(int p, int N, int device, lucas_ctx** out);
 int lucas_ctx_create_shape(int p, int N, int A, int C, int device, lucas_ctx** out);
+/* Context from a full plan description (p, N, A, C, threads, radix_pref as
+ * returned by lucas_plan_list / lucas_autotune_shape). */
+int lucas_ctx_create_plan(const lucas_plan_info* plan, int device, lucas_ctx** out);
 int lucas_ctx_destroy(lucas_ctx* ctx);                 /* freeArrays, gpuLucas.cu:1031-1057 */
 int lucas_ctx_info(const lucas_ctx* ctx, lucas_plan_info* out);
 /* Advance `iters` LL iterations (mersenneIter x iters, gpuLucas.cu:845-942). */

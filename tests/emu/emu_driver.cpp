@@ -56,6 +56,7 @@ extern "C" int emu_run(int p, int N, int A, int C, const int32_t* digits_in, lon
     fill_desc_shape(pl, &d);
     d.twA = (const double2*)t.twA.data(); d.twB = (const double2*)t.twB.data();
     d.twMA = (const double2*)t.twMA.data(); d.twMB = (const double2*)t.twMB.data();
+    d.twsA = (const double2*)t.twsA.data(); d.twsB = (const double2*)t.twsB.data();
     d.rlo = t.rlo.data(); d.rhi = t.rhi.data();
     d.wlo = t.wlo.data(); d.whi = t.whi.data(); d.iwlo = t.iwlo.data(); d.iwhi = t.iwhi.data();
     d.spec = spec.data(); d.spec2 = spec2.data(); d.digits = dig.data(); d.maxerr = &err;
