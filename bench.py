@@ -64,6 +64,70 @@ def batched_ratio(lucas, wl, iters, count):
     return batched_its, plain_its
 
 
+def measured_traffic(info):
+    """HBM bytes per launch of each kernel from the committed counter summary
+    (profiles/traffic_p2976221_N163840.md), if it was taken with this plan."""
+    path = os.path.join(ROOT, "profiles", "traffic_p2976221_N163840.md")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if line.startswith("traffic:"):
+            k, v = line[len("traffic:"):].split("=")
+            vals[k.strip()] = v.strip()
+    shape = f"A={info['A']} B={info['B']} C={info['C']} T={info['threads']} R={info['radix_pref']}"
+    if vals.get("plan") != shape:
+        return None
+    return {k: int(v) for k, v in vals.items() if k != "plan"}
+
+
+def t_iter_model_us(N):
+    """Solo ms/iter model of the engine (fit to tuned MI355X measurements:
+    13.5 us at N=4608, 21.9 us at 163840, 90.6 us at 2286144)."""
+    return 16.7 + 32.0 * N / 1e6
+
+
+def serving(lucas, wl, count, k_scale, load):
+    """Poisson arrivals of (p, K) jobs into the continuous-batching server.
+
+    Bounded sample of the serving workload: the first `count` jobs of seed 5
+    (p log-uniform in [1e5, 4e7]), K scaled by `k_scale`; the arrival rate is
+    set so that running the jobs one after another alone would keep the GPU
+    `load` busy."""
+    import threading
+
+    jobs = wl.serving_jobs(seed=5, count=count)
+    ks = [max(500, int(k * k_scale)) for _, _, k in jobs]
+    ns = [lucas.default_N(p) for _, p, _ in jobs]
+    solo = [k * t_iter_model_us(n) * 1e-6 for k, n in zip(ks, ns)]
+    mean_gap = (sum(solo) / len(solo)) / load
+    scale = mean_gap / (jobs[-1][0] / len(jobs))
+    arrivals = [t * scale for t, _, _ in jobs]
+    ids = {}
+    with lucas.Server(chunk_iters=256) as srv:
+        t0 = time.perf_counter()
+        for (t, p, _), k, a in zip(jobs, ks, arrivals):
+            dt = a - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            ids[srv.submit(p, k)] = (p, k)
+        srv.drain(timeout_s=600)
+        res = [srv.poll(j) for j in ids]
+        st = srv.stats()
+    lat = sorted(r["finish_s"] - r["submit_s"] for r in res)
+    span = max(r["finish_s"] for r in res) - min(r["submit_s"] for r in res)
+
+    def pct(q):
+        return lat[min(len(lat) - 1, int(round(q * (len(lat) - 1))))]
+
+    return {"workload": f"{count} Poisson jobs (seed 5), p log-uniform [1e5,4e7], K/{int(1 / k_scale)} "
+                        f"in [500,5000], offered load {load:.0%} of solo service",
+            "jobs": len(res), "p50_s": round(pct(0.5), 4), "p99_s": round(pct(0.99), 4),
+            "max_s": round(lat[-1], 4), "iters_per_s": round(sum(ks) / span, 1),
+            "peak_jobs": st["peak_jobs"], "max_roundoff": round(max(r["max_roundoff"] for r in res), 4),
+            "all_ok": all(r["status"] == 0 for r in res)}
+
+
 def cpu_baseline(seconds_target):
     import oracle
 
@@ -85,6 +149,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=48)
+    ap.add_argument("--profile-run", action="store_true",
+                    help="only the headline iterations (for rocprofv3 kernel-trace / counter passes)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only a device-to-device copy of BYTES (HBM counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {a.gpus}); the engine runs one "
@@ -92,6 +162,17 @@ def main():
         sys.exit(2)
 
     from gpulucas_amd import lucas, workloads as wl
+
+    if a.copy_calibration:
+        us = lucas.device_copy_bench(a.copy_calibration, reps=20)
+        print(json.dumps({"copy_bytes": a.copy_calibration, "us_per_copy": us,
+                          "GBps": 2 * a.copy_calibration / us / 1e3}), flush=True)
+        return
+    if a.profile_run:
+        with lucas.Context(P_HEAD, N_HEAD) as ctx:
+            ctx.step(a.warmup + a.steps)
+            print(json.dumps({"profile_run": ctx.info, "max_roundoff": ctx.max_roundoff()}), flush=True)
+        return
 
     ms, err, col_us, row_us, info = headline(lucas, a.steps, a.warmup)
     N = N_HEAD
@@ -123,11 +204,18 @@ def main():
                      "algorithmic_bytes_per_launch": kern_bytes},
         "fp64_tflops": round(5 * N * __import__("math").log2(N) / (ms * 1e-3) / 1e12, 4),
     }
+    tr = measured_traffic(info)
+    if tr:
+        key = "lucas_col_kernel" if dom_name == "lucas_col_kernel" else "lucas_row_kernel"
+        out["roofline"]["traffic"] = tr.get(key)
+        out["roofline"]["traffic_by_kernel"] = tr
     if not a.no_batch:
         bi, pi = batched_ratio(lucas, wl, a.batch_iters, 64)
         out["batched"] = {"workload": "64 primes in [1e6,5e6] (seed 4), default N each",
                           "iters_per_s": round(bi, 1), "plain_iters_per_s": round(pi, 1),
                           "ratio": round(bi / pi, 3)}
+    if not a.no_serve:
+        out["serving"] = serving(lucas, wl, a.serve_jobs, 0.1, 0.8)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
     print(json.dumps(out), flush=True)

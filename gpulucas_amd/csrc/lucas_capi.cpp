@@ -192,6 +192,37 @@ int lucas_device_count(int* count) {
     return LUCAS_OK;
 }
 
+int lucas_device_copy_bench(size_t bytes, int reps, int device, double* us_per_copy) {
+    if (!us_per_copy || bytes == 0 || reps <= 0) return LUCAS_ERR_INVALID_ARG;
+    if (hipSetDevice(device) != hipSuccess) return LUCAS_ERR_NO_DEVICE;
+    void *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, bytes) != hipSuccess) return LUCAS_ERR_OOM;
+    if (hipMalloc(&b, bytes) != hipSuccess) {
+        hipFree(a);
+        return LUCAS_ERR_OOM;
+    }
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = LUCAS_OK;
+    float ms = 0.f;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess || hipMemsetAsync(a, 1, bytes, s) != hipSuccess ||
+        hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess || hipEventRecord(e0, s) != hipSuccess)
+        rc = LUCAS_ERR_DEVICE;
+    for (int i = 0; i < reps && rc == LUCAS_OK; ++i)
+        if (hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess) rc = LUCAS_ERR_DEVICE;
+    if (rc == LUCAS_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                           hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        rc = LUCAS_ERR_DEVICE;
+    if (rc == LUCAS_OK) *us_per_copy = 1000.0 * ms / reps;
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (s) hipStreamDestroy(s);
+    hipFree(a);
+    hipFree(b);
+    return rc;
+}
+
 int lucas_default_N(int p) { return default_length(p); }
 
 int lucas_plan(int p, int N, lucas_plan_info* out) { return lucas_plan_shape(p, N, 0, 0, out); }

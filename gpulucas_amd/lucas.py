@@ -91,6 +91,7 @@ _SIGS = {
     "lucas_status_string": (C.c_char_p, [C.c_int]),
     "lucas_device_count": (C.c_int, [_P(C.c_int)]),
     "lucas_default_N": (C.c_int, [C.c_int]),
+    "lucas_device_copy_bench": (C.c_int, [C.c_size_t, C.c_int, C.c_int, _P(C.c_double)]),
     "lucas_plan": (C.c_int, [C.c_int, C.c_int, _P(PlanInfo)]),
     "lucas_plan_shape": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _P(PlanInfo)]),
     "lucas_plan_list": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_int, _P(C.c_int)]),
@@ -167,6 +168,13 @@ def device_count() -> int:
     n = C.c_int(0)
     _lib().lucas_device_count(C.byref(n))
     return n.value
+
+
+def device_copy_bench(nbytes: int, reps: int = 20, device: int = 0) -> float:
+    """Mean microseconds per device-to-device copy of `nbytes`."""
+    us = C.c_double(0)
+    _check(_lib().lucas_device_copy_bench(nbytes, reps, device, C.byref(us)), "device_copy_bench")
+    return us.value
 
 
 # ---- planning (host only) --------------------------------------------------

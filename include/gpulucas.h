@@ -105,6 +105,11 @@ int lucas_digit_tables(int p, int N, int* widths, double* weights, double* inv_w
 int lucas_digits_res64(int p, int N, const int32_t* digits, uint64_t* res64, int* is_zero,
                        int32_t* normalized);
 
+/* Device-to-device copy of `bytes` repeated `reps` times on `device`; mean
+ * microseconds per copy.  A known-size stream for bandwidth and HBM-counter
+ * calibration (no reference counterpart). */
+int lucas_device_copy_bench(size_t bytes, int reps, int device, double* us_per_copy);
+
 /* ---- single-exponent context ------------------------------------------ */
 /* Allocates device state for (p, N) and sets s_0 = 4 (loadValue4ToFFTarray,
  * gpuLucas.cu:718-726; mallocArrays gpuLucas.cu:948-1029). */
