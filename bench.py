@@ -153,6 +153,8 @@ def main():
     ap.add_argument("--serve-jobs", type=int, default=48)
     ap.add_argument("--profile-run", action="store_true",
                     help="only the headline iterations (for rocprofv3 kernel-trace / counter passes)")
+    ap.add_argument("--p", type=int, default=P_HEAD, help="exponent for --profile-run")
+    ap.add_argument("--N", type=int, default=N_HEAD, help="signal length for --profile-run (0: default)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only a device-to-device copy of BYTES (HBM counter calibration)")
     a = ap.parse_args()
@@ -169,7 +171,7 @@ def main():
                           "GBps": 2 * a.copy_calibration / us / 1e3}), flush=True)
         return
     if a.profile_run:
-        with lucas.Context(P_HEAD, N_HEAD) as ctx:
+        with lucas.Context(a.p, a.N) as ctx:
             ctx.step(a.warmup + a.steps)
             print(json.dumps({"profile_run": ctx.info, "max_roundoff": ctx.max_roundoff()}), flush=True)
         return

@@ -19,6 +19,7 @@ hipError_t launch_col_kernel(int threads, const LucasJobDesc* jobs, const int2* 
                              hipStream_t stream);
 hipError_t launch_row_kernel(int threads, const LucasJobDesc* jobs, const int2* map, int nwg, size_t lds,
                              hipStream_t stream);
+hipError_t configure_kernels(size_t max_lds);
 
 static const int kGraphIters = 32;
 
@@ -175,6 +176,7 @@ int Engine::init() {
     }
     if (device_ < 0 || device_ >= count) return LUCAS_ERR_INVALID_ARG;
     if (hipSetDevice(device_) != hipSuccess) return LUCAS_ERR_NO_DEVICE;
+    if (configure_kernels(kMaxLdsBytes) != hipSuccess) return LUCAS_ERR_DEVICE;
     if (hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking) != hipSuccess) return LUCAS_ERR_DEVICE;
     if (hipEventCreate(&ev0_) != hipSuccess || hipEventCreate(&ev1_) != hipSuccess) return LUCAS_ERR_DEVICE;
     return LUCAS_OK;

@@ -23,8 +23,9 @@
 namespace lucas {
 
 static const long double kPi = 3.141592653589793238462643383279502884L;
-// LDS one workgroup may allocate (column tile + digit buffer, or two rows).
-static const size_t kMaxLds = 65536;
+// LDS one workgroup may allocate (column tile + digit buffer, or two rows):
+// all of a gfx950 CU's 160 KB.
+static const size_t kMaxLds = kMaxLdsBytes;
 
 bool valid_length(long long N) {
     if (N < 4 || (N & 1)) return false;
@@ -140,10 +141,12 @@ std::vector<Plan> plan_candidates(int p, int N) {
         for (int C : {1, 2, 4, 8, 16, 32, 64}) {
             if (B % C) continue;
             const long long tile = std::max<long long>((long long)A * (C + pl.halo), 2LL * B);
-            // column tile, new-digit buffer, per-wave roundoff slots (up to 16 waves)
-            const size_t lds_col =
-                sizeof(double) * 2 * (size_t)A * (C + pl.halo) + sizeof(int) * 2 * (size_t)A * C + 16 * sizeof(float);
-            const size_t lds_row = sizeof(double) * 2 * 2 * (size_t)B;
+            // padded column tile (one pad word per 8), new-digit buffer, per-wave
+            // roundoff slots (up to 16 waves); two padded rows
+            const size_t ctile = (size_t)A * (C + pl.halo);
+            const size_t lds_col = sizeof(double) * 2 * (ctile + ctile / 8 + 1) + sizeof(int) * 2 * (size_t)A * C +
+                                   16 * sizeof(float);
+            const size_t lds_row = sizeof(double) * 2 * (2 * (size_t)B + 2 * (size_t)B / 8 + 1);
             if (lds_col > kMaxLds || lds_row > kMaxLds) continue;
             for (int T : {LUCAS_THREADS_SMALL, LUCAS_THREADS_LARGE, LUCAS_THREADS_MAX}) {
             if (tile > (long long)LUCAS_EMAX * T) continue;

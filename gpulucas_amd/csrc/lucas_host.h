@@ -12,6 +12,9 @@
 
 namespace lucas {
 
+// LDS a workgroup may use on gfx950 (160 KB per CU, all of it per workgroup).
+static const size_t kMaxLdsBytes = 163840;
+
 struct Plan {
     int p = 0, N = 0, M = 0;
     int A = 0, B = 0, C = 0;

@@ -61,6 +61,11 @@ __device__ __forceinline__ double2* lucas_dynamic_lds() {
 
 namespace {
 
+// LDS word index with one pad word after every 8 complex words.  Stockham
+// writes land at stride R (and R*Ns) words; unpadded, eight lanes of a
+// wave hit the same banks.
+__device__ __forceinline__ int pidx(int i) { return i + (i >> 3); }
+
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
@@ -221,7 +226,7 @@ __device__ __forceinline__ void fft_stage(double2* lds, const LucasStage& st, in
             const double2* const twk = tw + st.twoff + k * (R - 1);  // this butterfly's R-1 twiddles
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                double2 x = lds[base + (j + r * nb) * sIdx];
+                double2 x = lds[pidx(base + (j + r * nb) * sIdx)];
                 if (r > 0 && Ns > 1) {
                     const double2 w = gload2(twk, r - 1);
                     x = SIGN < 0 ? cmul(x, w) : cmulc(x, w);
@@ -238,7 +243,7 @@ __device__ __forceinline__ void fft_stage(double2* lds, const LucasStage& st, in
         const int t = tid + i * T;
         if (t < total) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) lds[outb[i] + r * Ns * sIdx] = v[i][r];
+            for (int r = 0; r < R; ++r) lds[pidx(outb[i] + r * Ns * sIdx)] = v[i][r];
         }
     }
     __syncthreads();
@@ -345,7 +350,7 @@ __device__ __forceinline__ int2 carry_element(const double2* lds, const LucasJob
         const int ie = k >> 1;  // element offset back
         int rr = r;
         if (gc - ie < 0) rr = (r == 0) ? A - 1 : r - 1;
-        const double2 v = lds[rr * ncl + (lc - ie)];
+        const double2 v = lds[pidx(rr * ncl + (lc - ie))];
         const long long c = (long long)((k & 1) ? v.x : v.y);
         neg[k] = c < 0;
         mag[k] = c < 0 ? (unsigned long long)(-c) : (unsigned long long)c;
@@ -389,7 +394,7 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc* __restrict__
     const int tid = threadIdx.x;
     const int ncl = C + H;  // local columns: [0,H) halo, [H, H+C) own
     const unsigned int mncl = J.m_ncl, mC = J.m_C;
-    int2* const xdig = reinterpret_cast<int2*>(lds + A * ncl);  // new digits (2m, 2m+1)
+    int2* const xdig = reinterpret_cast<int2*>(lds + pidx(A * ncl) + 1);  // new digits (2m, 2m+1)
     float* const red = reinterpret_cast<float*>(xdig + A * C);   // per-wave roundoff slots
     const unsigned int seen = *reinterpret_cast<volatile unsigned int*>(J.maxerr);
     const double2* const twA = J.twA;
@@ -412,7 +417,7 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc* __restrict__
             int gc = c0 - H + lc;
             if (gc < 0) gc += B;
             const double2 v = gload2(J.spec2, (size_t)r * B + gc);
-            lds[t] = cconj(cmulc(v, outer_twiddle(J, twA, twMB, gc * r)));
+            lds[pidx(t)] = cconj(cmulc(v, outer_twiddle(J, twA, twMB, gc * r)));
         }
     }
     for (;;) {
@@ -428,7 +433,7 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc* __restrict__
                 const double ae = gload(J.wlo, je) * wh * (gload(J.rlo, je) + rh >= N ? 0.5 : 1.0);
                 const double ao = gload(J.wlo, jo) * wh * (gload(J.rlo, jo) + rh >= N ? 0.5 : 1.0);
                 const int2 x = xdig[t];
-                lds[t] = make_double2(ae * (double)x.x, ao * (double)x.y);
+                lds[pidx(t)] = make_double2(ae * (double)x.x, ao * (double)x.y);
             }
         }
         __syncthreads();
@@ -441,7 +446,7 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc* __restrict__
             const int r = udiv(t, ncl, mncl), lc = t - r * ncl;
             int gc = c0 - H + lc;
             if (gc < 0) gc += B;
-            const double2 v = cconj(lds[t]);
+            const double2 v = cconj(lds[pidx(t)]);
             const int rh = gload(J.rhi, r);
             const double iwh = gload(J.iwhi, r);
             const int je = 2 * gc, jo = je + 1;
@@ -452,7 +457,7 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc* __restrict__
             if (gc == 0 && r == 0) ve -= 2.0;
             const double re = rint(ve), ro = rint(vo);
             err = fmaxf(err, (float)fmax(fabs(ve - re), fabs(vo - ro)));
-            lds[t] = make_double2(re, ro);
+            lds[pidx(t)] = make_double2(re, ro);
         }
         __syncthreads();
         // carry-save restoration + single ripple for the own columns
@@ -484,7 +489,7 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc* __restrict__
     for (int t = tid; t < A * C; t += T) {
         const int k = udiv(t, C, mC), lo = t - k * C;
         const int gc = c0 + lo;
-        gstore2(J.spec, (size_t)k * B + gc, cmul(lds[t], outer_twiddle(J, twA, twMB, gc * k)));
+        gstore2(J.spec, (size_t)k * B + gc, cmul(lds[pidx(t)], outer_twiddle(J, twA, twMB, gc * k)));
     }
     block_max_err<T>(err, red, seen, J.maxerr);
 }
@@ -515,7 +520,7 @@ __device__ __forceinline__ void row_kernel_body(const LucasJobDesc* __restrict__
 
     for (int t = tid; t < nrows * B; t += T) {
         const int rl = t >= B ? 1 : 0, kc = t - rl * B;
-        lds[t] = gload2(J.spec, (size_t)(rl ? r1 : r0) * B + kc);
+        lds[pidx(t)] = gload2(J.spec, (size_t)(rl ? r1 : r0) * B + kc);
     }
     // pass 0: forward row FFT; pass 1: inverse row FFT (conjugated input)
     for (int pass = 0;; ++pass) {
@@ -530,20 +535,20 @@ __device__ __forceinline__ void row_kernel_body(const LucasJobDesc* __restrict__
         for (int kc = tid; kc <= last; kc += T) {
             int pc = (self && r0 == 0) ? B - kc : B - 1 - kc;
             if (pc == B) pc = 0;
-            const double2 zk = lds[kc], zm = lds[poff + pc];
+            const double2 zk = lds[pidx(kc)], zm = lds[pidx(poff + pc)];
             const double2 w = cmul(wr, gload2(twB, kc));
             const double2 E = cadd(zk, cconj(zm));
             const double2 O = mul_si<-1>(csub(zk, cconj(zm)));
             const double2 P = cadd(cmul(E, E), cmul(w, cmul(O, O)));
             const double2 Q = cmul(E, O);
             // stored conjugated for the inverse transform
-            lds[kc] = make_double2(P.x - 2.0 * Q.y, -(P.y + 2.0 * Q.x));
-            if (poff + pc != kc) lds[poff + pc] = make_double2(P.x + 2.0 * Q.y, P.y - 2.0 * Q.x);
+            lds[pidx(kc)] = make_double2(P.x - 2.0 * Q.y, -(P.y + 2.0 * Q.x));
+            if (poff + pc != kc) lds[pidx(poff + pc)] = make_double2(P.x + 2.0 * Q.y, P.y - 2.0 * Q.x);
         }
     }
     for (int t = tid; t < nrows * B; t += T) {
         const int rl = t >= B ? 1 : 0, kc = t - rl * B;
-        gstore2(J.spec2, (size_t)(rl ? r1 : r0) * B + kc, cconj(lds[t]));
+        gstore2(J.spec2, (size_t)(rl ? r1 : r0) * B + kc, cconj(lds[pidx(t)]));
     }
 }
 
@@ -578,6 +583,18 @@ lucas_row_kernel_1024(const LucasJobDesc* __restrict__ jobs, const int2* __restr
 // Host-side launchers (the engine's only entry into device code).
 // ---------------------------------------------------------------------------
 namespace lucas {
+
+// Workgroups may use the whole 160 KB LDS of a gfx950 CU (64 KB without this).
+hipError_t configure_kernels(size_t max_lds) {
+    const void* fns[] = {(const void*)lucas_col_kernel_256,  (const void*)lucas_col_kernel_512,
+                         (const void*)lucas_col_kernel_1024, (const void*)lucas_row_kernel_256,
+                         (const void*)lucas_row_kernel_512,  (const void*)lucas_row_kernel_1024};
+    for (const void* f : fns) {
+        hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
 
 hipError_t launch_col_kernel(int threads, const LucasJobDesc* jobs, const int2* map, int nwg, size_t lds, int mode,
                              hipStream_t stream) {

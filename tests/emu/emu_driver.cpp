@@ -10,7 +10,7 @@
 
 thread_local emu_idx threadIdx, blockIdx;
 emu_idx blockDim;
-double2 lucas_lds[65536 / sizeof(double2)];
+double2 lucas_lds[163840 / sizeof(double2)];
 
 namespace emu {
 std::barrier<>* bar = nullptr;

@@ -36,6 +36,8 @@ typedef int hipError_t;
 typedef void* hipStream_t;
 #define hipSuccess 0
 inline hipError_t hipGetLastError() { return 0; }
+#define hipFuncAttributeMaxDynamicSharedMemorySize 0
+inline hipError_t hipFuncSetAttribute(const void*, int, int) { return 0; }
 
 namespace emu {
 extern std::barrier<>* bar;
