@@ -16,6 +16,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(PKG, "libgpulucas.so")
+CLI = os.path.join(PKG, "gpulucas")
 
 SOURCES = ["lucas_kernels.hip", "lucas_host.cpp", "lucas_engine.cpp", "lucas_capi.cpp"]
 HEADERS = ["lucas_desc.h", "lucas_host.h", "lucas_engine.h"]
@@ -78,7 +79,17 @@ def build_library(verbose: bool = False, jobs: int = 4) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"], verbose)
     os.replace(tmp, LIB)
+    build_cli(verbose)
     return LIB
+
+
+def build_cli(verbose: bool = False) -> str:
+    """The command-line driver `gpulucas_amd/gpulucas`, linked to the library."""
+    tmp = CLI + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I" + INCLUDE, os.path.join(CSRC, "lucas_cli.cpp"),
+          "-o", tmp, "-L" + PKG, "-lgpulucas", "-Wl,-rpath,$ORIGIN"], verbose)
+    os.replace(tmp, CLI)
+    return CLI
 
 
 def build_oracle(verbose: bool = False) -> None:
