@@ -51,6 +51,7 @@ struct LucasJobDesc {
     int col_wgs, row_wgs;   // workgroups per launch of each kernel
     int threads;            // work-items per workgroup (256, 512 or 1024)
     int nradA, nradB;
+    int ntwsA, ntwsB;       // entries of the two stage twiddle tables
     LucasStage stA[LUCAS_MAX_RADICES];  // column transform stages (length A)
     LucasStage stB[LUCAS_MAX_RADICES];  // row transform stages (length B)
     unsigned int m_ncl, m_C;             // reciprocals of C+halo and C
@@ -75,3 +76,16 @@ struct LucasJobDesc {
     int* digits;
     unsigned int* maxerr;   // max |x - rint(x)| since last reset, as float bits
 };
+
+// Dynamic LDS of the two kernels for one job, in 16-byte words.  Each kernel
+// carves its LDS in the order listed.
+//   column kernel: padded (C+H) x A tile; stage twiddles; per-row
+//     {2^(rhi/N), 2^(-rhi/N)/(4M)}; per local digit {2^(rlo/N), 2^(-rlo/N)};
+//     per-row rhi; per local digit rlo; roundoff slots
+//   row kernel: two padded rows; stage twiddles; omega_B^t, t < B
+__host__ __device__ inline int lucas_pad_words(int n) { return n + (n >> 3) + 1; }
+__host__ __device__ inline int lucas_col_lds_words(int A, int C, int H, int ntws) {
+    const int ncl = C + H;
+    return lucas_pad_words(A * ncl) + ntws + A + 2 * ncl + (A + 3) / 4 + (2 * ncl + 3) / 4 + 4;
+}
+__host__ __device__ inline int lucas_row_lds_words(int B, int ntws) { return lucas_pad_words(2 * B) + ntws + B; }

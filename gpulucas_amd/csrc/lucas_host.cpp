@@ -141,12 +141,9 @@ std::vector<Plan> plan_candidates(int p, int N) {
         for (int C : {1, 2, 4, 8, 16, 32, 64}) {
             if (B % C) continue;
             const long long tile = std::max<long long>((long long)A * (C + pl.halo), 2LL * B);
-            // padded column tile (one pad word per 8), new-digit buffer, per-wave
-            // roundoff slots (up to 16 waves); two padded rows
-            const size_t ctile = (size_t)A * (C + pl.halo);
-            const size_t lds_col = sizeof(double) * 2 * (ctile + ctile / 8 + 1) + sizeof(int) * 2 * (size_t)A * C +
-                                   16 * sizeof(float);
-            const size_t lds_row = sizeof(double) * 2 * (2 * (size_t)B + 2 * (size_t)B / 8 + 1);
+            const size_t lds_col =
+                    16 * (size_t)lucas_col_lds_words(A, C, pl.halo, stage_twiddle_count(A, ra));
+            const size_t lds_row = 16 * (size_t)lucas_row_lds_words(B, stage_twiddle_count(B, rb));
             if (lds_col > kMaxLds || lds_row > kMaxLds) continue;
             for (int T : {LUCAS_THREADS_SMALL, LUCAS_THREADS_LARGE, LUCAS_THREADS_MAX}) {
             if (tile > (long long)LUCAS_EMAX * T) continue;
@@ -298,6 +295,8 @@ void fill_desc_shape(const Plan& plan, LucasJobDesc* d) {
     d->threads = plan.threads;
     d->nradA = (int)plan.radA.size();
     d->nradB = (int)plan.radB.size();
+    d->ntwsA = stage_twiddle_count(plan.A, plan.radA);
+    d->ntwsB = stage_twiddle_count(plan.B, plan.radB);
     stage_table(plan.A, plan.radA, d->stA);
     stage_table(plan.B, plan.radB, d->stB);
     d->m_ncl = magic_u32((unsigned)(plan.C + plan.halo));

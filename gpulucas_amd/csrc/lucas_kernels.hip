@@ -191,7 +191,7 @@ template <int SIGN> struct Dft<8, SIGN> {
 
 // ---------------------------------------------------------------------------
 // One Stockham autosort stage over `nseq` sequences of length L held in LDS.
-// Element (idx, seq) lives at lds[seq*sSeq + idx*sIdx].  Stage with radix R
+// Element (idx, seq) lives at lds[off + seq*sSeq + idx*sIdx] (padded).  Stage with radix R
 // after Ns = product of the previous radices:
 //   for j < L/R, k = j mod Ns:
 //     v_r = x[j + r*L/R] * w_L^(r*k*L/(Ns*R))
@@ -202,7 +202,7 @@ template <int SIGN> struct Dft<8, SIGN> {
 // ---------------------------------------------------------------------------
 template <int T, int R, int SIGN, bool SEQ_FAST>
 __device__ __forceinline__ void fft_stage(double2* lds, const LucasStage& st, int nseq, unsigned int mseq, int sIdx,
-                                          int sSeq, const double2* __restrict__ tw, int tid) {
+                                          int sSeq, int off, const double2* tw, int tid) {
     constexpr int BM = (LUCAS_EMAX + R - 1) / R;
     const int Ns = st.Ns, nb = st.nb;
     const int total = nb * nseq;
@@ -222,17 +222,17 @@ __device__ __forceinline__ void fft_stage(double2* lds, const LucasStage& st, in
             }
             const int jq = udiv(j, Ns, st.mNs);
             const int k = j - jq * Ns;
-            const int base = seq * sSeq;
-            const double2* const twk = tw + st.twoff + k * (R - 1);  // this butterfly's R-1 twiddles
+            const int base = off + seq * sSeq;
+            // this butterfly's R-1 twiddles (LDS); the first stage's block holds
+            // 1+0i, a product that is exact, so no stage needs a branch here
+            const double2* const twk = tw + st.twoff + k * (R - 1);
+            double2 w[R - 1];
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                double2 x = lds[pidx(base + (j + r * nb) * sIdx)];
-                if (r > 0 && Ns > 1) {
-                    const double2 w = gload2(twk, r - 1);
-                    x = SIGN < 0 ? cmul(x, w) : cmulc(x, w);
-                }
-                v[i][r] = x;
-            }
+            for (int r = 0; r < R; ++r) v[i][r] = lds[pidx(base + (j + r * nb) * sIdx)];
+#pragma unroll
+            for (int r = 1; r < R; ++r) w[r - 1] = twk[r - 1];
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[i][r] = SIGN < 0 ? cmul(v[i][r], w[r - 1]) : cmulc(v[i][r], w[r - 1]);
             Dft<R, SIGN>::run(v[i]);
             outb[i] = base + (jq * Ns * R + k) * sIdx;
         }
@@ -254,16 +254,16 @@ __device__ __forceinline__ void fft_stage(double2* lds, const LucasStage& st, in
 // site each (one inlined copy per kernel).
 template <int T, int SIGN, bool SEQ_FAST>
 __device__ __forceinline__ void fft_lds(double2* lds, const LucasStage* stages, int nst, int nseq, unsigned int mseq,
-                                        int sIdx, int sSeq, const double2* __restrict__ tw, int tid) {
+                                        int sIdx, int sSeq, int off, const double2* tw, int tid) {
     for (int s = 0; s < nst; ++s) {
         const LucasStage st = stages[s];
         switch (st.R) {
-            case 2: fft_stage<T, 2, SIGN, SEQ_FAST>(lds, st, nseq, mseq, sIdx, sSeq, tw, tid); break;
-            case 3: fft_stage<T, 3, SIGN, SEQ_FAST>(lds, st, nseq, mseq, sIdx, sSeq, tw, tid); break;
-            case 4: fft_stage<T, 4, SIGN, SEQ_FAST>(lds, st, nseq, mseq, sIdx, sSeq, tw, tid); break;
-            case 5: fft_stage<T, 5, SIGN, SEQ_FAST>(lds, st, nseq, mseq, sIdx, sSeq, tw, tid); break;
-            case 7: fft_stage<T, 7, SIGN, SEQ_FAST>(lds, st, nseq, mseq, sIdx, sSeq, tw, tid); break;
-            default: fft_stage<T, 8, SIGN, SEQ_FAST>(lds, st, nseq, mseq, sIdx, sSeq, tw, tid); break;
+            case 2: fft_stage<T, 2, SIGN, SEQ_FAST>(lds, st, nseq, mseq, sIdx, sSeq, off, tw, tid); break;
+            case 3: fft_stage<T, 3, SIGN, SEQ_FAST>(lds, st, nseq, mseq, sIdx, sSeq, off, tw, tid); break;
+            case 4: fft_stage<T, 4, SIGN, SEQ_FAST>(lds, st, nseq, mseq, sIdx, sSeq, off, tw, tid); break;
+            case 5: fft_stage<T, 5, SIGN, SEQ_FAST>(lds, st, nseq, mseq, sIdx, sSeq, off, tw, tid); break;
+            case 7: fft_stage<T, 7, SIGN, SEQ_FAST>(lds, st, nseq, mseq, sIdx, sSeq, off, tw, tid); break;
+            default: fft_stage<T, 8, SIGN, SEQ_FAST>(lds, st, nseq, mseq, sIdx, sSeq, off, tw, tid); break;
         }
     }
 }
@@ -339,7 +339,7 @@ __device__ __forceinline__ int restore_digit(const unsigned long long* mag, cons
 // (loadIntToDoubleIBDWT, gpuLucas.cu:730-766).
 template <int NC>
 __device__ __forceinline__ int2 carry_element(const double2* lds, const LucasJobDesc& J, int r, int lc, int gc,
-                                              int ncl) {
+                                              int ncl, const int* rowr, const int* colr) {
     const int A = J.A, N = J.N, q = J.q, bb = J.b;
     unsigned long long mag[NC + 3];
     bool neg[NC + 3];
@@ -356,7 +356,7 @@ __device__ __forceinline__ int2 carry_element(const double2* lds, const LucasJob
         mag[k] = c < 0 ? (unsigned long long)(-c) : (unsigned long long)c;
     }
     // digit j is wide iff r_j < b, r_{j-1} = r_j + b (mod N)
-    int r2m = gload(J.rlo, 2 * gc) + gload(J.rhi, r);
+    int r2m = colr[2 * lc] + rowr[r];
     if (r2m >= N) r2m -= N;
     int rv = r2m - bb;
     if (rv < 0) rv += N;
@@ -380,7 +380,10 @@ __device__ __forceinline__ int2 carry_element(const double2* lds, const LucasJob
 
 // ---------------------------------------------------------------------------
 // Column kernel: one workgroup of T work-items per C-column tile of one job.
-// LDS: the (C+H) x A complex tile, then C x A int2 new digits.
+// The tile holds C+H columns, local column lc = global column c0-H+lc, element
+// (r, lc) at lds[pidx(r*(C+H) + lc)].  Everything that does not depend on the
+// data (stage twiddles, weight and r tables of the tile's rows and digits) is
+// copied into LDS while the tile's own loads are in flight.
 // ---------------------------------------------------------------------------
 template <int T>
 __device__ __forceinline__ void col_kernel_body(const LucasJobDesc* __restrict__ jobs, const int2* __restrict__ wgmap,
@@ -394,23 +397,26 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc* __restrict__
     const int tid = threadIdx.x;
     const int ncl = C + H;  // local columns: [0,H) halo, [H, H+C) own
     const unsigned int mncl = J.m_ncl, mC = J.m_C;
-    int2* const xdig = reinterpret_cast<int2*>(lds + pidx(A * ncl) + 1);  // new digits (2m, 2m+1)
-    float* const red = reinterpret_cast<float*>(xdig + A * C);   // per-wave roundoff slots
+    // LDS carve-up, in the order of lucas_col_lds_words()
+    double2* p = lds + pidx(A * ncl) + 1;
+    double2* const tws = p;  // stage twiddles of the length-A transform
+    p += J.ntwsA;
+    double2* const roww = p;  // {2^(rhi/N), 2^(-rhi/N)/(4M)} per row
+    p += A;
+    double2* const colw = p;  // {2^(rlo/N), 2^(-rlo/N)} per local digit 2*lc + e
+    p += 2 * ncl;
+    int* const rowr = reinterpret_cast<int*>(p);  // rhi per row
+    p += (A + 3) / 4;
+    int* const colr = reinterpret_cast<int*>(p);  // rlo per local digit
+    p += (2 * ncl + 3) / 4;
+    float* const red = reinterpret_cast<float*>(p);  // per-wave roundoff slots
     const unsigned int seen = *reinterpret_cast<volatile unsigned int*>(J.maxerr);
     const double2* const twA = J.twA;
     const double2* const twMB = J.twMB;
+    const bool in_digits = (mode & LUCAS_IN_DIGITS) != 0;
     float err = 0.0f;
 
-    // Pass 0 (inverse column FFT of the tile + halo, unweight, round, carry)
-    // runs unless the digits come from memory; pass 1 weights the new digits
-    // and runs the forward column FFT.  One FFT call site for both passes.
-    int pass = (mode & LUCAS_IN_DIGITS) ? 1 : 0;
-    if (pass == 1) {
-        for (int t = tid; t < A * C; t += T) {
-            const int r = udiv(t, C, mC), lo = t - r * C;
-            xdig[t] = gload2i(J.digits, (size_t)(c0 + lo) + (size_t)B * r);
-        }
-    } else {
+    if (!in_digits) {
         // load the tile with the inverse outer twiddle omega_M^(-c*r), conjugated
         for (int t = tid; t < A * ncl; t += T) {
             const int r = udiv(t, ncl, mncl), lc = t - r * ncl;
@@ -420,38 +426,50 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc* __restrict__
             lds[pidx(t)] = cconj(cmulc(v, outer_twiddle(J, twA, twMB, gc * r)));
         }
     }
-    for (;;) {
-        if (pass == 1) {
-            __syncthreads();
-            // weight: y_j = 2^(r_j/N) x_j into a compact C-column tile
-            for (int t = tid; t < A * C; t += T) {
-                const int r = udiv(t, C, mC), lo = t - r * C;
-                const int gc = c0 + lo;
-                const int rh = gload(J.rhi, r);
-                const double wh = gload(J.whi, r);
-                const int je = 2 * gc, jo = je + 1;
-                const double ae = gload(J.wlo, je) * wh * (gload(J.rlo, je) + rh >= N ? 0.5 : 1.0);
-                const double ao = gload(J.wlo, jo) * wh * (gload(J.rlo, jo) + rh >= N ? 0.5 : 1.0);
-                const int2 x = xdig[t];
-                lds[pidx(t)] = make_double2(ae * (double)x.x, ao * (double)x.y);
-            }
-        }
-        __syncthreads();
-        const int nseq = pass ? C : ncl;
-        fft_lds<T, -1, true>(lds, J.stA, J.nradA, nseq, pass ? mC : mncl, nseq, 1, J.twsA, tid);
-        if (pass == 1) break;
+    for (int t = tid; t < J.ntwsA; t += T) tws[t] = gload2(J.twsA, t);
+    for (int t = tid; t < A; t += T) {
+        roww[t] = make_double2(gload(J.whi, t), gload(J.iwhi, t));
+        rowr[t] = gload(J.rhi, t);
+    }
+    for (int t = tid; t < 2 * ncl; t += T) {
+        int gc = c0 - H + (t >> 1);
+        if (gc < 0) gc += B;
+        const int jj = 2 * gc + (t & 1);
+        colw[t] = make_double2(gload(J.wlo, jj), gload(J.iwlo, jj));
+        colr[t] = gload(J.rlo, jj);
+    }
+    __syncthreads();
 
+    // weight: (y_2m, y_2m+1) = 2^(r_j/N) (x_2m, x_2m+1) for own element (r, lo)
+    auto weighted = [&](int r, int lo, int2 x) {
+        const int e = 2 * (lo + H);
+        const int rh = rowr[r];
+        const double wh = roww[r].x;
+        const double ae = colw[e].x * wh * (colr[e] + rh >= N ? 0.5 : 1.0);
+        const double ao = colw[e + 1].x * wh * (colr[e + 1] + rh >= N ? 0.5 : 1.0);
+        return make_double2(ae * (double)x.x, ao * (double)x.y);
+    };
+
+    if (in_digits) {
+        for (int t = tid; t < A * C; t += T) {
+            const int r = udiv(t, C, mC), lo = t - r * C;
+            const int2 x = gload2i(J.digits, (size_t)(c0 + lo) + (size_t)B * r);
+            lds[pidx(r * ncl + H + lo)] = weighted(r, lo, x);
+        }
+    } else {
+        // inverse column FFT of the tile and halo
+        fft_lds<T, -1, true>(lds, J.stA, J.nradA, ncl, mncl, ncl, 1, 0, tws, tid);
         // unweight, -2 on digit 0, round half-even, roundoff
         for (int t = tid; t < A * ncl; t += T) {
             const int r = udiv(t, ncl, mncl), lc = t - r * ncl;
             int gc = c0 - H + lc;
             if (gc < 0) gc += B;
             const double2 v = cconj(lds[pidx(t)]);
-            const int rh = gload(J.rhi, r);
-            const double iwh = gload(J.iwhi, r);
-            const int je = 2 * gc, jo = je + 1;
-            const double ie = gload(J.iwlo, je) * iwh * (gload(J.rlo, je) + rh >= N ? 2.0 : 1.0);
-            const double io = gload(J.iwlo, jo) * iwh * (gload(J.rlo, jo) + rh >= N ? 2.0 : 1.0);
+            const int rh = rowr[r];
+            const double iwh = roww[r].y;
+            const int e = 2 * lc;
+            const double ie = colw[e].y * iwh * (colr[e] + rh >= N ? 2.0 : 1.0);
+            const double io = colw[e + 1].y * iwh * (colr[e + 1] + rh >= N ? 2.0 : 1.0);
             double ve = v.x * ie;
             const double vo = v.y * io;
             if (gc == 0 && r == 0) ve -= 2.0;
@@ -460,36 +478,49 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc* __restrict__
             lds[pidx(t)] = make_double2(re, ro);
         }
         __syncthreads();
-        // carry-save restoration + single ripple for the own columns
-        for (int t = tid; t < A * C; t += T) {
-            const int r = udiv(t, C, mC), lo = t - r * C;
-            const int lc = lo + H, gc = c0 + lo;
-            int2 x;
-            switch (n) {
-                case 2: x = carry_element<2>(lds, J, r, lc, gc, ncl); break;
-                case 3: x = carry_element<3>(lds, J, r, lc, gc, ncl); break;
-                case 4: x = carry_element<4>(lds, J, r, lc, gc, ncl); break;
-                case 5: x = carry_element<5>(lds, J, r, lc, gc, ncl); break;
-                default: x = carry_element<6>(lds, J, r, lc, gc, ncl); break;
+        // Carry-save restoration + single ripple for the own columns.  Element
+        // (r, lo) reads row r's columns at or left of its own, and for the wrap
+        // (global column 0) the halo of row r-1; T is a multiple of C, so the
+        // elements of a row share a pass of this loop.  After the barrier each
+        // work-item overwrites its own element with the weighted new digits:
+        // no later pass reads it, and halo columns are never written.
+        const int nper = (A * C + T - 1) / T;
+        for (int i = 0; i < nper; ++i) {
+            const int t = tid + i * T;
+            const bool own = t < A * C;
+            const int r = udiv(own ? t : 0, C, mC), lo = (own ? t : 0) - r * C;
+            int2 x = make_int2(0, 0);
+            if (own) {
+                const int lc = lo + H, gc = c0 + lo;
+                switch (n) {
+                    case 2: x = carry_element<2>(lds, J, r, lc, gc, ncl, rowr, colr); break;
+                    case 3: x = carry_element<3>(lds, J, r, lc, gc, ncl, rowr, colr); break;
+                    case 4: x = carry_element<4>(lds, J, r, lc, gc, ncl, rowr, colr); break;
+                    case 5: x = carry_element<5>(lds, J, r, lc, gc, ncl, rowr, colr); break;
+                    default: x = carry_element<6>(lds, J, r, lc, gc, ncl, rowr, colr); break;
+                }
             }
-            xdig[t] = x;
+            if (mode & LUCAS_OUT_DIGITS) {
+                if (own) gstore2i(J.digits, (size_t)(c0 + lo) + (size_t)B * r, x);
+                continue;
+            }
+            const double2 y = weighted(r, lo, x);
+            __syncthreads();
+            if (own) lds[pidx(r * ncl + H + lo)] = y;
         }
         if (mode & LUCAS_OUT_DIGITS) {
-            __syncthreads();
-            for (int t = tid; t < A * C; t += T) {
-                const int r = udiv(t, C, mC), lo = t - r * C;
-                gstore2i(J.digits, (size_t)(c0 + lo) + (size_t)B * r, xdig[t]);
-            }
             block_max_err<T>(err, red, seen, J.maxerr);
             return;
         }
-        pass = 1;
     }
-    // outer twiddle omega_M^(c*k) and store row-major
+    __syncthreads();
+    // forward column FFT of the own columns
+    fft_lds<T, -1, true>(lds, J.stA, J.nradA, C, mC, ncl, 1, H, tws, tid);
+    // store row-major; the outer twiddle omega_M^(c*k) is applied by the row
+    // kernel as it loads
     for (int t = tid; t < A * C; t += T) {
         const int k = udiv(t, C, mC), lo = t - k * C;
-        const int gc = c0 + lo;
-        gstore2(J.spec, (size_t)k * B + gc, cmul(lds[pidx(t)], outer_twiddle(J, twA, twMB, gc * k)));
+        gstore2(J.spec, (size_t)k * B + c0 + lo, lds[pidx(k * ncl + H + lo)]);
     }
     block_max_err<T>(err, red, seen, J.maxerr);
 }
@@ -516,18 +547,25 @@ __device__ __forceinline__ void row_kernel_body(const LucasJobDesc* __restrict__
     const bool self = (r0 == 0) || (2 * r0 == A);
     const int r1 = self ? r0 : A - r0;
     const int nrows = self ? 1 : 2;
-    const double2* const twB = J.twB;
+    // LDS carve-up, in the order of lucas_row_lds_words()
+    double2* const tws = lds + pidx(2 * B) + 1;  // stage twiddles of the length-B transform
+    double2* const twu = tws + J.ntwsB;          // omega_B^t, t < B (untangle)
 
+    // load both rows with the column kernel's outer twiddle omega_M^(c*k);
+    // the tables go to LDS while those loads are in flight
     for (int t = tid; t < nrows * B; t += T) {
         const int rl = t >= B ? 1 : 0, kc = t - rl * B;
-        lds[pidx(t)] = gload2(J.spec, (size_t)(rl ? r1 : r0) * B + kc);
+        const int row = rl ? r1 : r0;
+        lds[pidx(t)] = cmul(gload2(J.spec, (size_t)row * B + kc), outer_twiddle(J, J.twA, J.twMB, kc * row));
     }
+    for (int t = tid; t < J.ntwsB; t += T) tws[t] = gload2(J.twsB, t);
+    for (int t = tid; t < B; t += T) twu[t] = gload2(J.twB, t);
+    const double2 wr = gload2(J.twMA, r0);
     // pass 0: forward row FFT; pass 1: inverse row FFT (conjugated input)
     for (int pass = 0;; ++pass) {
         __syncthreads();
-        fft_lds<T, -1, false>(lds, J.stB, J.nradB, nrows, 0u, 1, B, J.twsB, tid);
+        fft_lds<T, -1, false>(lds, J.stB, J.nradB, nrows, 0u, 1, B, 0, tws, tid);
         if (pass == 1) break;
-        const double2 wr = gload2(J.twMA, r0);
         // pairs (kc in row r0) <-> (pc in row r1): rows differ, or row r0 pairs
         // with itself (row 0: pc = -kc mod B; row A/2: pc = B-1-kc)
         const int last = !self ? B - 1 : (r0 == 0 ? B / 2 : (B - 1) / 2);
@@ -536,7 +574,7 @@ __device__ __forceinline__ void row_kernel_body(const LucasJobDesc* __restrict__
             int pc = (self && r0 == 0) ? B - kc : B - 1 - kc;
             if (pc == B) pc = 0;
             const double2 zk = lds[pidx(kc)], zm = lds[pidx(poff + pc)];
-            const double2 w = cmul(wr, gload2(twB, kc));
+            const double2 w = cmul(wr, twu[kc]);
             const double2 E = cadd(zk, cconj(zm));
             const double2 O = mul_si<-1>(csub(zk, cconj(zm)));
             const double2 P = cadd(cmul(E, E), cmul(w, cmul(O, O)));

@@ -24,20 +24,23 @@ def build():
 def lib():
     global _lib
     if _lib is None:
-        newest = max(os.path.getmtime(s) for s in SOURCES + [os.path.join(HERE, "hip", "hip_runtime.h")])
+        headers = [os.path.join(HERE, "hip", "hip_runtime.h"),
+                   os.path.join(ROOT, "gpulucas_amd", "csrc", "lucas_desc.h"),
+                   os.path.join(ROOT, "gpulucas_amd", "csrc", "lucas_host.h")]
+        newest = max(os.path.getmtime(s) for s in SOURCES + headers)
         if not os.path.exists(LIB) or os.path.getmtime(LIB) < newest:
             build()
         _lib = C.CDLL(LIB)
-        _lib.emu_run.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_longlong, C.c_void_p,
+        _lib.emu_run.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_longlong, C.c_void_p,
                                  C.POINTER(C.c_double)]
     return _lib
 
 
-def run(p, N, digits, iters, A=0, C_=0):
+def run(p, N, digits, iters, A=0, C_=0, T=0):
     x = np.ascontiguousarray(digits, dtype=np.int32)
     out = np.zeros(N, dtype=np.int32)
     err = C.c_double(0)
-    rc = lib().emu_run(p, N, A, C_, x.ctypes.data_as(C.c_void_p), iters, out.ctypes.data_as(C.c_void_p),
+    rc = lib().emu_run(p, N, A, C_, T, x.ctypes.data_as(C.c_void_p), iters, out.ctypes.data_as(C.c_void_p),
                        C.byref(err))
     assert rc == 0, "plan refused"
     return out, err.value

@@ -42,11 +42,11 @@ hipError_t launch_col_kernel(int, const LucasJobDesc*, const int2*, int, size_t,
 hipError_t launch_row_kernel(int, const LucasJobDesc*, const int2*, int, size_t, hipStream_t);
 }
 
-extern "C" int emu_run(int p, int N, int A, int C, const int32_t* digits_in, long long iters, int32_t* digits_out,
-                       double* maxerr) {
+extern "C" int emu_run(int p, int N, int A, int C, int T, const int32_t* digits_in, long long iters,
+                       int32_t* digits_out, double* maxerr) {
     using namespace lucas;
     Plan pl;
-    if (!make_plan(p, N, A, C, &pl).empty()) return -1;
+    if (!(T ? make_plan_exact(p, N, A, C, T, 0, &pl) : make_plan(p, N, A, C, &pl)).empty()) return -1;
     Tables t;
     make_tables(pl, &t);
     std::vector<double2> spec(pl.M), spec2(pl.M);

@@ -8,16 +8,17 @@ import emu
 import oracle
 
 
-@pytest.mark.parametrize("p,N,A,C,iters", [
-    (1279, 64, 4, 1, 3),     # C < halo: halo spans a previous tile and a row wrap
-    (1279, 64, 8, 4, 3),     # single column tile
-    (4423, 240, 15, 8, 2),   # odd A (radix 3, 5), one tile
-    (9941, 560, 7, 4, 2),    # radix 7 columns
-    (2203, 120, 0, 0, 2),    # planner's own choice
+@pytest.mark.parametrize("p,N,A,C,iters,T", [
+    (1279, 64, 4, 1, 3, 0),       # C < halo: halo spans a previous tile and a row wrap
+    (1279, 64, 8, 4, 3, 0),       # single column tile
+    (4423, 240, 15, 8, 2, 0),     # odd A (radix 3, 5), one tile
+    (9941, 560, 7, 4, 2, 0),      # radix 7 columns
+    (2203, 120, 0, 0, 2, 0),      # planner's own choice
+    (44497, 2560, 40, 16, 2, 256),  # A*C > T: the carry runs in three passes
 ])
-def test_emulated_kernels_match_oracle(p, N, A, C, iters):
+def test_emulated_kernels_match_oracle(p, N, A, C, iters, T):
     x = oracle.s0_digits(N)
-    got, err = emu.run(p, N, x, iters, A, C)
+    got, err = emu.run(p, N, x, iters, A, C, T)
     want, _ = oracle.ibdwt_iterate(p, N, x, iters)
     assert np.array_equal(got, want)
     assert err < 0.1
