@@ -18,7 +18,7 @@
 #define LUCAS_MAX_RADICES 16
 // Elements (complex words) each work-item may hold while an FFT stage is in
 // flight; a tile of a workgroup with T work-items holds at most EMAX*T words.
-#define LUCAS_EMAX 8
+#define LUCAS_EMAX 4
 // Work-items per workgroup: kernels are built for 256, 512 and 1024.
 #define LUCAS_THREADS_SMALL 256
 #define LUCAS_THREADS_LARGE 512
@@ -52,9 +52,15 @@ struct LucasJobDesc {
     int threads;            // work-items per workgroup (256, 512 or 1024)
     int nradA, nradB;
     int ntwsA, ntwsB;       // entries of the two stage twiddle tables
-    LucasStage stA[LUCAS_MAX_RADICES];  // column transform stages (length A)
-    LucasStage stB[LUCAS_MAX_RADICES];  // row transform stages (length B)
+    // Each transform of length L = L1 * W runs as Stockham passes of length
+    // L1 in LDS and a W = 64P point wave-register factor (P = 0: no wave
+    // factor, L1 = L).
+    int pA, pB;             // P of the column (length A) and row (length B) transforms
+    LucasStage stA[LUCAS_MAX_RADICES];  // column transform Stockham stages (length A1)
+    LucasStage stB[LUCAS_MAX_RADICES];  // row transform Stockham stages (length B1)
     unsigned int m_ncl, m_C;             // reciprocals of C+halo and C
+    unsigned int m_A1, m_B1;             // reciprocals of A1 and B1
+    unsigned int m_sqA0, m_sqA1;         // reciprocals of (C+halo)*64pA and C*64pA
     double invB;                         // 1/B
     // Twiddle tables (forward sign, exp(-2*pi*i*t/L)), rounded from long double.
     const double2* twA;     // omega_A^t, t < A
@@ -81,11 +87,15 @@ struct LucasJobDesc {
 // carves its LDS in the order listed.
 //   column kernel: padded (C+H) x A tile; stage twiddles; per-row
 //     {2^(rhi/N), 2^(-rhi/N)/(4M)}; per local digit {2^(rlo/N), 2^(-rlo/N)};
-//     per-row rhi; per local digit rlo; roundoff slots
-//   row kernel: two padded rows; stage twiddles; omega_B^t, t < B
+//     per-row rhi; per local digit rlo; roundoff slots; stage table;
+//     omega_A^t, t < A (with a wave factor only)
+//   row kernel: two padded rows; stage twiddles; omega_B^t, t < B; stage table
 __host__ __device__ inline int lucas_pad_words(int n) { return n + (n >> 3) + 1; }
-__host__ __device__ inline int lucas_col_lds_words(int A, int C, int H, int ntws) {
+__host__ __device__ inline int lucas_col_lds_words(int A, int C, int H, int ntws, int nst, int pA) {
     const int ncl = C + H;
-    return lucas_pad_words(A * ncl) + ntws + A + 2 * ncl + (A + 3) / 4 + (2 * ncl + 3) / 4 + 4;
+    return lucas_pad_words(A * ncl) + ntws + A + 2 * ncl + (A + 3) / 4 + (2 * ncl + 3) / 4 + 4 + 2 * nst +
+           (pA ? A : 0);
 }
-__host__ __device__ inline int lucas_row_lds_words(int B, int ntws) { return lucas_pad_words(2 * B) + ntws + B; }
+__host__ __device__ inline int lucas_row_lds_words(int B, int ntws, int nst) {
+    return lucas_pad_words(2 * B) + ntws + B + 2 * nst;
+}

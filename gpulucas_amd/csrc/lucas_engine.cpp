@@ -11,13 +11,15 @@
 
 #include <string.h>
 
+#include <algorithm>
+
 #include "gpulucas.h"
 
 namespace lucas {
 
-hipError_t launch_col_kernel(int threads, const LucasJobDesc* jobs, const int2* map, int nwg, size_t lds, int mode,
-                             hipStream_t stream);
-hipError_t launch_row_kernel(int threads, const LucasJobDesc* jobs, const int2* map, int nwg, size_t lds,
+hipError_t launch_col_kernel(int threads, int p, const LucasJobDesc* jobs, const int2* map, int nwg, size_t lds,
+                             int mode, hipStream_t stream);
+hipError_t launch_row_kernel(int threads, int p, const LucasJobDesc* jobs, const int2* map, int nwg, size_t lds,
                              hipStream_t stream);
 hipError_t configure_kernels(size_t max_lds);
 
@@ -159,10 +161,9 @@ Engine::~Engine() {
     if (gexec_) hipGraphExecDestroy(gexec_);
     if (graph_) hipGraphDestroy(graph_);
     if (d_desc_) hipFree(d_desc_);
-    for (Group& g : groups_) {
-        if (g.d_colmap) hipFree(g.d_colmap);
-        if (g.d_rowmap) hipFree(g.d_rowmap);
-    }
+    for (Group* gs : {colg_, rowg_})
+        for (int i = 0; i < kGroups; ++i)
+            if (gs[i].d_map) hipFree(gs[i].d_map);
     if (ev0_) hipEventDestroy(ev0_);
     if (ev1_) hipEventDestroy(ev1_);
     if (stream_) hipStreamDestroy(stream_);
@@ -199,20 +200,26 @@ int Engine::prepare() {
         graph_ = nullptr;
     }
     std::vector<LucasJobDesc> desc;
-    std::vector<int2> cmap[3], rmap[3];
-    groups_[0].threads = LUCAS_THREADS_SMALL;
-    groups_[1].threads = LUCAS_THREADS_LARGE;
-    groups_[2].threads = LUCAS_THREADS_MAX;
-    for (Group& g : groups_) g.lds_col = g.lds_row = 0;
+    std::vector<int2> cmap[kGroups], rmap[kGroups];
+    static const int kThreads[3] = {LUCAS_THREADS_SMALL, LUCAS_THREADS_LARGE, LUCAS_THREADS_MAX};
+    static const int kP[4] = {0, 1, 2, 4};
+    for (int i = 0; i < kGroups; ++i) {
+        colg_[i].threads = rowg_[i].threads = kThreads[i / 4];
+        colg_[i].p = rowg_[i].p = kP[i % 4];
+        colg_[i].lds = rowg_[i].lds = 0;
+    }
+    auto group_of = [](int threads, int p) {
+        const int ti = threads == LUCAS_THREADS_MAX ? 2 : (threads == LUCAS_THREADS_LARGE ? 1 : 0);
+        return 4 * ti + (p == 4 ? 3 : p);
+    };
     for (size_t i = 0; i < jobs_.size(); ++i) {
         const Job* j = jobs_[i];
-        const int gi = j->plan.threads == LUCAS_THREADS_MAX ? 2 : (j->plan.threads == LUCAS_THREADS_LARGE ? 1 : 0);
-        Group& g = groups_[gi];
         desc.push_back(j->desc);
-        for (int w = 0; w < j->plan.col_wgs; ++w) cmap[gi].push_back(make_int2((int)i, w));
-        for (int w = 0; w < j->plan.row_wgs; ++w) rmap[gi].push_back(make_int2((int)i, w));
-        if (j->plan.lds_col > g.lds_col) g.lds_col = j->plan.lds_col;
-        if (j->plan.lds_row > g.lds_row) g.lds_row = j->plan.lds_row;
+        const int gc = group_of(j->plan.threads, j->plan.wA / 64), gr = group_of(j->plan.threads, j->plan.wB / 64);
+        for (int w = 0; w < j->plan.col_wgs; ++w) cmap[gc].push_back(make_int2((int)i, w));
+        for (int w = 0; w < j->plan.row_wgs; ++w) rmap[gr].push_back(make_int2((int)i, w));
+        colg_[gc].lds = std::max(colg_[gc].lds, j->plan.lds_col);
+        rowg_[gr].lds = std::max(rowg_[gr].lds, j->plan.lds_row);
     }
     auto grow = [&](void** ptr, size_t* cap, size_t need) -> bool {
         if (need <= *cap) return true;
@@ -228,17 +235,14 @@ int Engine::prepare() {
     hipError_t e = hipSuccess;
     if (!desc.empty())
         e = hipMemcpyAsync(d_desc_, desc.data(), sizeof(LucasJobDesc) * desc.size(), hipMemcpyHostToDevice, stream_);
-    for (int gi = 0; gi < 3 && e == hipSuccess; ++gi) {
-        Group& g = groups_[gi];
-        if (!grow((void**)&g.d_colmap, &g.cap_col, sizeof(int2) * (cmap[gi].size() + 1)) ||
-            !grow((void**)&g.d_rowmap, &g.cap_row, sizeof(int2) * (rmap[gi].size() + 1)))
-            return LUCAS_ERR_OOM;
-        g.ncol = (int)cmap[gi].size();
-        g.nrow = (int)rmap[gi].size();
-        if (g.ncol)
-            e = hipMemcpyAsync(g.d_colmap, cmap[gi].data(), sizeof(int2) * g.ncol, hipMemcpyHostToDevice, stream_);
-        if (e == hipSuccess && g.nrow)
-            e = hipMemcpyAsync(g.d_rowmap, rmap[gi].data(), sizeof(int2) * g.nrow, hipMemcpyHostToDevice, stream_);
+    for (int gi = 0; gi < kGroups && e == hipSuccess; ++gi) {
+        for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+            Group& g = k == 0 ? colg_[gi] : rowg_[gi];
+            const std::vector<int2>& m = k == 0 ? cmap[gi] : rmap[gi];
+            if (!grow((void**)&g.d_map, &g.cap, sizeof(int2) * (m.size() + 1))) return LUCAS_ERR_OOM;
+            g.n = (int)m.size();
+            if (g.n) e = hipMemcpyAsync(g.d_map, m.data(), sizeof(int2) * g.n, hipMemcpyHostToDevice, stream_);
+        }
     }
     if (e == hipSuccess) e = hipStreamSynchronize(stream_);
     if (e != hipSuccess) return LUCAS_ERR_DEVICE;
@@ -247,18 +251,18 @@ int Engine::prepare() {
 }
 
 int Engine::launch_col(int mode) {
-    for (const Group& g : groups_) {
-        if (!g.ncol) continue;
-        if (launch_col_kernel(g.threads, d_desc_, g.d_colmap, g.ncol, g.lds_col, mode, stream_) != hipSuccess)
+    for (const Group& g : colg_) {
+        if (!g.n) continue;
+        if (launch_col_kernel(g.threads, g.p, d_desc_, g.d_map, g.n, g.lds, mode, stream_) != hipSuccess)
             return LUCAS_ERR_DEVICE;
     }
     return LUCAS_OK;
 }
 
 int Engine::launch_row() {
-    for (const Group& g : groups_) {
-        if (!g.nrow) continue;
-        if (launch_row_kernel(g.threads, d_desc_, g.d_rowmap, g.nrow, g.lds_row, stream_) != hipSuccess)
+    for (const Group& g : rowg_) {
+        if (!g.n) continue;
+        if (launch_row_kernel(g.threads, g.p, d_desc_, g.d_map, g.n, g.lds, stream_) != hipSuccess)
             return LUCAS_ERR_DEVICE;
     }
     return LUCAS_OK;

@@ -55,14 +55,15 @@ private:
     int launch_row();
     int build_graph();
 
-    // Jobs whose tiles need the same workgroup size share one launch.
+    // Jobs whose kernel has the same workgroup size and wave factor share
+    // one launch; column and row launches are grouped separately.
+    static const int kGroups = 12;  // 3 workgroup sizes x 4 wave factors (P = 0, 1, 2, 4)
     struct Group {
-        int threads = 0;
-        int2* d_colmap = nullptr;
-        int2* d_rowmap = nullptr;
-        size_t cap_col = 0, cap_row = 0;
-        int ncol = 0, nrow = 0;
-        size_t lds_col = 0, lds_row = 0;
+        int threads = 0, p = 0;
+        int2* d_map = nullptr;
+        size_t cap = 0;
+        int n = 0;
+        size_t lds = 0;
     };
 
     int device_;
@@ -72,7 +73,7 @@ private:
     bool dirty_ = true;
     LucasJobDesc* d_desc_ = nullptr;
     size_t cap_desc_ = 0;
-    Group groups_[3];
+    Group colg_[kGroups], rowg_[kGroups];
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t gexec_ = nullptr;
     int graph_iters_ = 0;

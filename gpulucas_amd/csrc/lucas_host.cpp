@@ -125,6 +125,15 @@ static long long model_cost(const Plan& pl) {
     return 64 * (col_t + row_t + coal) + total / 64;
 }
 
+// Wave-register factor W (256, 128 or 64) of a length-L transform over nseq
+// sequences with T work-items: W | L, and the nseq*L/W sub-transforms fit
+// one per wave of the workgroup.  0: none.
+static int wave_factor(int L, int nseq, int T) {
+    for (int W : {256, 128, 64})
+        if (L % W == 0 && (long long)nseq * (L / W) <= T / 64) return W;
+    return 0;
+}
+
 std::vector<Plan> plan_candidates(int p, int N) {
     std::vector<Plan> out;
     Plan pl;
@@ -133,37 +142,43 @@ std::vector<Plan> plan_candidates(int p, int N) {
         if (pl.M % A) continue;
         const int B = pl.M / A;
         if (B < pl.halo) continue;
-        for (int pref : {8, 4}) {
-        std::vector<int> ra = radix_plan(A, pref), rb = radix_plan(B, pref);
-        if (pref == 4 && ra == radix_plan(A, 8) && rb == radix_plan(B, 8)) continue;
-        if (ra.empty() || rb.empty()) continue;
-        if ((int)ra.size() > LUCAS_MAX_RADICES || (int)rb.size() > LUCAS_MAX_RADICES) continue;
+        if (radix_plan(A, 8).empty() || radix_plan(B, 8).empty()) continue;
         for (int C : {1, 2, 4, 8, 16, 32, 64}) {
             if (B % C) continue;
             const long long tile = std::max<long long>((long long)A * (C + pl.halo), 2LL * B);
-            const size_t lds_col =
-                    16 * (size_t)lucas_col_lds_words(A, C, pl.halo, stage_twiddle_count(A, ra));
-            const size_t lds_row = 16 * (size_t)lucas_row_lds_words(B, stage_twiddle_count(B, rb));
-            if (lds_col > kMaxLds || lds_row > kMaxLds) continue;
             for (int T : {LUCAS_THREADS_SMALL, LUCAS_THREADS_LARGE, LUCAS_THREADS_MAX}) {
-            if (tile > (long long)LUCAS_EMAX * T) continue;
-            // a workgroup more than twice the tile would leave most work-items idle
-            if (T > LUCAS_THREADS_SMALL && tile < T / 2) continue;
-            Plan c = pl;
-            c.A = A;
-            c.B = B;
-            c.C = C;
-            c.threads = T;
-            c.radA = ra;
-            c.radB = rb;
-            c.col_wgs = B / C;
-            c.row_wgs = A / 2 + 1;
-            c.lds_col = lds_col;
-            c.lds_row = lds_row;
-            c.radix_pref = pref;
-            out.push_back(c);
+                if (tile > (long long)LUCAS_EMAX * T) continue;
+                // a workgroup more than twice the tile would leave most work-items idle
+                if (T > LUCAS_THREADS_SMALL && tile < T / 2) continue;
+                const int wA = wave_factor(A, C + pl.halo, T), wB = wave_factor(B, 2, T);
+                const int A1 = wA ? A / wA : A, B1 = wB ? B / wB : B;
+                for (int pref : {8, 4}) {
+                    std::vector<int> ra = radix_plan(A1, pref), rb = radix_plan(B1, pref);
+                    if (pref == 4 && ra == radix_plan(A1, 8) && rb == radix_plan(B1, 8)) continue;
+                    if ((ra.empty() && A1 > 1) || (rb.empty() && B1 > 1)) continue;
+                    if ((int)ra.size() > LUCAS_MAX_RADICES || (int)rb.size() > LUCAS_MAX_RADICES) continue;
+                    const size_t lds_col = 16 * (size_t)lucas_col_lds_words(
+                            A, C, pl.halo, stage_twiddle_count(A1, ra), (int)ra.size(), wA / 64);
+                    const size_t lds_row =
+                            16 * (size_t)lucas_row_lds_words(B, stage_twiddle_count(B1, rb), (int)rb.size());
+                    if (lds_col > kMaxLds || lds_row > kMaxLds) continue;
+                    Plan c = pl;
+                    c.A = A;
+                    c.B = B;
+                    c.C = C;
+                    c.threads = T;
+                    c.wA = wA;
+                    c.wB = wB;
+                    c.radA = ra;
+                    c.radB = rb;
+                    c.col_wgs = B / C;
+                    c.row_wgs = A / 2 + 1;
+                    c.lds_col = lds_col;
+                    c.lds_row = lds_row;
+                    c.radix_pref = pref;
+                    out.push_back(c);
+                }
             }
-        }
         }
     }
     std::stable_sort(out.begin(), out.end(), [](const Plan& a, const Plan& b) { return model_cost(a) < model_cost(b); });
@@ -293,12 +308,19 @@ void fill_desc_shape(const Plan& plan, LucasJobDesc* d) {
     d->col_wgs = plan.col_wgs;
     d->row_wgs = plan.row_wgs;
     d->threads = plan.threads;
+    const int A1 = plan.wA ? plan.A / plan.wA : plan.A, B1 = plan.wB ? plan.B / plan.wB : plan.B;
     d->nradA = (int)plan.radA.size();
     d->nradB = (int)plan.radB.size();
-    d->ntwsA = stage_twiddle_count(plan.A, plan.radA);
-    d->ntwsB = stage_twiddle_count(plan.B, plan.radB);
-    stage_table(plan.A, plan.radA, d->stA);
-    stage_table(plan.B, plan.radB, d->stB);
+    d->ntwsA = stage_twiddle_count(A1, plan.radA);
+    d->ntwsB = stage_twiddle_count(B1, plan.radB);
+    d->pA = plan.wA / 64;
+    d->pB = plan.wB / 64;
+    stage_table(A1, plan.radA, d->stA);
+    stage_table(B1, plan.radB, d->stB);
+    d->m_A1 = magic_u32((unsigned)A1);
+    d->m_B1 = magic_u32((unsigned)B1);
+    d->m_sqA0 = magic_u32((unsigned)((plan.C + plan.halo) * std::max(1, plan.wA)));
+    d->m_sqA1 = magic_u32((unsigned)(plan.C * std::max(1, plan.wA)));
     d->m_ncl = magic_u32((unsigned)(plan.C + plan.halo));
     d->m_C = magic_u32((unsigned)plan.C);
     d->invB = 1.0 / (double)plan.B;
@@ -351,8 +373,8 @@ static void stage_twiddles(int L, const std::vector<int>& rad, std::vector<doubl
 }
 
 void make_tables(const Plan& pl, Tables* t) {
-    stage_twiddles(pl.A, pl.radA, &t->twsA);
-    stage_twiddles(pl.B, pl.radB, &t->twsB);
+    stage_twiddles(pl.wA ? pl.A / pl.wA : pl.A, pl.radA, &t->twsA);
+    stage_twiddles(pl.wB ? pl.B / pl.wB : pl.B, pl.radB, &t->twsB);
     root_table(pl.A, pl.A, &t->twA);
     root_table(pl.B, pl.B, &t->twB);
     root_table(pl.A, pl.M, &t->twMA);

@@ -190,22 +190,45 @@ template <int SIGN> struct Dft<8, SIGN> {
 };
 
 // ---------------------------------------------------------------------------
-// One Stockham autosort stage over `nseq` sequences of length L held in LDS.
-// Element (idx, seq) lives at lds[off + seq*sSeq + idx*sIdx] (padded).  Stage with radix R
-// after Ns = product of the previous radices:
+// Sequences in the LDS tile.  Sequence q (q < nsq) starts at word
+//   base(q) = off + (q mod Q1)*s1 + (q div Q1)*s2     (Q1 = 0: off + q*s1)
+// and its element j is word base(q) + j*sIdx (before padding).
+// ---------------------------------------------------------------------------
+struct SeqGeom {
+    int nsq;
+    unsigned int msq;  // reciprocal of nsq for udiv
+    int Q1;
+    unsigned int mQ1;
+    int s1, s2, off, sIdx;
+};
+
+__device__ __forceinline__ int seq_base(const SeqGeom& g, int q) {
+    if (g.Q1 == 0) return g.off + q * g.s1;
+    const int hi = udiv(q, g.Q1, g.mQ1);
+    return g.off + (q - hi * g.Q1) * g.s1 + hi * g.s2;
+}
+
+// Reciprocal of a power of two for udiv (2^k: t / 2^k = umulhi(t, 2^(32-k))).
+__device__ __forceinline__ unsigned int pow2_recip(int d) {
+    return d <= 1 ? 0u : (1u << (32 - (31 - __clz(d))));
+}
+
+// ---------------------------------------------------------------------------
+// One Stockham autosort stage over the sequences of g, each of length L.
+// Stage with radix R after Ns = product of the previous radices:
 //   for j < L/R, k = j mod Ns:
 //     v_r = x[j + r*L/R] * w_L^(r*k*L/(Ns*R))
 //     y[(j/Ns)*Ns*R + k + s*Ns] = DFT_R(v)_s
 // Reads of one stage all happen before its writes (registers, then barrier),
 // so one LDS buffer serves all stages.  SEQ_FAST maps consecutive threads to
-// consecutive sequences (column tiles) instead of consecutive butterflies.
+// consecutive sequences instead of consecutive butterflies.
 // ---------------------------------------------------------------------------
 template <int T, int R, int SIGN, bool SEQ_FAST>
-__device__ __forceinline__ void fft_stage(double2* lds, const LucasStage& st, int nseq, unsigned int mseq, int sIdx,
-                                          int sSeq, int off, const double2* tw, int tid) {
+__device__ __forceinline__ void fft_stage(double2* lds, const LucasStage& st, const SeqGeom& g, const double2* tw,
+                                          int tid) {
     constexpr int BM = (LUCAS_EMAX + R - 1) / R;
-    const int Ns = st.Ns, nb = st.nb;
-    const int total = nb * nseq;
+    const int Ns = st.Ns, nb = st.nb, sIdx = g.sIdx;
+    const int total = nb * g.nsq;
     double2 v[BM][R];
     int outb[BM];
 #pragma unroll
@@ -214,15 +237,15 @@ __device__ __forceinline__ void fft_stage(double2* lds, const LucasStage& st, in
         if (t < total) {
             int seq, j;
             if (SEQ_FAST) {
-                j = udiv(t, nseq, mseq);
-                seq = t - j * nseq;
+                j = udiv(t, g.nsq, g.msq);
+                seq = t - j * g.nsq;
             } else {
                 seq = udiv(t, nb, st.mnb);
                 j = t - seq * nb;
             }
             const int jq = udiv(j, Ns, st.mNs);
             const int k = j - jq * Ns;
-            const int base = off + seq * sSeq;
+            const int base = seq_base(g, seq);
             // this butterfly's R-1 twiddles (LDS); the first stage's block holds
             // 1+0i, a product that is exact, so no stage needs a branch here
             const double2* const twk = tw + st.twoff + k * (R - 1);
@@ -249,22 +272,180 @@ __device__ __forceinline__ void fft_stage(double2* lds, const LucasStage& st, in
     __syncthreads();
 }
 
+// Stage s of a table copied into LDS, made wave-uniform (scalar registers)
+// so the radix dispatch below is a scalar branch.
+__device__ __forceinline__ LucasStage uniform_stage(const LucasStage* lst, int s) {
+    const int4* q = reinterpret_cast<const int4*>(lst + s);
+    const int4 a = q[0], b = q[1];
+    LucasStage st;
+    st.R = __builtin_amdgcn_readfirstlane(a.x);
+    st.Ns = __builtin_amdgcn_readfirstlane(a.y);
+    st.nb = __builtin_amdgcn_readfirstlane(a.z);
+    st.twstep = __builtin_amdgcn_readfirstlane(a.w);
+    st.mNs = (unsigned int)__builtin_amdgcn_readfirstlane(b.x);
+    st.mnb = (unsigned int)__builtin_amdgcn_readfirstlane(b.y);
+    st.twoff = __builtin_amdgcn_readfirstlane(b.z);
+    st.pad = 0;
+    return st;
+}
+
+// Copy a stage table (nst entries of 8 ints) from the job descriptor to LDS.
+template <int T>
+__device__ __forceinline__ void copy_stages(LucasStage* lst, const LucasStage* src, int nst, int tid) {
+    int* const d = reinterpret_cast<int*>(lst);
+    const int* const s = reinterpret_cast<const int*>(src);
+    for (int t = tid; t < 8 * nst; t += T) d[t] = s[t];
+}
+
 // Forward transforms only: an inverse transform is conj(DFT(conj(x))), so the
-// kernels conjugate on the way in and out and reach this code from one call
-// site each (one inlined copy per kernel).
+// kernels conjugate on the way in and out.
 template <int T, int SIGN, bool SEQ_FAST>
-__device__ __forceinline__ void fft_lds(double2* lds, const LucasStage* stages, int nst, int nseq, unsigned int mseq,
-                                        int sIdx, int sSeq, int off, const double2* tw, int tid) {
+__device__ __forceinline__ void fft_lds(double2* lds, const LucasStage* stages, int nst, const SeqGeom& g,
+                                        const double2* tw, int tid) {
     for (int s = 0; s < nst; ++s) {
-        const LucasStage st = stages[s];
+        const LucasStage st = uniform_stage(stages, s);
         switch (st.R) {
-            case 2: fft_stage<T, 2, SIGN, SEQ_FAST>(lds, st, nseq, mseq, sIdx, sSeq, off, tw, tid); break;
-            case 3: fft_stage<T, 3, SIGN, SEQ_FAST>(lds, st, nseq, mseq, sIdx, sSeq, off, tw, tid); break;
-            case 4: fft_stage<T, 4, SIGN, SEQ_FAST>(lds, st, nseq, mseq, sIdx, sSeq, off, tw, tid); break;
-            case 5: fft_stage<T, 5, SIGN, SEQ_FAST>(lds, st, nseq, mseq, sIdx, sSeq, off, tw, tid); break;
-            case 7: fft_stage<T, 7, SIGN, SEQ_FAST>(lds, st, nseq, mseq, sIdx, sSeq, off, tw, tid); break;
-            default: fft_stage<T, 8, SIGN, SEQ_FAST>(lds, st, nseq, mseq, sIdx, sSeq, off, tw, tid); break;
+            case 2: fft_stage<T, 2, SIGN, SEQ_FAST>(lds, st, g, tw, tid); break;
+            case 3: fft_stage<T, 3, SIGN, SEQ_FAST>(lds, st, g, tw, tid); break;
+            case 4: fft_stage<T, 4, SIGN, SEQ_FAST>(lds, st, g, tw, tid); break;
+            case 5: fft_stage<T, 5, SIGN, SEQ_FAST>(lds, st, g, tw, tid); break;
+            case 7: fft_stage<T, 7, SIGN, SEQ_FAST>(lds, st, g, tw, tid); break;
+            default: fft_stage<T, 8, SIGN, SEQ_FAST>(lds, st, g, tw, tid); break;
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Wave-register transforms of W = 64P points: one wave64 holds a sequence as
+// v[m] = element 64m + lane and runs the radix-2 decimation-in-frequency
+// levels in registers (half-sizes >= 64) and across lanes (32 .. 1) with no
+// LDS traffic and no workgroup barrier.
+// ---------------------------------------------------------------------------
+
+// Value held by lane ^ MASK of the same wave: DPP quad permutes (1, 2), DPP
+// row rotate by 8 (8), ds_swizzle bit mode within 32 lanes (4, 16),
+// ds_bpermute (32).
+template <int MASK>
+__device__ __forceinline__ int lane_xor_i(int x) {
+    if constexpr (MASK == 1) return __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false);
+    else if constexpr (MASK == 2) return __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false);
+    else if constexpr (MASK == 8) return __builtin_amdgcn_update_dpp(0, x, 0x128, 0xF, 0xF, false);
+    else if constexpr (MASK == 4 || MASK == 16) return __builtin_amdgcn_ds_swizzle(x, 0x1F | (MASK << 10));
+    else return __builtin_amdgcn_ds_bpermute((int)((__lane_id() ^ 32u) << 2), x);
+}
+
+template <int MASK>
+__device__ __forceinline__ double lane_xor_d(double v) {
+    const int hi = lane_xor_i<MASK>(__double2hiint(v));
+    const int lo = lane_xor_i<MASK>(__double2loint(v));
+    return __hiloint2double(hi, lo);
+}
+
+// Radix-2 DIF level of half-size H across lanes.  The lane with bit H clear
+// holds a and ends with a+b; its partner holds b and ends with (a-b)*w.
+// s = +1 / -1 on the low / high lane; the low lane's w is omega^0.
+template <int H>
+__device__ __forceinline__ double2 xlevel(double2 v, double s, double2 w) {
+    const double2 p = make_double2(lane_xor_d<H>(v.x), lane_xor_d<H>(v.y));
+    return cmul(make_double2(fma(s, v.x, p.x), fma(s, v.y, p.y)), w);
+}
+
+// Bit reversal of m over log2(P) bits.
+template <int P>
+__device__ __forceinline__ constexpr int brev_slot(int m) {
+    return P == 4 ? (((m & 1) << 1) | (m >> 1)) : m;
+}
+
+// W = 64P point DIF of v (element 64m + lane in v[m]).  omega_W^u is read
+// from the LDS table twL of omega_L (t < L = L1*W) at u*L1.  Afterwards v[m]
+// holds X[brev6(lane)*P + brev_slot(m)].
+template <int P>
+__device__ __forceinline__ void wave_dif(double2 (&v)[P], const double2* twL, int L1, int lane) {
+    // cross-lane twiddles: omega_2h^(lane mod h) = omega_W^((lane mod h)*32P/h) on the high lane
+    double2 w[6];
+    double sg[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        const int h = 32 >> i;
+        const bool hi = (lane & h) != 0;
+        w[i] = twL[hi ? (lane & (h - 1)) * (32 * P / h) * L1 : 0];
+        sg[i] = hi ? -1.0 : 1.0;
+    }
+    // in-register levels, half-size h = 64*hs: omega_2h^e = omega_W^(e*P/(2hs)), e = (64m + lane) mod h
+#pragma unroll
+    for (int hs = P / 2; hs >= 1; hs >>= 1) {
+#pragma unroll
+        for (int m = 0; m < P; ++m) {
+            if (m & hs) continue;
+            const double2 a = v[m], b = v[m + hs];
+            const int e = 64 * (m & (hs - 1)) + lane;
+            v[m] = cadd(a, b);
+            v[m + hs] = cmul(csub(a, b), twL[e * (P / (2 * hs)) * L1]);
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < P; ++m) v[m] = xlevel<32>(v[m], sg[0], w[0]);
+#pragma unroll
+    for (int m = 0; m < P; ++m) v[m] = xlevel<16>(v[m], sg[1], w[1]);
+#pragma unroll
+    for (int m = 0; m < P; ++m) v[m] = xlevel<8>(v[m], sg[2], w[2]);
+#pragma unroll
+    for (int m = 0; m < P; ++m) v[m] = xlevel<4>(v[m], sg[3], w[3]);
+#pragma unroll
+    for (int m = 0; m < P; ++m) v[m] = xlevel<2>(v[m], sg[4], w[4]);
+#pragma unroll
+    for (int m = 0; m < P; ++m) v[m] = xlevel<1>(v[m], sg[5], w[5]);
+}
+
+// Last factor of a length L = L1*W forward transform of the sequences of g,
+// after the length-L1 Stockham passes over the W interleaved sub-sequences
+// (n2 = element mod W).  Four-step: sub-transform (q, k1) takes elements
+// W*k1 + n2 of sequence q, n2 = 64m + lane, multiplies them by
+// omega_L^(n2*k1), runs the W-point DIF in one wave and writes X[k1 + L1*k2]
+// in natural order.  Wave w takes sub-transform w (the planner keeps their
+// number within the workgroup's waves); all reads precede all writes.
+template <int T, int P>
+__device__ __forceinline__ void fft_wave(double2* lds, const SeqGeom& g, int L1, unsigned int mL1,
+                                         const double2* twL, int tid) {
+    constexpr int W = 64 * P;
+    const int lane = tid & 63, sub = tid >> 6;
+    const bool active = sub < g.nsq * L1;  // wave-uniform
+    double2 v[P];
+    int ob = 0;
+    if (active) {
+        const int q = udiv(sub, L1, mL1);
+        const int k1 = sub - q * L1;
+        const int base = seq_base(g, q);
+#pragma unroll
+        for (int m = 0; m < P; ++m) {
+            const int n2 = 64 * m + lane;
+            const double2 x = lds[pidx(base + (W * k1 + n2) * g.sIdx)];
+            v[m] = L1 > 1 ? cmul(x, twL[n2 * k1]) : x;
+        }
+        wave_dif<P>(v, twL, L1, lane);
+        ob = base + k1 * g.sIdx;
+    }
+    __syncthreads();
+    if (active) {
+        const int kb = (int)(__brev((unsigned int)lane) >> 26) * P;
+#pragma unroll
+        for (int m = 0; m < P; ++m) lds[pidx(ob + L1 * (kb + brev_slot<P>(m)) * g.sIdx)] = v[m];
+    }
+    __syncthreads();
+}
+
+// Forward transform of length L of the sequences of g: the length-L1
+// Stockham passes (stages, twiddles tws) over the sub-sequences of gA, then,
+// when P > 0, the W = 64P wave-register factor.  P = 0: Stockham only over g.
+template <int T, int P>
+__device__ __forceinline__ void fft_seq(double2* lds, const LucasStage* stages, int nst, const double2* tws,
+                                        const SeqGeom& g, const SeqGeom& gA, int L1, unsigned int mL1,
+                                        const double2* twL, int tid) {
+    if constexpr (P > 0) {
+        fft_lds<T, -1, true>(lds, stages, nst, gA, tws, tid);
+        fft_wave<T, P>(lds, g, L1, mL1, twL, tid);
+    } else {
+        fft_lds<T, -1, true>(lds, stages, nst, g, tws, tid);
     }
 }
 
@@ -385,7 +566,7 @@ __device__ __forceinline__ int2 carry_element(const double2* lds, const LucasJob
 // data (stage twiddles, weight and r tables of the tile's rows and digits) is
 // copied into LDS while the tile's own loads are in flight.
 // ---------------------------------------------------------------------------
-template <int T>
+template <int T, int PA>
 __device__ __forceinline__ void col_kernel_body(const LucasJobDesc* __restrict__ jobs, const int2* __restrict__ wgmap,
                                                 int mode) {
     double2* const lds = lucas_dynamic_lds();
@@ -410,6 +591,10 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc* __restrict__
     int* const colr = reinterpret_cast<int*>(p);  // rlo per local digit
     p += (2 * ncl + 3) / 4;
     float* const red = reinterpret_cast<float*>(p);  // per-wave roundoff slots
+    p += 4;
+    LucasStage* const lst = reinterpret_cast<LucasStage*>(p);  // column transform stages
+    p += 2 * J.nradA;
+    double2* const twL = p;  // omega_A^t, t < A (wave factor only)
     const unsigned int seen = *reinterpret_cast<volatile unsigned int*>(J.maxerr);
     const double2* const twA = J.twA;
     const double2* const twMB = J.twMB;
@@ -427,6 +612,11 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc* __restrict__
         }
     }
     for (int t = tid; t < J.ntwsA; t += T) tws[t] = gload2(J.twsA, t);
+    copy_stages<T>(lst, J.stA, J.nradA, tid);
+    if (PA)
+        for (int t = tid; t < A; t += T) twL[t] = gload2(twA, t);
+    constexpr int wA = 64 * PA;
+    const int A1 = PA ? A / wA : A;
     for (int t = tid; t < A; t += T) {
         roww[t] = make_double2(gload(J.whi, t), gload(J.iwhi, t));
         rowr[t] = gload(J.rhi, t);
@@ -458,7 +648,9 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc* __restrict__
         }
     } else {
         // inverse column FFT of the tile and halo
-        fft_lds<T, -1, true>(lds, J.stA, J.nradA, ncl, mncl, ncl, 1, 0, tws, tid);
+        const SeqGeom g{ncl, mncl, 0, 0u, 1, 0, 0, ncl};
+        const SeqGeom gA{ncl * wA, J.m_sqA0, 0, 0u, 1, 0, 0, wA * ncl};
+        fft_seq<T, PA>(lds, lst, J.nradA, tws, g, gA, A1, J.m_A1, twL, tid);
         // unweight, -2 on digit 0, round half-even, roundoff
         for (int t = tid; t < A * ncl; t += T) {
             const int r = udiv(t, ncl, mncl), lc = t - r * ncl;
@@ -515,7 +707,11 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc* __restrict__
     }
     __syncthreads();
     // forward column FFT of the own columns
-    fft_lds<T, -1, true>(lds, J.stA, J.nradA, C, mC, ncl, 1, H, tws, tid);
+    {
+        const SeqGeom g{C, mC, 0, 0u, 1, 0, H, ncl};
+        const SeqGeom gA{C * wA, J.m_sqA1, C, mC, 1, ncl, H, wA * ncl};
+        fft_seq<T, PA>(lds, lst, J.nradA, tws, g, gA, A1, J.m_A1, twL, tid);
+    }
     // store row-major; the outer twiddle omega_M^(c*k) is applied by the row
     // kernel as it loads
     for (int t = tid; t < A * C; t += T) {
@@ -535,7 +731,7 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc* __restrict__
 // which is 4x the packed spectrum of the squared real signal; the 1/4 is
 // folded into the unweight table.
 // ---------------------------------------------------------------------------
-template <int T>
+template <int T, int PB>
 __device__ __forceinline__ void row_kernel_body(const LucasJobDesc* __restrict__ jobs,
                                                 const int2* __restrict__ wgmap) {
     double2* const lds = lucas_dynamic_lds();
@@ -550,6 +746,7 @@ __device__ __forceinline__ void row_kernel_body(const LucasJobDesc* __restrict__
     // LDS carve-up, in the order of lucas_row_lds_words()
     double2* const tws = lds + pidx(2 * B) + 1;  // stage twiddles of the length-B transform
     double2* const twu = tws + J.ntwsB;          // omega_B^t, t < B (untangle)
+    LucasStage* const lst = reinterpret_cast<LucasStage*>(twu + B);  // row transform stages
 
     // load both rows with the column kernel's outer twiddle omega_M^(c*k);
     // the tables go to LDS while those loads are in flight
@@ -560,12 +757,17 @@ __device__ __forceinline__ void row_kernel_body(const LucasJobDesc* __restrict__
     }
     for (int t = tid; t < J.ntwsB; t += T) tws[t] = gload2(J.twsB, t);
     for (int t = tid; t < B; t += T) twu[t] = gload2(J.twB, t);
+    copy_stages<T>(lst, J.stB, J.nradB, tid);
     const double2 wr = gload2(J.twMA, r0);
-    // pass 0: forward row FFT; pass 1: inverse row FFT (conjugated input)
-    for (int pass = 0;; ++pass) {
-        __syncthreads();
-        fft_lds<T, -1, false>(lds, J.stB, J.nradB, nrows, 0u, 1, B, 0, tws, tid);
-        if (pass == 1) break;
+    constexpr int wB = 64 * PB;
+    const int B1 = PB ? B / wB : B;
+    const SeqGeom g{nrows, pow2_recip(nrows), 0, 0u, B, 0, 0, 1};
+    const SeqGeom gA{nrows * wB, pow2_recip(nrows * wB), wB, pow2_recip(wB), 1, B, 0, wB};
+    // forward row FFT; square; inverse row FFT (conjugated input).  Two call
+    // sites: a loop over the two transforms costs registers across the loop.
+    __syncthreads();
+    fft_seq<T, PB>(lds, lst, J.nradB, tws, g, gA, B1, J.m_B1, twu, tid);
+    {
         // pairs (kc in row r0) <-> (pc in row r1): rows differ, or row r0 pairs
         // with itself (row 0: pc = -kc mod B; row A/2: pc = B-1-kc)
         const int last = !self ? B - 1 : (r0 == 0 ? B / 2 : (B - 1) / 2);
@@ -584,6 +786,8 @@ __device__ __forceinline__ void row_kernel_body(const LucasJobDesc* __restrict__
             if (poff + pc != kc) lds[pidx(poff + pc)] = make_double2(P.x + 2.0 * Q.y, P.y - 2.0 * Q.x);
         }
     }
+    __syncthreads();
+    fft_seq<T, PB>(lds, lst, J.nradB, tws, g, gA, B1, J.m_B1, twu, tid);
     for (int t = tid; t < nrows * B; t += T) {
         const int rl = t >= B ? 1 : 0, kc = t - rl * B;
         gstore2(J.spec2, (size_t)(rl ? r1 : r0) * B + kc, cconj(lds[pidx(t)]));
@@ -592,67 +796,75 @@ __device__ __forceinline__ void row_kernel_body(const LucasJobDesc* __restrict__
 
 }  // namespace
 
-extern "C" __global__ void __launch_bounds__(256)
-lucas_col_kernel_256(const LucasJobDesc* __restrict__ jobs, const int2* __restrict__ wgmap, int mode) {
-    col_kernel_body<256>(jobs, wgmap, mode);
-}
-extern "C" __global__ void __launch_bounds__(512)
-lucas_col_kernel_512(const LucasJobDesc* __restrict__ jobs, const int2* __restrict__ wgmap, int mode) {
-    col_kernel_body<512>(jobs, wgmap, mode);
-}
-extern "C" __global__ void __launch_bounds__(1024)
-lucas_col_kernel_1024(const LucasJobDesc* __restrict__ jobs, const int2* __restrict__ wgmap, int mode) {
-    col_kernel_body<1024>(jobs, wgmap, mode);
-}
-extern "C" __global__ void __launch_bounds__(256)
-lucas_row_kernel_256(const LucasJobDesc* __restrict__ jobs, const int2* __restrict__ wgmap) {
-    row_kernel_body<256>(jobs, wgmap);
-}
-extern "C" __global__ void __launch_bounds__(512)
-lucas_row_kernel_512(const LucasJobDesc* __restrict__ jobs, const int2* __restrict__ wgmap) {
-    row_kernel_body<512>(jobs, wgmap);
-}
-extern "C" __global__ void __launch_bounds__(1024)
-lucas_row_kernel_1024(const LucasJobDesc* __restrict__ jobs, const int2* __restrict__ wgmap) {
-    row_kernel_body<1024>(jobs, wgmap);
-}
+// One kernel per (workgroup size, wave factor P of its transform): each
+// holds a single wave-factor variant, which keeps the code of a column/row
+// kernel pair within the instruction cache.
+#define LUCAS_KERNEL_PAIR(T, P)                                                                            \
+    extern "C" __global__ void __launch_bounds__(T)                                                      \
+            lucas_col_kernel_##T##_p##P(const LucasJobDesc* __restrict__ jobs,                            \
+                                        const int2* __restrict__ wgmap, int mode) {                       \
+        col_kernel_body<T, P>(jobs, wgmap, mode);                                                        \
+    }                                                                                                    \
+    extern "C" __global__ void __launch_bounds__(T)                                                      \
+            lucas_row_kernel_##T##_p##P(const LucasJobDesc* __restrict__ jobs,                            \
+                                        const int2* __restrict__ wgmap) {                                 \
+        row_kernel_body<T, P>(jobs, wgmap);                                                              \
+    }
+LUCAS_KERNEL_PAIR(256, 0)
+LUCAS_KERNEL_PAIR(256, 1)
+LUCAS_KERNEL_PAIR(256, 2)
+LUCAS_KERNEL_PAIR(256, 4)
+LUCAS_KERNEL_PAIR(512, 0)
+LUCAS_KERNEL_PAIR(512, 1)
+LUCAS_KERNEL_PAIR(512, 2)
+LUCAS_KERNEL_PAIR(512, 4)
+LUCAS_KERNEL_PAIR(1024, 0)
+LUCAS_KERNEL_PAIR(1024, 1)
+LUCAS_KERNEL_PAIR(1024, 2)
+LUCAS_KERNEL_PAIR(1024, 4)
 
-// ---------------------------------------------------------------------------
-// Host-side launchers (the engine's only entry into device code).
-// ---------------------------------------------------------------------------
 namespace lucas {
 
-// Workgroups may use the whole 160 KB LDS of a gfx950 CU (64 KB without this).
+namespace {
+typedef void (*ColKernel)(const LucasJobDesc*, const int2*, int);
+typedef void (*RowKernel)(const LucasJobDesc*, const int2*);
+// [workgroup size 256/512/1024][P 0/1/2/4]
+const ColKernel kColKernels[3][4] = {
+        {lucas_col_kernel_256_p0, lucas_col_kernel_256_p1, lucas_col_kernel_256_p2, lucas_col_kernel_256_p4},
+        {lucas_col_kernel_512_p0, lucas_col_kernel_512_p1, lucas_col_kernel_512_p2, lucas_col_kernel_512_p4},
+        {lucas_col_kernel_1024_p0, lucas_col_kernel_1024_p1, lucas_col_kernel_1024_p2, lucas_col_kernel_1024_p4}};
+const RowKernel kRowKernels[3][4] = {
+        {lucas_row_kernel_256_p0, lucas_row_kernel_256_p1, lucas_row_kernel_256_p2, lucas_row_kernel_256_p4},
+        {lucas_row_kernel_512_p0, lucas_row_kernel_512_p1, lucas_row_kernel_512_p2, lucas_row_kernel_512_p4},
+        {lucas_row_kernel_1024_p0, lucas_row_kernel_1024_p1, lucas_row_kernel_1024_p2, lucas_row_kernel_1024_p4}};
+int threads_index(int threads) { return threads == 1024 ? 2 : (threads == 512 ? 1 : 0); }
+int p_index(int p) { return p == 4 ? 3 : p; }
+}  // namespace
+
 hipError_t configure_kernels(size_t max_lds) {
-    const void* fns[] = {(const void*)lucas_col_kernel_256,  (const void*)lucas_col_kernel_512,
-                         (const void*)lucas_col_kernel_1024, (const void*)lucas_row_kernel_256,
-                         (const void*)lucas_row_kernel_512,  (const void*)lucas_row_kernel_1024};
-    for (const void* f : fns) {
-        hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
-        if (e != hipSuccess) return e;
-    }
+    for (int t = 0; t < 3; ++t)
+        for (int q = 0; q < 4; ++q) {
+            hipError_t e = hipFuncSetAttribute((const void*)kColKernels[t][q],
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
+            if (e == hipSuccess)
+                e = hipFuncSetAttribute((const void*)kRowKernels[t][q], hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)max_lds);
+            if (e != hipSuccess) return e;
+        }
     return hipSuccess;
 }
 
-hipError_t launch_col_kernel(int threads, const LucasJobDesc* jobs, const int2* map, int nwg, size_t lds, int mode,
-                             hipStream_t stream) {
-    if (threads == 1024)
-        hipLaunchKernelGGL(lucas_col_kernel_1024, dim3(nwg), dim3(1024), lds, stream, jobs, map, mode);
-    else if (threads == 512)
-        hipLaunchKernelGGL(lucas_col_kernel_512, dim3(nwg), dim3(512), lds, stream, jobs, map, mode);
-    else
-        hipLaunchKernelGGL(lucas_col_kernel_256, dim3(nwg), dim3(256), lds, stream, jobs, map, mode);
+hipError_t launch_col_kernel(int threads, int p, const LucasJobDesc* jobs, const int2* map, int nwg, size_t lds,
+                             int mode, hipStream_t stream) {
+    hipLaunchKernelGGL(kColKernels[threads_index(threads)][p_index(p)], dim3(nwg), dim3(threads), lds, stream, jobs,
+                       map, mode);
     return hipGetLastError();
 }
 
-hipError_t launch_row_kernel(int threads, const LucasJobDesc* jobs, const int2* map, int nwg, size_t lds,
+hipError_t launch_row_kernel(int threads, int p, const LucasJobDesc* jobs, const int2* map, int nwg, size_t lds,
                              hipStream_t stream) {
-    if (threads == 1024)
-        hipLaunchKernelGGL(lucas_row_kernel_1024, dim3(nwg), dim3(1024), lds, stream, jobs, map);
-    else if (threads == 512)
-        hipLaunchKernelGGL(lucas_row_kernel_512, dim3(nwg), dim3(512), lds, stream, jobs, map);
-    else
-        hipLaunchKernelGGL(lucas_row_kernel_256, dim3(nwg), dim3(256), lds, stream, jobs, map);
+    hipLaunchKernelGGL(kRowKernels[threads_index(threads)][p_index(p)], dim3(nwg), dim3(threads), lds, stream, jobs,
+                       map);
     return hipGetLastError();
 }
 

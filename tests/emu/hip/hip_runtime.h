@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -22,6 +23,9 @@
 
 struct double2 { double x, y; };
 struct int2 { int x, y; };
+struct alignas(16) int4 { int x, y, z, w; };
+// one work-item per host thread: every value is its own first lane
+inline int __builtin_amdgcn_readfirstlane(int x) { return x; }
 inline double2 make_double2(double x, double y) { return {x, y}; }
 inline int2 make_int2(int x, int y) { return {x, y}; }
 struct dim3 {
@@ -43,14 +47,56 @@ namespace emu {
 extern std::barrier<>* bar;
 extern std::vector<float> shfl;
 extern std::mutex atom_mu;
+// cross-lane exchange within one 64-wide wave: one barrier per wave
+extern std::vector<std::unique_ptr<std::barrier<>>> wave_bar;
+extern std::vector<int> lane_buf;
 void run_grid(unsigned nblocks, unsigned nthreads, const std::function<void()>& body);
+// value of `x` held by lane `src` of the calling work-item's wave
+inline int wave_read(int x, int src) {
+    const unsigned w = threadIdx.x >> 6;
+    lane_buf[threadIdx.x] = x;
+    wave_bar[w]->arrive_and_wait();
+    const int r = lane_buf[(w << 6) + (src & 63)];
+    wave_bar[w]->arrive_and_wait();
+    return r;
+}
 }
 inline void __syncthreads() { emu::bar->arrive_and_wait(); }
+inline unsigned __lane_id() { return threadIdx.x & 63; }
+// DPP: quad_perm (ctrl < 0x100) and row_ror:n (0x121..0x12F) are the forms used
+inline int __builtin_amdgcn_update_dpp(int /*old*/, int src, int ctrl, int, int, bool) {
+    const int l = threadIdx.x & 63;
+    int s = l;
+    if (ctrl < 0x100) s = (l & ~3) | ((ctrl >> (2 * (l & 3))) & 3);
+    else if (ctrl >= 0x121 && ctrl <= 0x12F) s = (l & ~15) | ((l + (ctrl - 0x120)) & 15);
+    return emu::wave_read(src, s);
+}
+// ds_swizzle, bit-mask mode (offset bit 15 clear) within 32-lane halves
+inline int __builtin_amdgcn_ds_swizzle(int src, int pattern) {
+    const int l = threadIdx.x & 63;
+    const int am = pattern & 0x1F, om = (pattern >> 5) & 0x1F, xm = (pattern >> 10) & 0x1F;
+    return emu::wave_read(src, (l & 32) | ((((l & 31) & am) | om) ^ xm));
+}
+inline int __builtin_amdgcn_ds_bpermute(int addr, int src) { return emu::wave_read(src, (addr >> 2) & 63); }
+inline int __double2loint(double d) { unsigned long long u; std::memcpy(&u, &d, 8); return (int)(u & 0xffffffffu); }
+inline int __double2hiint(double d) { unsigned long long u; std::memcpy(&u, &d, 8); return (int)(u >> 32); }
+inline double __hiloint2double(int hi, int lo) {
+    const unsigned long long u = ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+    double d;
+    std::memcpy(&d, &u, 8);
+    return d;
+}
 inline float __shfl_xor(float v, int mask, int /*width*/) {
     emu::shfl[threadIdx.x] = v;
     __syncthreads();
     float r = emu::shfl[threadIdx.x ^ mask];
     __syncthreads();
+    return r;
+}
+inline int __clz(int x) { return x == 0 ? 32 : __builtin_clz((unsigned)x); }
+inline unsigned __brev(unsigned x) {
+    unsigned r = 0;
+    for (int i = 0; i < 32; ++i) r |= ((x >> i) & 1u) << (31 - i);
     return r;
 }
 inline unsigned __umulhi(unsigned a, unsigned b) { return (unsigned)(((unsigned long long)a * b) >> 32); }

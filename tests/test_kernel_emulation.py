@@ -15,6 +15,10 @@ import oracle
     (9941, 560, 7, 4, 2, 0),      # radix 7 columns
     (2203, 120, 0, 0, 2, 0),      # planner's own choice
     (44497, 2560, 40, 16, 2, 256),  # A*C > T: the carry runs in three passes
+    (294919, 16384, 64, 4, 1, 512),    # wave factors only: columns 64 (P=1), rows 128 (P=2)
+    (218737, 12288, 192, 2, 1, 1024),  # columns 3 x 64: Stockham radix 3 then wave factor
+    (294919, 16384, 512, 2, 1, 512),   # columns 2 x 256 (P=4)
+    (368647, 20480, 16, 8, 1, 1024),   # rows 5 x 128: radix 5 then wave factor
 ])
 def test_emulated_kernels_match_oracle(p, N, A, C, iters, T):
     x = oracle.s0_digits(N)
