@@ -85,12 +85,14 @@ struct LucasJobDesc {
 
 // Dynamic LDS of the two kernels for one job, in 16-byte words.  Each kernel
 // carves its LDS in the order listed.
-//   column kernel: padded (C+H) x A tile; stage twiddles; per-row
+//   column kernel: (C+H) x A tile; stage twiddles; per-row
 //     {2^(rhi/N), 2^(-rhi/N)/(4M)}; per local digit {2^(rlo/N), 2^(-rlo/N)};
 //     per-row rhi; per local digit rlo; roundoff slots; stage table;
 //     omega_A^t, t < A (with a wave factor only)
-//   row kernel: two padded rows; stage twiddles; omega_B^t, t < B; stage table
-__host__ __device__ inline int lucas_pad_words(int n) { return n + (n >> 3) + 1; }
+//   row kernel: two rows; stage twiddles; omega_B^t, t < B; stage table
+// A tile of n words occupies whole 16-word blocks (its word index is permuted
+// within each block).
+__host__ __device__ inline int lucas_pad_words(int n) { return (n + 15) & ~15; }
 __host__ __device__ inline int lucas_col_lds_words(int A, int C, int H, int ntws, int nst, int pA) {
     const int ncl = C + H;
     return lucas_pad_words(A * ncl) + ntws + A + 2 * ncl + (A + 3) / 4 + (2 * ncl + 3) / 4 + 4 + 2 * nst +

@@ -61,10 +61,12 @@ __device__ __forceinline__ double2* lucas_dynamic_lds() {
 
 namespace {
 
-// LDS word index with one pad word after every 8 complex words.  Stockham
-// writes land at stride R (and R*Ns) words; unpadded, eight lanes of a
-// wave hit the same banks.
-__device__ __forceinline__ int pidx(int i) { return i + (i >> 3); }
+// LDS word of tile element i (16-byte words).  The low four bits (the bank
+// group within a 256-byte row of banks) are XORed with bits 4-7 and 6-9, a
+// permutation within each aligned 16-word block: power-of-two strides
+// (Stockham writes at stride R*Ns, wave-factor outputs at stride L1*P)
+// spread over all bank groups instead of piling onto one.
+__device__ __forceinline__ int pidx(int i) { return i ^ ((i >> 4) & 15) ^ ((i >> 6) & 15); }
 
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
@@ -579,7 +581,7 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc* __restrict__
     const int ncl = C + H;  // local columns: [0,H) halo, [H, H+C) own
     const unsigned int mncl = J.m_ncl, mC = J.m_C;
     // LDS carve-up, in the order of lucas_col_lds_words()
-    double2* p = lds + pidx(A * ncl) + 1;
+    double2* p = lds + lucas_pad_words(A * ncl);
     double2* const tws = p;  // stage twiddles of the length-A transform
     p += J.ntwsA;
     double2* const roww = p;  // {2^(rhi/N), 2^(-rhi/N)/(4M)} per row
@@ -744,7 +746,7 @@ __device__ __forceinline__ void row_kernel_body(const LucasJobDesc* __restrict__
     const int r1 = self ? r0 : A - r0;
     const int nrows = self ? 1 : 2;
     // LDS carve-up, in the order of lucas_row_lds_words()
-    double2* const tws = lds + pidx(2 * B) + 1;  // stage twiddles of the length-B transform
+    double2* const tws = lds + lucas_pad_words(2 * B);  // stage twiddles of the length-B transform
     double2* const twu = tws + J.ntwsB;          // omega_B^t, t < B (untangle)
     LucasStage* const lst = reinterpret_cast<LucasStage*>(twu + B);  // row transform stages
 
