@@ -46,9 +46,9 @@ def headline(lucas, steps, warmup):
     return ms, err, col_us, row_us, info
 
 
-def batched_ratio(lucas, wl, iters, count):
+def batched_ratio(lucas, wl, iters, count, seed=4, lo=1_000_000, hi=5_000_000):
     """Aggregate iterations/s: all exponents in one batch vs one at a time."""
-    ps = wl.batch_primes(count=count)
+    ps = wl.batch_primes(seed=seed, count=count, lo=lo, hi=hi)
     b = lucas.Batch(ps)
     b.step(2)
     ms_batch = b.time_iterations(iters)
@@ -216,6 +216,10 @@ def main():
         out["batched"] = {"workload": "64 primes in [1e6,5e6] (seed 4), default N each",
                           "iters_per_s": round(bi, 1), "plain_iters_per_s": round(pi, 1),
                           "ratio": round(bi / pi, 3)}
+        bi, pi = batched_ratio(lucas, wl, a.batch_iters, 64, seed=6, lo=100_000, hi=1_200_000)
+        out["batched_small"] = {"workload": "64 primes in [1e5,1.2e6] (seed 6), default N each (N <= 64K)",
+                                "iters_per_s": round(bi, 1), "plain_iters_per_s": round(pi, 1),
+                                "ratio": round(bi / pi, 3)}
     if not a.no_serve:
         out["serving"] = serving(lucas, wl, a.serve_jobs, 0.1, 0.8)
     if not a.no_cpu:

@@ -55,6 +55,8 @@ void fill_info(const Plan& pl, lucas_plan_info* out) {
     out->row_wgs = pl.row_wgs;
     out->threads = pl.threads;
     out->radix_pref = pl.radix_pref;
+    out->wave_A = pl.wA;
+    out->wave_B = pl.wB;
     out->device_bytes = job_bytes(pl);
 }
 

@@ -17,6 +17,8 @@
 
 namespace lucas {
 
+static const int kXcds = 8;  // accelerator complex dies of an MI355X
+
 hipError_t launch_col_kernel(int threads, int p, const LucasJobDesc* jobs, const int2* map, int nwg, size_t lds,
                              int mode, hipStream_t stream);
 hipError_t launch_row_kernel(int threads, int p, const LucasJobDesc* jobs, const int2* map, int nwg, size_t lds,
@@ -220,6 +222,22 @@ int Engine::prepare() {
         for (int w = 0; w < j->plan.row_wgs; ++w) rmap[gr].push_back(make_int2((int)i, w));
         colg_[gc].lds = std::max(colg_[gc].lds, j->plan.lds_col);
         rowg_[gr].lds = std::max(rowg_[gr].lds, j->plan.lds_row);
+    }
+    // Workgroups are dealt to the 8 XCDs round-robin by workgroup id, and each
+    // XCD has its own L2.  Neighbouring column tiles share halo columns and
+    // cache lines, so each XCD gets a contiguous range of tiles: workgroup
+    // 8k + x runs tile k of XCD x's range.
+    for (std::vector<int2>& m : cmap) {
+        const size_t n = m.size();
+        if (n < 16) continue;
+        std::vector<int2> out;
+        out.reserve(n);
+        size_t start[kXcds + 1];
+        for (int x = 0; x <= kXcds; ++x) start[x] = n * x / kXcds;
+        for (size_t k = 0; out.size() < n; ++k)
+            for (int x = 0; x < kXcds; ++x)
+                if (start[x] + k < start[x + 1]) out.push_back(m[start[x] + k]);
+        m.swap(out);
     }
     auto grow = [&](void** ptr, size_t* cap, size_t need) -> bool {
         if (need <= *cap) return true;

@@ -43,9 +43,11 @@ def measure(pairs, iters, max_candidates, out):
         info, ms = lucas.autotune_shape(p, N, iters=iters, max_candidates=max_candidates)
         base = lucas.plan(p, N)
         results[str(N)] = {"p": p, "N": N, "A": info["A"], "C": info["C"], "threads": info["threads"],
-                           "radix_pref": info["radix_pref"], "ms_per_iter": ms,
+                           "radix_pref": info["radix_pref"], "wave_A": info["wave_A"], "wave_B": info["wave_B"],
+                           "ms_per_iter": ms,
                            "model_A": base["A"], "model_C": base["C"]}
         print(f"N={N} p={p}: A={info['A']} C={info['C']} T={info['threads']} R{info['radix_pref']} "
+              f"W{info['wave_A']}/{info['wave_B']} "
               f"{ms * 1e3:.2f} us/iter "
               f"({time.time() - t0:.1f}s)", flush=True)
         with open(out, "w") as f:

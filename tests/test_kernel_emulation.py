@@ -37,3 +37,16 @@ def test_emulated_kernels_steady_state():
     got, _ = emu.run(p, N, x, 2, 15, 8)
     want, _ = oracle.ibdwt_iterate(p, N, x, 2)
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("p,N,A,C,T,wA,wB", [
+    (294919, 16384, 64, 4, 512, 64, 128),
+    (218737, 12288, 192, 2, 1024, 64, 0),
+    (294919, 16384, 512, 2, 512, 256, 0),
+    (368647, 20480, 16, 8, 1024, 0, 128),
+])
+def test_emulated_shapes_take_the_wave_path(p, N, A, C, T, wA, wB):
+    """The wave-factor cases above really run the wave-register transforms."""
+    from gpulucas_amd import lucas
+    pl = [x for x in lucas.plan_list(p, N) if x["A"] == A and x["C"] == C and x["threads"] == T][0]
+    assert (pl["wave_A"], pl["wave_B"]) == (wA, wB)
