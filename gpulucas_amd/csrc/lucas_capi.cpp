@@ -57,6 +57,7 @@ void fill_info(const Plan& pl, lucas_plan_info* out) {
     out->radix_pref = pl.radix_pref;
     out->wave_A = pl.wA;
     out->wave_B = pl.wB;
+    out->col_dense = pl.col_dense;
     out->device_bytes = job_bytes(pl);
 }
 

@@ -19,8 +19,8 @@ namespace lucas {
 
 static const int kXcds = 8;  // accelerator complex dies of an MI355X
 
-hipError_t launch_col_kernel(int threads, int p, const LucasJobDesc* jobs, const int2* map, int nwg, size_t lds,
-                             int mode, hipStream_t stream);
+hipError_t launch_col_kernel(int threads, int p, int dense, const LucasJobDesc* jobs, const int2* map, int nwg,
+                             size_t lds, int mode, hipStream_t stream);
 hipError_t launch_row_kernel(int threads, int p, const LucasJobDesc* jobs, const int2* map, int nwg, size_t lds,
                              hipStream_t stream);
 hipError_t configure_kernels(size_t max_lds);
@@ -203,21 +203,24 @@ int Engine::prepare() {
     }
     std::vector<LucasJobDesc> desc;
     std::vector<int2> cmap[kGroups], rmap[kGroups];
-    static const int kThreads[3] = {LUCAS_THREADS_SMALL, LUCAS_THREADS_LARGE, LUCAS_THREADS_MAX};
+    static const int kThreads[4] = {LUCAS_THREADS_SMALL, LUCAS_THREADS_LARGE, LUCAS_THREADS_MAX, LUCAS_THREADS_MAX};
     static const int kP[4] = {0, 1, 2, 4};
     for (int i = 0; i < kGroups; ++i) {
         colg_[i].threads = rowg_[i].threads = kThreads[i / 4];
         colg_[i].p = rowg_[i].p = kP[i % 4];
+        colg_[i].dense = i / 4 == 3;
+        rowg_[i].dense = 0;
         colg_[i].lds = rowg_[i].lds = 0;
     }
-    auto group_of = [](int threads, int p) {
-        const int ti = threads == LUCAS_THREADS_MAX ? 2 : (threads == LUCAS_THREADS_LARGE ? 1 : 0);
+    auto group_of = [](int threads, int p, int dense) {
+        const int ti = threads == LUCAS_THREADS_MAX ? (dense ? 3 : 2) : (threads == LUCAS_THREADS_LARGE ? 1 : 0);
         return 4 * ti + (p == 4 ? 3 : p);
     };
     for (size_t i = 0; i < jobs_.size(); ++i) {
         const Job* j = jobs_[i];
         desc.push_back(j->desc);
-        const int gc = group_of(j->plan.threads, j->plan.wA / 64), gr = group_of(j->plan.threads, j->plan.wB / 64);
+        const int gc = group_of(j->plan.threads, j->plan.wA / 64, j->plan.col_dense);
+        const int gr = group_of(j->plan.threads, j->plan.wB / 64, 0);
         for (int w = 0; w < j->plan.col_wgs; ++w) cmap[gc].push_back(make_int2((int)i, w));
         for (int w = 0; w < j->plan.row_wgs; ++w) rmap[gr].push_back(make_int2((int)i, w));
         colg_[gc].lds = std::max(colg_[gc].lds, j->plan.lds_col);
@@ -271,7 +274,7 @@ int Engine::prepare() {
 int Engine::launch_col(int mode) {
     for (const Group& g : colg_) {
         if (!g.n) continue;
-        if (launch_col_kernel(g.threads, g.p, d_desc_, g.d_map, g.n, g.lds, mode, stream_) != hipSuccess)
+        if (launch_col_kernel(g.threads, g.p, g.dense, d_desc_, g.d_map, g.n, g.lds, mode, stream_) != hipSuccess)
             return LUCAS_ERR_DEVICE;
     }
     return LUCAS_OK;

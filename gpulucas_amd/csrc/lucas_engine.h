@@ -57,9 +57,10 @@ private:
 
     // Jobs whose kernel has the same workgroup size and wave factor share
     // one launch; column and row launches are grouped separately.
-    static const int kGroups = 12;  // 3 workgroup sizes x 4 wave factors (P = 0, 1, 2, 4)
+    // 3 workgroup sizes (and the dense 1024 column variant) x 4 wave factors (P = 0, 1, 2, 4)
+    static const int kGroups = 16;
     struct Group {
-        int threads = 0, p = 0;
+        int threads = 0, p = 0, dense = 0;
         int2* d_map = nullptr;
         size_t cap = 0;
         int n = 0;

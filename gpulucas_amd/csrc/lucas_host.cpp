@@ -26,6 +26,7 @@ static const long double kPi = 3.141592653589793238462643383279502884L;
 // LDS one workgroup may allocate (column tile + digit buffer, or two rows):
 // all of a gfx950 CU's 160 KB.
 static const size_t kMaxLds = kMaxLdsBytes;
+static const int kCUs = 256;  // compute units of an MI355X
 
 bool valid_length(long long N) {
     if (N < 4 || (N & 1)) return false;
@@ -176,6 +177,8 @@ std::vector<Plan> plan_candidates(int p, int N) {
                     c.lds_col = lds_col;
                     c.lds_row = lds_row;
                     c.radix_pref = pref;
+                    // more column workgroups than CUs: two per CU if their LDS fits
+                    c.col_dense = T == LUCAS_THREADS_MAX && c.col_wgs > kCUs && 2 * lds_col <= kMaxLds;
                     out.push_back(c);
                 }
             }

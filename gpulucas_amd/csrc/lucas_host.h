@@ -25,6 +25,7 @@ struct Plan {
     int threads = 0;        // work-items per workgroup (256, 512 or 1024)
     int radix_pref = 8;     // preferred power-of-two radix (8 or 4)
     int wA = 0, wB = 0;     // wave-register factor of each transform (64, 128, 256; 0: none)
+    int col_dense = 0;      // column kernel at 8 waves per SIMD (two 1024-item workgroups per CU)
     std::vector<int> radA, radB;  // Stockham radices of the remaining factors A/wA, B/wB
     size_t lds_col = 0, lds_row = 0;  // bytes of LDS per workgroup
 };

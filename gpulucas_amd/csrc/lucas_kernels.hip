@@ -825,16 +825,32 @@ LUCAS_KERNEL_PAIR(1024, 1)
 LUCAS_KERNEL_PAIR(1024, 2)
 LUCAS_KERNEL_PAIR(1024, 4)
 
+// Column kernels of 1024 work-items limited to 64 VGPRs (8 waves per SIMD),
+// so two workgroups share a CU: for grids with more column workgroups than
+// CUs.  The register cap costs a few spilled values.
+#define LUCAS_DENSE_COL(P)                                                                                   \
+    extern "C" __global__ void __launch_bounds__(1024, 8)                                                  \
+            lucas_col_kernel_1024d_p##P(const LucasJobDesc* __restrict__ jobs,                              \
+                                        const int2* __restrict__ wgmap, int mode) {                         \
+        col_kernel_body<1024, P>(jobs, wgmap, mode);                                                       \
+    }
+LUCAS_DENSE_COL(0)
+LUCAS_DENSE_COL(1)
+LUCAS_DENSE_COL(2)
+LUCAS_DENSE_COL(4)
+
 namespace lucas {
 
 namespace {
 typedef void (*ColKernel)(const LucasJobDesc*, const int2*, int);
 typedef void (*RowKernel)(const LucasJobDesc*, const int2*);
 // [workgroup size 256/512/1024][P 0/1/2/4]
-const ColKernel kColKernels[3][4] = {
+const ColKernel kColKernels[4][4] = {
         {lucas_col_kernel_256_p0, lucas_col_kernel_256_p1, lucas_col_kernel_256_p2, lucas_col_kernel_256_p4},
         {lucas_col_kernel_512_p0, lucas_col_kernel_512_p1, lucas_col_kernel_512_p2, lucas_col_kernel_512_p4},
-        {lucas_col_kernel_1024_p0, lucas_col_kernel_1024_p1, lucas_col_kernel_1024_p2, lucas_col_kernel_1024_p4}};
+        {lucas_col_kernel_1024_p0, lucas_col_kernel_1024_p1, lucas_col_kernel_1024_p2, lucas_col_kernel_1024_p4},
+        {lucas_col_kernel_1024d_p0, lucas_col_kernel_1024d_p1, lucas_col_kernel_1024d_p2,
+         lucas_col_kernel_1024d_p4}};
 const RowKernel kRowKernels[3][4] = {
         {lucas_row_kernel_256_p0, lucas_row_kernel_256_p1, lucas_row_kernel_256_p2, lucas_row_kernel_256_p4},
         {lucas_row_kernel_512_p0, lucas_row_kernel_512_p1, lucas_row_kernel_512_p2, lucas_row_kernel_512_p4},
@@ -844,11 +860,11 @@ int p_index(int p) { return p == 4 ? 3 : p; }
 }  // namespace
 
 hipError_t configure_kernels(size_t max_lds) {
-    for (int t = 0; t < 3; ++t)
+    for (int t = 0; t < 4; ++t)
         for (int q = 0; q < 4; ++q) {
             hipError_t e = hipFuncSetAttribute((const void*)kColKernels[t][q],
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
-            if (e == hipSuccess)
+            if (e == hipSuccess && t < 3)
                 e = hipFuncSetAttribute((const void*)kRowKernels[t][q], hipFuncAttributeMaxDynamicSharedMemorySize,
                                         (int)max_lds);
             if (e != hipSuccess) return e;
@@ -856,10 +872,11 @@ hipError_t configure_kernels(size_t max_lds) {
     return hipSuccess;
 }
 
-hipError_t launch_col_kernel(int threads, int p, const LucasJobDesc* jobs, const int2* map, int nwg, size_t lds,
-                             int mode, hipStream_t stream) {
-    hipLaunchKernelGGL(kColKernels[threads_index(threads)][p_index(p)], dim3(nwg), dim3(threads), lds, stream, jobs,
-                       map, mode);
+// dense: the 64-VGPR variant (1024 work-items only)
+hipError_t launch_col_kernel(int threads, int p, int dense, const LucasJobDesc* jobs, const int2* map, int nwg,
+                             size_t lds, int mode, hipStream_t stream) {
+    const int ti = threads == 1024 && dense ? 3 : threads_index(threads);
+    hipLaunchKernelGGL(kColKernels[ti][p_index(p)], dim3(nwg), dim3(threads), lds, stream, jobs, map, mode);
     return hipGetLastError();
 }
 

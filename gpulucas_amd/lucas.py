@@ -46,7 +46,7 @@ class PlanInfo(C.Structure):
     _fields_ = [("p", C.c_int), ("N", C.c_int), ("q", C.c_int), ("n_wide", C.c_int),
                 ("carry_depth", C.c_int), ("A", C.c_int), ("B", C.c_int), ("C", C.c_int),
                 ("halo", C.c_int), ("col_wgs", C.c_int), ("row_wgs", C.c_int),
-                ("threads", C.c_int), ("radix_pref", C.c_int), ("wave_A", C.c_int), ("wave_B", C.c_int),
+                ("threads", C.c_int), ("radix_pref", C.c_int), ("wave_A", C.c_int), ("wave_B", C.c_int), ("col_dense", C.c_int),
                 ("device_bytes", C.c_size_t)]
 
     def as_dict(self):

@@ -58,6 +58,7 @@ typedef struct {
     int threads;       /* work-items per workgroup */
     int radix_pref;    /* power-of-two radix preference of the FFT stages (8 or 4) */
     int wave_A, wave_B; /* wave-register factor of the column / row transform (64, 128, 256; 0: none) */
+    int col_dense;     /* 1: column kernel capped at 64 VGPRs, two 1024-item workgroups per CU */
     size_t device_bytes; /* device footprint of one job */
 } lucas_plan_info;
 

@@ -44,7 +44,7 @@ void run_grid(unsigned nblocks, unsigned nthreads, const std::function<void()>& 
 }  // namespace emu
 
 namespace lucas {
-hipError_t launch_col_kernel(int, int, const LucasJobDesc*, const int2*, int, size_t, int, hipStream_t);
+hipError_t launch_col_kernel(int, int, int, const LucasJobDesc*, const int2*, int, size_t, int, hipStream_t);
 hipError_t launch_row_kernel(int, int, const LucasJobDesc*, const int2*, int, size_t, hipStream_t);
 }
 
@@ -70,10 +70,10 @@ extern "C" int emu_run(int p, int N, int A, int C, int T, const int32_t* digits_
     for (int w = 0; w < pl.col_wgs; ++w) cmap.push_back({0, w});
     for (int w = 0; w < pl.row_wgs; ++w) rmap.push_back({0, w});
     if (iters > 0) {
-        launch_col_kernel(pl.threads, pl.wA / 64, &d, cmap.data(), pl.col_wgs, 0, LUCAS_IN_DIGITS, nullptr);
+        launch_col_kernel(pl.threads, pl.wA / 64, pl.col_dense, &d, cmap.data(), pl.col_wgs, 0, LUCAS_IN_DIGITS, nullptr);
         for (long long i = 0; i < iters; ++i) {
             launch_row_kernel(pl.threads, pl.wB / 64, &d, rmap.data(), pl.row_wgs, 0, nullptr);
-            launch_col_kernel(pl.threads, pl.wA / 64, &d, cmap.data(), pl.col_wgs, 0, i + 1 == iters ? LUCAS_OUT_DIGITS : 0, nullptr);
+            launch_col_kernel(pl.threads, pl.wA / 64, pl.col_dense, &d, cmap.data(), pl.col_wgs, 0, i + 1 == iters ? LUCAS_OUT_DIGITS : 0, nullptr);
         }
     }
     std::memcpy(digits_out, dig.data(), sizeof(int32_t) * N);

@@ -50,6 +50,33 @@ def test_digits_bit_exact_vs_oracle_from_s0(p, N, A, C):
         assert ctx.max_roundoff() < ROUNDOFF_LIMIT
 
 
+# (p, N, A, C, threads): wave-register factors P = 1, 2, 4 after 0..2 Stockham
+# passes, and a dense (two workgroups per CU) column kernel
+WAVE_SHAPES = [
+    (294919, 16384, 64, 4, 512),     # columns 64 (P=1), rows 128 (P=2)
+    (218737, 12288, 192, 2, 1024),   # columns 3 x 64
+    (294919, 16384, 512, 2, 512),    # columns 2 x 256 (P=4)
+    (368647, 20480, 16, 8, 1024),    # rows 5 x 128
+    (2424833, 131072, 64, 2, 1024),  # 512 column workgroups: dense column kernel
+]
+
+
+@pytest.mark.parametrize("p,N,A,C,T", WAVE_SHAPES)
+def test_wave_factor_shapes_bit_exact_vs_oracle(p, N, A, C, T):
+    print(f"case p={p} N={N} A={A} C={C} T={T}", flush=True)
+    plan = [x for x in lucas.plan_list(p, N) if x["A"] == A and x["C"] == C and x["threads"] == T][0]
+    assert plan["wave_A"] or plan["wave_B"]
+    if N == 131072:
+        assert plan["col_dense"] == 1
+    x = random_balanced(p, N, 12)
+    with lucas.Context(p, N, plan=plan) as ctx:
+        ctx.set_digits(x, 0)
+        ctx.step(2)
+        y, _ = oracle.ibdwt_iterate(p, N, x, 2)
+        assert np.array_equal(ctx.digits(), y)
+        assert ctx.max_roundoff() < ROUNDOFF_LIMIT
+
+
 @pytest.mark.parametrize("p,N,A,C", SHAPES[::2])
 def test_digits_bit_exact_vs_oracle_steady_state(p, N, A, C):
     print(f"case p={p} N={N} A={A} C={C}", flush=True)
