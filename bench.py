@@ -73,9 +73,10 @@ def measured_traffic(info):
     vals = {}
     for line in open(path):
         if line.startswith("traffic:"):
-            k, v = line[len("traffic:"):].split("=")
+            k, v = line[len("traffic:"):].split("=", 1)
             vals[k.strip()] = v.strip()
-    shape = f"A={info['A']} B={info['B']} C={info['C']} T={info['threads']} R={info['radix_pref']}"
+    shape = (f"A={info['A']} B={info['B']} C={info['C']} T={info['threads']} R={info['radix_pref']} "
+             f"W={info['wave_A']}/{info['wave_B']}")
     if vals.get("plan") != shape:
         return None
     return {k: int(v) for k, v in vals.items() if k != "plan"}
