@@ -173,7 +173,7 @@ std::vector<Plan> plan_candidates(int p, int N) {
                     c.radA = ra;
                     c.radB = rb;
                     c.col_wgs = B / C;
-                    c.row_wgs = A / 2 + 1;
+                    c.row_wgs = (A + 1) / 2;  // row pairs; rows 0 and A/2 share one
                     c.lds_col = lds_col;
                     c.lds_row = lds_row;
                     c.radix_pref = pref;

@@ -724,8 +724,9 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc* __restrict__
 }
 
 // ---------------------------------------------------------------------------
-// Row kernel: one workgroup per row pair (k, A-k) of one job.  Rows 0 and A/2
-// pair with themselves.  Frequency K = k + A*kc of the half-length complex
+// Row kernel: one workgroup per row pair (k, A-k) of one job, 0 < k < A/2.
+// Rows 0 and A/2 pair with themselves and share workgroup 0, so a launch has
+// ceil(A/2) workgroups.  Frequency K = k + A*kc of the half-length complex
 // transform Z pairs with M-K; the real-signal square is formed per pair as
 //   E = Z_K + conj(Z_{M-K}),  O = -i (Z_K - conj(Z_{M-K}))   (2x scaled)
 //   Z'_K     = E^2 + w O^2 + 2i E O,          w = omega_M^K
@@ -741,10 +742,12 @@ __device__ __forceinline__ void row_kernel_body(const LucasJobDesc* __restrict__
     const LucasJobDesc& J = jobs[jm.x];
     const int A = J.A, B = J.B;
     const int tid = threadIdx.x;
+    // workgroup 0: row 0, and row A/2 when A is even (both self-paired);
+    // workgroup k > 0: rows k and A-k
     const int r0 = jm.y;
-    const bool self = (r0 == 0) || (2 * r0 == A);
-    const int r1 = self ? r0 : A - r0;
-    const int nrows = self ? 1 : 2;
+    const bool self = r0 == 0;
+    const int r1 = self ? A / 2 : A - r0;
+    const int nrows = (self && (A & 1)) ? 1 : 2;
     // LDS carve-up, in the order of lucas_row_lds_words()
     double2* const tws = lds + lucas_pad_words(2 * B);  // stage twiddles of the length-B transform
     double2* const twu = tws + J.ntwsB;          // omega_B^t, t < B (untangle)
@@ -760,7 +763,7 @@ __device__ __forceinline__ void row_kernel_body(const LucasJobDesc* __restrict__
     for (int t = tid; t < J.ntwsB; t += T) tws[t] = gload2(J.twsB, t);
     for (int t = tid; t < B; t += T) twu[t] = gload2(J.twB, t);
     copy_stages<T>(lst, J.stB, J.nradB, tid);
-    const double2 wr = gload2(J.twMA, r0);
+    const double2 wr0 = gload2(J.twMA, r0), wr1 = gload2(J.twMA, r1);
     constexpr int wB = 64 * PB;
     const int B1 = PB ? B / wB : B;
     const SeqGeom g{nrows, pow2_recip(nrows), 0, 0u, B, 0, 0, 1};
@@ -770,22 +773,28 @@ __device__ __forceinline__ void row_kernel_body(const LucasJobDesc* __restrict__
     __syncthreads();
     fft_seq<T, PB>(lds, lst, J.nradB, tws, g, gA, B1, J.m_B1, twu, tid);
     {
-        // pairs (kc in row r0) <-> (pc in row r1): rows differ, or row r0 pairs
-        // with itself (row 0: pc = -kc mod B; row A/2: pc = B-1-kc)
-        const int last = !self ? B - 1 : (r0 == 0 ? B / 2 : (B - 1) / 2);
-        const int poff = self ? 0 : B;
-        for (int kc = tid; kc <= last; kc += T) {
-            int pc = (self && r0 == 0) ? B - kc : B - 1 - kc;
+        // Pairs (kc in row r0) <-> (pc in row r1) for rows k, A-k: kc < B,
+        // pc = B-1-kc.  Workgroup 0 pairs within each row instead: row 0 has
+        // kc <= B/2 with pc = -kc mod B, then row A/2 (LDS row 1) has
+        // kc <= (B-1)/2 with pc = B-1-kc.
+        const int n0 = self ? B / 2 + 1 : B;
+        const int npairs = self ? n0 + (nrows == 2 ? (B + 1) / 2 : 0) : B;
+        for (int t = tid; t < npairs; t += T) {
+            const bool half = self && t >= n0;  // row A/2 of workgroup 0
+            const int kc = half ? t - n0 : t;
+            int pc = (self && !half) ? B - kc : B - 1 - kc;
             if (pc == B) pc = 0;
-            const double2 zk = lds[pidx(kc)], zm = lds[pidx(poff + pc)];
-            const double2 w = cmul(wr, twu[kc]);
+            const int ko = half ? B + kc : kc;
+            const int po = (self && !half) ? pc : B + pc;
+            const double2 zk = lds[pidx(ko)], zm = lds[pidx(po)];
+            const double2 w = cmul(half ? wr1 : wr0, twu[kc]);
             const double2 E = cadd(zk, cconj(zm));
             const double2 O = mul_si<-1>(csub(zk, cconj(zm)));
             const double2 P = cadd(cmul(E, E), cmul(w, cmul(O, O)));
             const double2 Q = cmul(E, O);
             // stored conjugated for the inverse transform
-            lds[pidx(kc)] = make_double2(P.x - 2.0 * Q.y, -(P.y + 2.0 * Q.x));
-            if (poff + pc != kc) lds[pidx(poff + pc)] = make_double2(P.x + 2.0 * Q.y, P.y - 2.0 * Q.x);
+            lds[pidx(ko)] = make_double2(P.x - 2.0 * Q.y, -(P.y + 2.0 * Q.x));
+            if (po != ko) lds[pidx(po)] = make_double2(P.x + 2.0 * Q.y, P.y - 2.0 * Q.x);
         }
     }
     __syncthreads();

@@ -46,6 +46,9 @@ def plan():
     runs["small_primes_full"] = [(p, sorted({1, 2, 5, 100, p - 2})) for p in small]
     runs["small_composite_full"] = [(p, [1, 5, 100, 1000, p - 2]) for p in SMALL_COMPOSITE_MP]
     runs["mid_p859433"] = [(859433, [1000, 50000, 859431])]
+    # Long transforms (N = 2^19 .. 2.3M): the top of the serving range.
+    runs["large"] = [(9600037, [1000, 5000]), (20996011, [100, 2000]), (37156667, [100, 2000]),
+                     (37732501, [2000])]
     return runs
 
 

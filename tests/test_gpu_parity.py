@@ -143,6 +143,36 @@ def test_config3_p2976221_n163840(gmp_fixtures):
         assert ctx.max_roundoff() < ROUNDOFF_LIMIT
 
 
+def test_config3_p2976221_full_run_is_prime(gmp_fixtures):
+    """All 2,976,219 iterations of M2976221 at N=163840: residues at the GMP
+    checkpoints on the way, final residue 0."""
+    p, N = 2976221, 163840
+    with lucas.Context(p, N) as ctx:
+        for k in (20000, 100000):
+            ctx.step(k - ctx.iteration)
+            want, zero = fixture_res64(gmp_fixtures, "c3_p2976221", p, k)
+            assert ctx.res64() == (want, zero), k
+        while ctx.iteration < p - 2:
+            ctx.step(min(500000, p - 2 - ctx.iteration))
+            print(f"k={ctx.iteration} roundoff={ctx.max_roundoff():.4f}", flush=True)
+        assert ctx.res64() == (0, True)
+        assert ctx.max_roundoff() < ROUNDOFF_LIMIT
+
+
+@pytest.mark.parametrize("p,k", [(9600037, 1000), (9600037, 5000), (20996011, 100), (20996011, 2000),
+                                 (37156667, 100), (37156667, 2000), (37732501, 2000)])
+def test_long_transforms_vs_gmp(p, k, gmp_fixtures):
+    """N = 2^19 .. 2293760 (the top of the serving range, carry depth 3 at
+    q = 17): Res64 against GMP."""
+    N = lucas.default_N(p)
+    print(f"case p={p} N={N} k={k}", flush=True)
+    with lucas.Context(p, N) as ctx:
+        ctx.step(k)
+        want, zero = fixture_res64(gmp_fixtures, "large", p, k)
+        assert ctx.res64() == (want, zero)
+        assert ctx.max_roundoff() < ROUNDOFF_LIMIT
+
+
 def test_config3_near3m_primes(gmp_fixtures):
     from gpulucas_amd import workloads as wl
     for p in wl.near3m_primes():

@@ -44,7 +44,7 @@ def test_plan_config_shapes(p, N, q, n):
     assert info["halo"] == (n + 2) // 2
     assert info["B"] % info["C"] == 0
     assert info["col_wgs"] == info["B"] // info["C"]
-    assert info["row_wgs"] == info["A"] // 2 + 1
+    assert info["row_wgs"] == (info["A"] + 1) // 2
     assert info["device_bytes"] >= 20 * N
 
 
