@@ -22,6 +22,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "ms/LL-iter at p=2976221 N=160K; batched iters/s vs plain; p50/p99 latency"
 P_HEAD, N_HEAD = 2976221, 163840
+P_LONG, N_LONG = 37156667, 2097152  # Mersenne prime, N = 2^21
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6
 
@@ -44,6 +45,29 @@ def headline(lucas, steps, warmup):
     col_us, row_us = ctx.time_kernels(200)
     ctx.close()
     return ms, err, col_us, row_us, info
+
+
+def long_transform(lucas, p, N, steps, warmup):
+    """ms/iter at a long transform (N >= 2^21), where the iteration should be
+    HBM-bound: achieved = 32N algorithmic bytes per iteration / iteration time."""
+    with lucas.Context(p, N) as ctx:
+        info = ctx.info
+        ctx.step(max(warmup, 1))
+        ctx.max_roundoff(reset=True)
+        t0 = time.perf_counter()
+        ctx.step(steps)
+        t1 = time.perf_counter()
+        err = ctx.max_roundoff()
+        col_us, row_us = ctx.time_kernels(50)
+    us = 1e6 * (t1 - t0) / steps
+    gbs = 32 * N / (us * 1e-6) / 1e9
+    return {"workload": f"LL iteration p={p} N={N}, FP64 IBDWT", "p": p, "N": N,
+            "A": info["A"], "B": info["B"], "C": info["C"], "steps": steps,
+            "us_per_iter": round(us, 3), "iters_per_s": round(1e6 / us, 1), "max_roundoff": round(err, 5),
+            "roofline_iteration": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                                   "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 5),
+                                   "algorithmic_bytes_per_iter": 32 * N},
+            "kernel_us": {"lucas_col_kernel": round(col_us, 3), "lucas_row_kernel": round(row_us, 3)}}
 
 
 def batched_ratio(lucas, wl, iters, count, seed=4, lo=1_000_000, hi=5_000_000):
@@ -151,6 +175,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-long", action="store_true")
+    ap.add_argument("--long-steps", type=int, default=3000)
     ap.add_argument("--serve-jobs", type=int, default=48)
     ap.add_argument("--profile-run", action="store_true",
                     help="only the headline iterations (for rocprofv3 kernel-trace / counter passes)")
@@ -212,6 +238,8 @@ def main():
         key = "lucas_col_kernel" if dom_name == "lucas_col_kernel" else "lucas_row_kernel"
         out["roofline"]["traffic"] = tr.get(key)
         out["roofline"]["traffic_by_kernel"] = tr
+    if not a.no_long:
+        out["long_transform"] = long_transform(lucas, P_LONG, N_LONG, a.long_steps, 64)
     if not a.no_batch:
         bi, pi = batched_ratio(lucas, wl, a.batch_iters, 64)
         out["batched"] = {"workload": "64 primes in [1e6,5e6] (seed 4), default N each",

@@ -516,34 +516,33 @@ __device__ __forceinline__ int restore_digit(const unsigned long long* mag, cons
     return sum - h * (1 << W);
 }
 
-// New digits (2m, 2m+1) of element m = (row r, local column lc, global
-// column gc) from the rounded products in the tile: restoration of digits
-// 2m+1, 2m, 2m-1 and the single ripple x_j = d_j + h_{j-1}
-// (loadIntToDoubleIBDWT, gpuLucas.cu:730-766).
+// Carry-save restoration for element m = (row r, local column lc) of the
+// tile: digits 2m+1 and 2m, and with `first` also digit 2m-1 (carry only).
+// The products of digits 2m+1-k, k = 0 .. NC+2, are the rounded convolution
+// outputs stored as int64 bit patterns in the tile, read top-down; a local
+// column left of global column 0 is the previous row's halo (the cyclic wrap
+// of IrrBaseBalanced.cu:64-70).  r2m = r_{2m}; widths follow from
+// r_{j-1} = r_j + b (mod N).  Returns (d_2m, d_2m+1 + h_2m) and the carries
+// h_2m+1 and h_2m-1.
 template <int NC>
-__device__ __forceinline__ int2 carry_element(const double2* lds, const LucasJobDesc& J, int r, int lc, int gc,
-                                              int ncl, const int* rowr, const int* colr) {
-    const int A = J.A, N = J.N, q = J.q, bb = J.b;
+__device__ __forceinline__ int2 restore_element(const double2* lds, const LucasJobDesc& J, int r, int lc, int c0mH,
+                                                int ncl, int r2m, bool first, int* h1, int* hm1) {
     unsigned long long mag[NC + 3];
     bool neg[NC + 3];
     int w[NC + 3];
-    // products and widths of digits 2m+1-k, k = 0 .. NC+2 (top-down)
 #pragma unroll
     for (int k = 0; k < NC + 3; ++k) {
         const int ie = k >> 1;  // element offset back
-        int rr = r;
-        if (gc - ie < 0) rr = (r == 0) ? A - 1 : r - 1;
+        const int rr = (c0mH + lc - ie < 0) ? (r == 0 ? J.A - 1 : r - 1) : r;
         const double2 v = lds[pidx(rr * ncl + (lc - ie))];
-        const long long c = (long long)((k & 1) ? v.x : v.y);
+        const long long c = __double_as_longlong((k & 1) ? v.x : v.y);
         neg[k] = c < 0;
         mag[k] = c < 0 ? (unsigned long long)(-c) : (unsigned long long)c;
     }
-    // digit j is wide iff r_j < b, r_{j-1} = r_j + b (mod N)
-    int r2m = colr[2 * lc] + rowr[r];
-    if (r2m >= N) r2m -= N;
+    const int N = J.N, bb = J.b;
     int rv = r2m - bb;
     if (rv < 0) rv += N;
-    w[0] = q + (rv < bb ? 1 : 0);  // digit 2m+1
+    w[0] = J.q + (rv < bb ? 1 : 0);  // digit 2m+1
     rv = r2m;
 #pragma unroll
     for (int k = 1; k < NC + 3; ++k) {
@@ -551,14 +550,14 @@ __device__ __forceinline__ int2 carry_element(const double2* lds, const LucasJob
             rv += bb;
             if (rv >= N) rv -= N;
         }
-        w[k] = q + (rv < bb ? 1 : 0);  // digit 2m+1-k
+        w[k] = J.q + (rv < bb ? 1 : 0);  // digit 2m+1-k
     }
-    int hm1, h0, h1;
-    const int d1 = restore_digit<NC>(mag, neg, w, &h1);              // digit 2m+1
+    int h0;
+    const int d1 = restore_digit<NC>(mag, neg, w, h1);               // digit 2m+1
     const int d0 = restore_digit<NC>(mag + 1, neg + 1, w + 1, &h0);  // digit 2m
-    (void)restore_digit<NC>(mag + 2, neg + 2, w + 2, &hm1);          // digit 2m-1: carry only
-    (void)h1;
-    return make_int2(d0 + hm1, d1 + h0);
+    *hm1 = 0;
+    if (first) (void)restore_digit<NC>(mag + 2, neg + 2, w + 2, hm1);  // digit 2m-1: carry only
+    return make_int2(d0, d1 + h0);
 }
 
 // ---------------------------------------------------------------------------
@@ -597,6 +596,8 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc* __restrict__
     LucasStage* const lst = reinterpret_cast<LucasStage*>(p);  // column transform stages
     p += 2 * J.nradA;
     double2* const twL = p;  // omega_A^t, t < A (wave factor only)
+    p += PA ? A : 0;
+    signed char* const hb = reinterpret_cast<signed char*>(p);  // carries h_2m+1 of own elements, [A][C]
     const unsigned int seen = *reinterpret_cast<volatile unsigned int*>(J.maxerr);
     const double2* const twA = J.twA;
     const double2* const twMB = J.twMB;
@@ -669,38 +670,48 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc* __restrict__
             if (gc == 0 && r == 0) ve -= 2.0;
             const double re = rint(ve), ro = rint(vo);
             err = fmaxf(err, (float)fmax(fabs(ve - re), fabs(vo - ro)));
-            lds[pidx(t)] = make_double2(re, ro);
+            // |c| < 2^53 whenever the roundoff is meaningful: exact int64
+            lds[pidx(t)] = make_double2(__longlong_as_double((long long)re), __longlong_as_double((long long)ro));
         }
         __syncthreads();
-        // Carry-save restoration + single ripple for the own columns.  Element
-        // (r, lo) reads row r's columns at or left of its own, and for the wrap
-        // (global column 0) the halo of row r-1; T is a multiple of C, so the
-        // elements of a row share a pass of this loop.  After the barrier each
-        // work-item overwrites its own element with the weighted new digits:
-        // no later pass reads it, and halo columns are never written.
+        // Carry-save restoration + single ripple x_j = d_j + h_{j-1}
+        // (loadIntToDoubleIBDWT, gpuLucas.cu:730-766), one work-item per own
+        // element, rows in order.  T is a multiple of C, so a pass holds whole
+        // rows: its items read their own row's products and the halo of the
+        // row before, which no pass writes.  Stage 1 restores digits 2m, 2m+1
+        // (the first element of a row also digit 2m-1 for its carry) and
+        // publishes h_2m+1 in hb; after the barrier stage 2 adds h_2m-1 from
+        // the element to the left and overwrites its own element with the
+        // weighted new digits (or stores them).
+        const int c0mH = c0 - H;
         const int nper = (A * C + T - 1) / T;
         for (int i = 0; i < nper; ++i) {
             const int t = tid + i * T;
             const bool own = t < A * C;
             const int r = udiv(own ? t : 0, C, mC), lo = (own ? t : 0) - r * C;
             int2 x = make_int2(0, 0);
+            int h1 = 0, hm1 = 0;
             if (own) {
-                const int lc = lo + H, gc = c0 + lo;
+                const int lc = lo + H;
+                int r2m = colr[2 * lc] + rowr[r];
+                if (r2m >= N) r2m -= N;
                 switch (n) {
-                    case 2: x = carry_element<2>(lds, J, r, lc, gc, ncl, rowr, colr); break;
-                    case 3: x = carry_element<3>(lds, J, r, lc, gc, ncl, rowr, colr); break;
-                    case 4: x = carry_element<4>(lds, J, r, lc, gc, ncl, rowr, colr); break;
-                    case 5: x = carry_element<5>(lds, J, r, lc, gc, ncl, rowr, colr); break;
-                    default: x = carry_element<6>(lds, J, r, lc, gc, ncl, rowr, colr); break;
+                    case 2: x = restore_element<2>(lds, J, r, lc, c0mH, ncl, r2m, lo == 0, &h1, &hm1); break;
+                    case 3: x = restore_element<3>(lds, J, r, lc, c0mH, ncl, r2m, lo == 0, &h1, &hm1); break;
+                    case 4: x = restore_element<4>(lds, J, r, lc, c0mH, ncl, r2m, lo == 0, &h1, &hm1); break;
+                    case 5: x = restore_element<5>(lds, J, r, lc, c0mH, ncl, r2m, lo == 0, &h1, &hm1); break;
+                    default: x = restore_element<6>(lds, J, r, lc, c0mH, ncl, r2m, lo == 0, &h1, &hm1); break;
                 }
+                hb[t] = (signed char)h1;
             }
+            __syncthreads();
+            if (!own) continue;
+            x.x += lo == 0 ? hm1 : (int)hb[t - 1];
             if (mode & LUCAS_OUT_DIGITS) {
-                if (own) gstore2i(J.digits, (size_t)(c0 + lo) + (size_t)B * r, x);
+                gstore2i(J.digits, (size_t)(c0 + lo) + (size_t)B * r, x);
                 continue;
             }
-            const double2 y = weighted(r, lo, x);
-            __syncthreads();
-            if (own) lds[pidx(r * ncl + H + lo)] = y;
+            lds[pidx(r * ncl + H + lo)] = weighted(r, lo, x);
         }
         if (mode & LUCAS_OUT_DIGITS) {
             block_max_err<T>(err, red, seen, J.maxerr);

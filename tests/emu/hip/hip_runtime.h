@@ -93,6 +93,8 @@ inline float __shfl_xor(float v, int mask, int /*width*/) {
     __syncthreads();
     return r;
 }
+inline long long __double_as_longlong(double d) { long long u; std::memcpy(&u, &d, 8); return u; }
+inline double __longlong_as_double(long long u) { double d; std::memcpy(&d, &u, 8); return d; }
 inline int __clz(int x) { return x == 0 ? 32 : __builtin_clz((unsigned)x); }
 inline unsigned __brev(unsigned x) {
     unsigned r = 0;
