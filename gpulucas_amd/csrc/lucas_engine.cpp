@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "gpulucas.h"
 
@@ -349,27 +350,45 @@ int Engine::sync() {
 }
 
 int Engine::time_kernels(int iters, double* col_us, double* row_us) {
-    if (iters <= 0 || jobs_.empty()) return LUCAS_ERR_INVALID_ARG;
+    if (iters <= 0 || iters > 4096 || jobs_.empty()) return LUCAS_ERR_INVALID_ARG;
     hipSetDevice(device_);
     int rc = prepare();
     if (rc != LUCAS_OK) return rc;
-    // valid spectra in both buffers (digits untouched)
-    rc = launch_col(LUCAS_IN_DIGITS);
-    if (rc == LUCAS_OK) rc = launch_row();
+    // The iteration sequence row, col, row, col, ... (each kernel reads what
+    // the other just wrote, as in step()), with an event after every launch:
+    // interval 2i -> 2i+1 is row launch i, 2i+1 -> 2i+2 col launch i.
+    std::vector<hipEvent_t> ev(2 * (size_t)iters + 1, nullptr);
+    auto cleanup = [&]() {
+        for (hipEvent_t e : ev)
+            if (e) hipEventDestroy(e);
+    };
+    for (hipEvent_t& e : ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            cleanup();
+            return LUCAS_ERR_DEVICE;
+        }
+    rc = launch_col(LUCAS_IN_DIGITS);  // valid spectra; digits untouched
+    if (rc == LUCAS_OK && hipEventRecord(ev[0], stream_) != hipSuccess) rc = LUCAS_ERR_DEVICE;
+    for (int i = 0; i < iters && rc == LUCAS_OK; ++i) {
+        rc = launch_row();
+        if (rc == LUCAS_OK && hipEventRecord(ev[2 * i + 1], stream_) != hipSuccess) rc = LUCAS_ERR_DEVICE;
+        if (rc == LUCAS_OK) rc = launch_col(0);
+        if (rc == LUCAS_OK && hipEventRecord(ev[2 * i + 2], stream_) != hipSuccess) rc = LUCAS_ERR_DEVICE;
+    }
+    if (rc == LUCAS_OK && hipEventSynchronize(ev[2 * (size_t)iters]) != hipSuccess) rc = LUCAS_ERR_DEVICE;
+    double sum_row = 0.0, sum_col = 0.0;
+    for (int i = 0; i < iters && rc == LUCAS_OK; ++i) {
+        float a = 0.f, b = 0.f;
+        if (hipEventElapsedTime(&a, ev[2 * i], ev[2 * i + 1]) != hipSuccess ||
+            hipEventElapsedTime(&b, ev[2 * i + 1], ev[2 * i + 2]) != hipSuccess)
+            rc = LUCAS_ERR_DEVICE;
+        sum_row += a;
+        sum_col += b;
+    }
+    cleanup();
     if (rc != LUCAS_OK) return rc;
-    float ms = 0.f;
-    if (hipEventRecord(ev0_, stream_) != hipSuccess) return LUCAS_ERR_DEVICE;
-    for (int i = 0; i < iters && rc == LUCAS_OK; ++i) rc = launch_col(0);
-    if (rc != LUCAS_OK || hipEventRecord(ev1_, stream_) != hipSuccess) return LUCAS_ERR_DEVICE;
-    if (hipEventSynchronize(ev1_) != hipSuccess || hipEventElapsedTime(&ms, ev0_, ev1_) != hipSuccess)
-        return LUCAS_ERR_DEVICE;
-    *col_us = 1000.0 * ms / iters;
-    if (hipEventRecord(ev0_, stream_) != hipSuccess) return LUCAS_ERR_DEVICE;
-    for (int i = 0; i < iters && rc == LUCAS_OK; ++i) rc = launch_row();
-    if (rc != LUCAS_OK || hipEventRecord(ev1_, stream_) != hipSuccess) return LUCAS_ERR_DEVICE;
-    if (hipEventSynchronize(ev1_) != hipSuccess || hipEventElapsedTime(&ms, ev0_, ev1_) != hipSuccess)
-        return LUCAS_ERR_DEVICE;
-    *row_us = 1000.0 * ms / iters;
+    *row_us = 1000.0 * sum_row / iters;
+    *col_us = 1000.0 * sum_col / iters;
     return LUCAS_OK;
 }
 

@@ -44,8 +44,8 @@ public:
 
     // Advances every job by `iters` (>= 1) iterations; digits in, digits out.
     int step(long long iters);
-    // Mean device microseconds of each kernel over `iters` back-to-back
-    // launches (state is left valid: digits are not touched).
+    // Mean microseconds of each kernel launch over `iters` iterations in the
+    // step() order, from events after every launch (digits are not touched).
     int time_kernels(int iters, double* col_us, double* row_us);
     int sync();
 

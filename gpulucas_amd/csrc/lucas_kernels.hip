@@ -251,13 +251,11 @@ __device__ __forceinline__ void fft_stage(double2* lds, const LucasStage& st, co
             // this butterfly's R-1 twiddles (LDS); the first stage's block holds
             // 1+0i, a product that is exact, so no stage needs a branch here
             const double2* const twk = tw + st.twoff + k * (R - 1);
-            double2 w[R - 1];
 #pragma unroll
             for (int r = 0; r < R; ++r) v[i][r] = lds[pidx(base + (j + r * nb) * sIdx)];
+            // each twiddle is read where it is used (fewer live registers)
 #pragma unroll
-            for (int r = 1; r < R; ++r) w[r - 1] = twk[r - 1];
-#pragma unroll
-            for (int r = 1; r < R; ++r) v[i][r] = SIGN < 0 ? cmul(v[i][r], w[r - 1]) : cmulc(v[i][r], w[r - 1]);
+            for (int r = 1; r < R; ++r) v[i][r] = SIGN < 0 ? cmul(v[i][r], twk[r - 1]) : cmulc(v[i][r], twk[r - 1]);
             Dft<R, SIGN>::run(v[i]);
             outb[i] = base + (jq * Ns * R + k) * sIdx;
         }
@@ -358,21 +356,24 @@ __device__ __forceinline__ constexpr int brev_slot(int m) {
     return P == 4 ? (((m & 1) << 1) | (m >> 1)) : m;
 }
 
+// Cross-lane level of half-size H for all P values of the lane: the twiddle
+// omega_2H^(lane mod H) = omega_W^((lane mod H)*32P/H) is read from the LDS
+// table on the high lane (the low lane's is omega^0), just before use, so
+// no level's twiddle stays live across the others.
+template <int H, int P>
+__device__ __forceinline__ void xlevel_all(double2 (&v)[P], const double2* twL, int L1, int lane) {
+    const bool hi = (lane & H) != 0;
+    const double2 w = twL[hi ? (lane & (H - 1)) * (32 * P / H) * L1 : 0];
+    const double s = hi ? -1.0 : 1.0;
+#pragma unroll
+    for (int m = 0; m < P; ++m) v[m] = xlevel<H>(v[m], s, w);
+}
+
 // W = 64P point DIF of v (element 64m + lane in v[m]).  omega_W^u is read
 // from the LDS table twL of omega_L (t < L = L1*W) at u*L1.  Afterwards v[m]
 // holds X[brev6(lane)*P + brev_slot(m)].
 template <int P>
 __device__ __forceinline__ void wave_dif(double2 (&v)[P], const double2* twL, int L1, int lane) {
-    // cross-lane twiddles: omega_2h^(lane mod h) = omega_W^((lane mod h)*32P/h) on the high lane
-    double2 w[6];
-    double sg[6];
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-        const int h = 32 >> i;
-        const bool hi = (lane & h) != 0;
-        w[i] = twL[hi ? (lane & (h - 1)) * (32 * P / h) * L1 : 0];
-        sg[i] = hi ? -1.0 : 1.0;
-    }
     // in-register levels, half-size h = 64*hs: omega_2h^e = omega_W^(e*P/(2hs)), e = (64m + lane) mod h
 #pragma unroll
     for (int hs = P / 2; hs >= 1; hs >>= 1) {
@@ -385,18 +386,12 @@ __device__ __forceinline__ void wave_dif(double2 (&v)[P], const double2* twL, in
             v[m + hs] = cmul(csub(a, b), twL[e * (P / (2 * hs)) * L1]);
         }
     }
-#pragma unroll
-    for (int m = 0; m < P; ++m) v[m] = xlevel<32>(v[m], sg[0], w[0]);
-#pragma unroll
-    for (int m = 0; m < P; ++m) v[m] = xlevel<16>(v[m], sg[1], w[1]);
-#pragma unroll
-    for (int m = 0; m < P; ++m) v[m] = xlevel<8>(v[m], sg[2], w[2]);
-#pragma unroll
-    for (int m = 0; m < P; ++m) v[m] = xlevel<4>(v[m], sg[3], w[3]);
-#pragma unroll
-    for (int m = 0; m < P; ++m) v[m] = xlevel<2>(v[m], sg[4], w[4]);
-#pragma unroll
-    for (int m = 0; m < P; ++m) v[m] = xlevel<1>(v[m], sg[5], w[5]);
+    xlevel_all<32, P>(v, twL, L1, lane);
+    xlevel_all<16, P>(v, twL, L1, lane);
+    xlevel_all<8, P>(v, twL, L1, lane);
+    xlevel_all<4, P>(v, twL, L1, lane);
+    xlevel_all<2, P>(v, twL, L1, lane);
+    xlevel_all<1, P>(v, twL, L1, lane);
 }
 
 // Last factor of a length L = L1*W forward transform of the sequences of g,

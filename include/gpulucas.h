@@ -132,8 +132,10 @@ int lucas_ctx_set_digits(lucas_ctx* ctx, const int32_t* digits, long long iter);
 int lucas_ctx_res64(lucas_ctx* ctx, uint64_t* res64, int* is_zero);
 /* Max roundoff since the last reset (findMaxErrorHOST, gpuLucas.cu:811-819). */
 int lucas_ctx_max_roundoff(lucas_ctx* ctx, int reset, double* out);
-/* Mean device time per kernel over `iters` back-to-back launches of each
- * kernel alone, from HIP events on the context's stream (microseconds). */
+/* Mean time per kernel launch over `iters` iterations launched as in
+ * lucas_ctx_step (row kernel, column kernel, alternating), from HIP events
+ * recorded on the context's stream after every launch (microseconds; each
+ * interval includes that launch's dispatch gap).  Max iters 4096. */
 int lucas_ctx_time_kernels(lucas_ctx* ctx, int iters, double* col_us, double* row_us);
 /* Mean wall time of `iters` iterations (graph-launched), in ms/iteration. */
 int lucas_ctx_time_iterations(lucas_ctx* ctx, long long iters, double* ms_per_iter);
