@@ -161,7 +161,7 @@ std::vector<Plan> plan_candidates(int p, int N) {
                     const size_t lds_col = 16 * (size_t)lucas_col_lds_words(
                             A, C, pl.halo, stage_twiddle_count(A1, ra), (int)ra.size(), wA / 64);
                     const size_t lds_row =
-                            16 * (size_t)lucas_row_lds_words(B, stage_twiddle_count(B1, rb), (int)rb.size());
+                            16 * (size_t)lucas_row_lds_words(B, stage_twiddle_count(B1, rb), (int)rb.size(), wB / 64);
                     if (lds_col > kMaxLds || lds_row > kMaxLds) continue;
                     Plan c = pl;
                     c.A = A;

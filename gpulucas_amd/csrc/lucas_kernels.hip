@@ -248,14 +248,17 @@ __device__ __forceinline__ void fft_stage(double2* lds, const LucasStage& st, co
             const int jq = udiv(j, Ns, st.mNs);
             const int k = j - jq * Ns;
             const int base = seq_base(g, seq);
-            // this butterfly's R-1 twiddles (LDS); the first stage's block holds
-            // 1+0i, a product that is exact, so no stage needs a branch here
+            // this butterfly's R-1 twiddles (LDS)
             const double2* const twk = tw + st.twoff + k * (R - 1);
 #pragma unroll
             for (int r = 0; r < R; ++r) v[i][r] = lds[pidx(base + (j + r * nb) * sIdx)];
-            // each twiddle is read where it is used (fewer live registers)
+            // each twiddle is read where it is used (fewer live registers); the
+            // first stage's are all 1 (a wave-uniform branch skips them)
+            if (Ns > 1) {
 #pragma unroll
-            for (int r = 1; r < R; ++r) v[i][r] = SIGN < 0 ? cmul(v[i][r], twk[r - 1]) : cmulc(v[i][r], twk[r - 1]);
+                for (int r = 1; r < R; ++r)
+                    v[i][r] = SIGN < 0 ? cmul(v[i][r], twk[r - 1]) : cmulc(v[i][r], twk[r - 1]);
+            }
             Dft<R, SIGN>::run(v[i]);
             outb[i] = base + (jq * Ns * R + k) * sIdx;
         }
@@ -356,24 +359,42 @@ __device__ __forceinline__ constexpr int brev_slot(int m) {
     return P == 4 ? (((m & 1) << 1) | (m >> 1)) : m;
 }
 
-// Cross-lane level of half-size H for all P values of the lane: the twiddle
-// omega_2H^(lane mod H) = omega_W^((lane mod H)*32P/H) is read from the LDS
-// table on the high lane (the low lane's is omega^0), just before use, so
+// Cross-lane level of half-size H for all P values of the lane.  tab holds
+// omega_2H^t, t < H: the high lane takes omega_2H^(lane mod H), the low lane
+// omega^0 = tab[0].  Lanes read one address each, consecutive or shared, so
+// the LDS read is free of bank conflicts; it is issued just before use, so
 // no level's twiddle stays live across the others.
 template <int H, int P>
-__device__ __forceinline__ void xlevel_all(double2 (&v)[P], const double2* twL, int L1, int lane) {
+__device__ __forceinline__ void xlevel_all(double2 (&v)[P], const double2* tab, int lane) {
     const bool hi = (lane & H) != 0;
-    const double2 w = twL[hi ? (lane & (H - 1)) * (32 * P / H) * L1 : 0];
+    const double2 w = tab[hi ? (lane & (H - 1)) : 0];
     const double s = hi ? -1.0 : 1.0;
 #pragma unroll
     for (int m = 0; m < P; ++m) v[m] = xlevel<H>(v[m], s, w);
 }
 
-// W = 64P point DIF of v (element 64m + lane in v[m]).  omega_W^u is read
-// from the LDS table twL of omega_L (t < L = L1*W) at u*L1.  Afterwards v[m]
-// holds X[brev6(lane)*P + brev_slot(m)].
+// LDS twiddle table of a W = 64P point wave factor (lucas_wave_table_words(P)
+// entries): omega_W^u for u < W/2 (in-register levels), then for each
+// cross-lane half-size H = 1, 2, .., 32 the H entries omega_2H^t at offset
+// W/2 + H - 1.  Filled from the global table gtw of omega_L (L = L1*W).
+template <int T, int P>
+__device__ __forceinline__ void fill_wave_table(double2* twW, const double2* gtw, int L1, int tid) {
+    for (int t = tid; t < 32 * P + 63; t += T) {
+        int u = t;
+        if (t >= 32 * P) {
+            const int i = t - 32 * P;                // (H - 1) + tt
+            const int H = 1 << (31 - __clz(i + 1));  // largest power of two <= i + 1
+            u = (i - (H - 1)) * (32 * P / H);        // omega_2H^tt = omega_W^(tt*W/(2H))
+        }
+        twW[t] = gload2(gtw, (size_t)u * L1);
+    }
+}
+
+// W = 64P point DIF of v (element 64m + lane in v[m]), twiddles from the
+// wave table twW (fill_wave_table).  Afterwards v[m] holds
+// X[brev6(lane)*P + brev_slot(m)].
 template <int P>
-__device__ __forceinline__ void wave_dif(double2 (&v)[P], const double2* twL, int L1, int lane) {
+__device__ __forceinline__ void wave_dif(double2 (&v)[P], const double2* twW, int lane) {
     // in-register levels, half-size h = 64*hs: omega_2h^e = omega_W^(e*P/(2hs)), e = (64m + lane) mod h
 #pragma unroll
     for (int hs = P / 2; hs >= 1; hs >>= 1) {
@@ -383,15 +404,16 @@ __device__ __forceinline__ void wave_dif(double2 (&v)[P], const double2* twL, in
             const double2 a = v[m], b = v[m + hs];
             const int e = 64 * (m & (hs - 1)) + lane;
             v[m] = cadd(a, b);
-            v[m + hs] = cmul(csub(a, b), twL[e * (P / (2 * hs)) * L1]);
+            v[m + hs] = cmul(csub(a, b), twW[e * (P / (2 * hs))]);
         }
     }
-    xlevel_all<32, P>(v, twL, L1, lane);
-    xlevel_all<16, P>(v, twL, L1, lane);
-    xlevel_all<8, P>(v, twL, L1, lane);
-    xlevel_all<4, P>(v, twL, L1, lane);
-    xlevel_all<2, P>(v, twL, L1, lane);
-    xlevel_all<1, P>(v, twL, L1, lane);
+    const double2* const xt = twW + 32 * P - 1;  // level H at xt + H
+    xlevel_all<32, P>(v, xt + 32, lane);
+    xlevel_all<16, P>(v, xt + 16, lane);
+    xlevel_all<8, P>(v, xt + 8, lane);
+    xlevel_all<4, P>(v, xt + 4, lane);
+    xlevel_all<2, P>(v, xt + 2, lane);
+    xlevel_all<1, P>(v, xt + 1, lane);
 }
 
 // Last factor of a length L = L1*W forward transform of the sequences of g,
@@ -403,7 +425,7 @@ __device__ __forceinline__ void wave_dif(double2 (&v)[P], const double2* twL, in
 // number within the workgroup's waves); all reads precede all writes.
 template <int T, int P>
 __device__ __forceinline__ void fft_wave(double2* lds, const SeqGeom& g, int L1, unsigned int mL1,
-                                         const double2* twL, int tid) {
+                                         const double2* twL, const double2* twW, int tid) {
     constexpr int W = 64 * P;
     const int lane = tid & 63, sub = tid >> 6;
     const bool active = sub < g.nsq * L1;  // wave-uniform
@@ -419,7 +441,7 @@ __device__ __forceinline__ void fft_wave(double2* lds, const SeqGeom& g, int L1,
             const double2 x = lds[pidx(base + (W * k1 + n2) * g.sIdx)];
             v[m] = L1 > 1 ? cmul(x, twL[n2 * k1]) : x;
         }
-        wave_dif<P>(v, twL, L1, lane);
+        wave_dif<P>(v, twW, lane);
         ob = base + k1 * g.sIdx;
     }
     __syncthreads();
@@ -437,10 +459,10 @@ __device__ __forceinline__ void fft_wave(double2* lds, const SeqGeom& g, int L1,
 template <int T, int P>
 __device__ __forceinline__ void fft_seq(double2* lds, const LucasStage* stages, int nst, const double2* tws,
                                         const SeqGeom& g, const SeqGeom& gA, int L1, unsigned int mL1,
-                                        const double2* twL, int tid) {
+                                        const double2* twL, const double2* twW, int tid) {
     if constexpr (P > 0) {
         fft_lds<T, -1, true>(lds, stages, nst, gA, tws, tid);
-        fft_wave<T, P>(lds, g, L1, mL1, twL, tid);
+        fft_wave<T, P>(lds, g, L1, mL1, twL, twW, tid);
     } else {
         fft_lds<T, -1, true>(lds, stages, nst, g, tws, tid);
     }
@@ -592,6 +614,8 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc* __restrict__
     p += 2 * J.nradA;
     double2* const twL = p;  // omega_A^t, t < A (wave factor only)
     p += PA ? A : 0;
+    double2* const twW = p;  // wave-factor twiddles (fill_wave_table)
+    p += lucas_wave_table_words(PA);
     signed char* const hb = reinterpret_cast<signed char*>(p);  // carries h_2m+1 of own elements, [A][C]
     const unsigned int seen = *reinterpret_cast<volatile unsigned int*>(J.maxerr);
     const double2* const twA = J.twA;
@@ -611,10 +635,12 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc* __restrict__
     }
     for (int t = tid; t < J.ntwsA; t += T) tws[t] = gload2(J.twsA, t);
     copy_stages<T>(lst, J.stA, J.nradA, tid);
-    if (PA)
-        for (int t = tid; t < A; t += T) twL[t] = gload2(twA, t);
     constexpr int wA = 64 * PA;
     const int A1 = PA ? A / wA : A;
+    if (PA) {
+        for (int t = tid; t < A; t += T) twL[t] = gload2(twA, t);
+        fill_wave_table<T, PA>(twW, twA, A1, tid);
+    }
     for (int t = tid; t < A; t += T) {
         roww[t] = make_double2(gload(J.whi, t), gload(J.iwhi, t));
         rowr[t] = gload(J.rhi, t);
@@ -648,7 +674,7 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc* __restrict__
         // inverse column FFT of the tile and halo
         const SeqGeom g{ncl, mncl, 0, 0u, 1, 0, 0, ncl};
         const SeqGeom gA{ncl * wA, J.m_sqA0, 0, 0u, 1, 0, 0, wA * ncl};
-        fft_seq<T, PA>(lds, lst, J.nradA, tws, g, gA, A1, J.m_A1, twL, tid);
+        fft_seq<T, PA>(lds, lst, J.nradA, tws, g, gA, A1, J.m_A1, twL, twW, tid);
         // unweight, -2 on digit 0, round half-even, roundoff
         for (int t = tid; t < A * ncl; t += T) {
             const int r = udiv(t, ncl, mncl), lc = t - r * ncl;
@@ -718,7 +744,7 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc* __restrict__
     {
         const SeqGeom g{C, mC, 0, 0u, 1, 0, H, ncl};
         const SeqGeom gA{C * wA, J.m_sqA1, C, mC, 1, ncl, H, wA * ncl};
-        fft_seq<T, PA>(lds, lst, J.nradA, tws, g, gA, A1, J.m_A1, twL, tid);
+        fft_seq<T, PA>(lds, lst, J.nradA, tws, g, gA, A1, J.m_A1, twL, twW, tid);
     }
     // store row-major; the outer twiddle omega_M^(c*k) is applied by the row
     // kernel as it loads
@@ -758,6 +784,7 @@ __device__ __forceinline__ void row_kernel_body(const LucasJobDesc* __restrict__
     double2* const tws = lds + lucas_pad_words(2 * B);  // stage twiddles of the length-B transform
     double2* const twu = tws + J.ntwsB;          // omega_B^t, t < B (untangle)
     LucasStage* const lst = reinterpret_cast<LucasStage*>(twu + B);  // row transform stages
+    double2* const twW = twu + B + 2 * J.nradB;  // wave-factor twiddles (fill_wave_table)
 
     // load both rows with the column kernel's outer twiddle omega_M^(c*k);
     // the tables go to LDS while those loads are in flight
@@ -769,6 +796,7 @@ __device__ __forceinline__ void row_kernel_body(const LucasJobDesc* __restrict__
     for (int t = tid; t < J.ntwsB; t += T) tws[t] = gload2(J.twsB, t);
     for (int t = tid; t < B; t += T) twu[t] = gload2(J.twB, t);
     copy_stages<T>(lst, J.stB, J.nradB, tid);
+    if (PB) fill_wave_table<T, PB>(twW, J.twB, J.B / (64 * PB), tid);
     const double2 wr0 = gload2(J.twMA, r0), wr1 = gload2(J.twMA, r1);
     constexpr int wB = 64 * PB;
     const int B1 = PB ? B / wB : B;
@@ -777,7 +805,7 @@ __device__ __forceinline__ void row_kernel_body(const LucasJobDesc* __restrict__
     // forward row FFT; square; inverse row FFT (conjugated input).  Two call
     // sites: a loop over the two transforms costs registers across the loop.
     __syncthreads();
-    fft_seq<T, PB>(lds, lst, J.nradB, tws, g, gA, B1, J.m_B1, twu, tid);
+    fft_seq<T, PB>(lds, lst, J.nradB, tws, g, gA, B1, J.m_B1, twu, twW, tid);
     {
         // Pairs (kc in row r0) <-> (pc in row r1) for rows k, A-k: kc < B,
         // pc = B-1-kc.  Workgroup 0 pairs within each row instead: row 0 has
@@ -804,7 +832,7 @@ __device__ __forceinline__ void row_kernel_body(const LucasJobDesc* __restrict__
         }
     }
     __syncthreads();
-    fft_seq<T, PB>(lds, lst, J.nradB, tws, g, gA, B1, J.m_B1, twu, tid);
+    fft_seq<T, PB>(lds, lst, J.nradB, tws, g, gA, B1, J.m_B1, twu, twW, tid);
     for (int t = tid; t < nrows * B; t += T) {
         const int rl = t >= B ? 1 : 0, kc = t - rl * B;
         gstore2(J.spec2, (size_t)(rl ? r1 : r0) * B + kc, cconj(lds[pidx(t)]));
