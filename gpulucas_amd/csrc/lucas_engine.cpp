@@ -24,6 +24,9 @@ hipError_t launch_col_kernel(int threads, int p, int dense, const LucasJobDesc* 
                              size_t lds, int mode, hipStream_t stream);
 hipError_t launch_row_kernel(int threads, int p, const LucasJobDesc* jobs, const int2* map, int nwg, size_t lds,
                              hipStream_t stream);
+hipError_t launch_col1_kernel(int threads, int p, int dense, const LucasJobDesc& job, int nwg, size_t lds, int mode,
+                              hipStream_t stream);
+hipError_t launch_row1_kernel(int threads, int p, const LucasJobDesc& job, int nwg, size_t lds, hipStream_t stream);
 hipError_t configure_kernels(size_t max_lds);
 
 static const int kGraphIters = 32;
@@ -268,6 +271,9 @@ int Engine::prepare() {
     }
     if (e == hipSuccess) e = hipStreamSynchronize(stream_);
     if (e != hipSuccess) return LUCAS_ERR_DEVICE;
+    // one job: its descriptor goes by value with every launch
+    single_ = desc.size() == 1;
+    if (single_) one_ = desc[0];
     dirty_ = false;
     return LUCAS_OK;
 }
@@ -275,8 +281,10 @@ int Engine::prepare() {
 int Engine::launch_col(int mode) {
     for (const Group& g : colg_) {
         if (!g.n) continue;
-        if (launch_col_kernel(g.threads, g.p, g.dense, d_desc_, g.d_map, g.n, g.lds, mode, stream_) != hipSuccess)
-            return LUCAS_ERR_DEVICE;
+        const hipError_t e =
+                single_ ? launch_col1_kernel(g.threads, g.p, g.dense, one_, g.n, g.lds, mode, stream_)
+                        : launch_col_kernel(g.threads, g.p, g.dense, d_desc_, g.d_map, g.n, g.lds, mode, stream_);
+        if (e != hipSuccess) return LUCAS_ERR_DEVICE;
     }
     return LUCAS_OK;
 }
@@ -284,8 +292,9 @@ int Engine::launch_col(int mode) {
 int Engine::launch_row() {
     for (const Group& g : rowg_) {
         if (!g.n) continue;
-        if (launch_row_kernel(g.threads, g.p, d_desc_, g.d_map, g.n, g.lds, stream_) != hipSuccess)
-            return LUCAS_ERR_DEVICE;
+        const hipError_t e = single_ ? launch_row1_kernel(g.threads, g.p, one_, g.n, g.lds, stream_)
+                                     : launch_row_kernel(g.threads, g.p, d_desc_, g.d_map, g.n, g.lds, stream_);
+        if (e != hipSuccess) return LUCAS_ERR_DEVICE;
     }
     return LUCAS_OK;
 }

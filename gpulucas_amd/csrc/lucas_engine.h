@@ -74,6 +74,8 @@ private:
     bool dirty_ = true;
     LucasJobDesc* d_desc_ = nullptr;
     size_t cap_desc_ = 0;
+    bool single_ = false;  // one job: launch the by-value-descriptor kernels with one_
+    LucasJobDesc one_;
     Group colg_[kGroups], rowg_[kGroups];
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t gexec_ = nullptr;

@@ -585,14 +585,11 @@ __device__ __forceinline__ int2 restore_element(const double2* lds, const LucasJ
 // copied into LDS while the tile's own loads are in flight.
 // ---------------------------------------------------------------------------
 template <int T, int PA>
-__device__ __forceinline__ void col_kernel_body(const LucasJobDesc* __restrict__ jobs, const int2* __restrict__ wgmap,
-                                                int mode) {
+__device__ __forceinline__ void col_kernel_body(const LucasJobDesc& J, int tile, int mode) {
     double2* const lds = lucas_dynamic_lds();
-    const int2 jm = wgmap[blockIdx.x];
-    const LucasJobDesc& J = jobs[jm.x];
     const int A = J.A, B = J.B, C = J.C, H = J.halo, N = J.N;
     const int n = J.ncarry;
-    const int c0 = jm.y * C;
+    const int c0 = tile * C;
     const int tid = threadIdx.x;
     const int ncl = C + H;  // local columns: [0,H) halo, [H, H+C) own
     const unsigned int mncl = J.m_ncl, mC = J.m_C;
@@ -767,16 +764,12 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc* __restrict__
 // folded into the unweight table.
 // ---------------------------------------------------------------------------
 template <int T, int PB>
-__device__ __forceinline__ void row_kernel_body(const LucasJobDesc* __restrict__ jobs,
-                                                const int2* __restrict__ wgmap) {
+__device__ __forceinline__ void row_kernel_body(const LucasJobDesc& J, int r0) {
     double2* const lds = lucas_dynamic_lds();
-    const int2 jm = wgmap[blockIdx.x];
-    const LucasJobDesc& J = jobs[jm.x];
     const int A = J.A, B = J.B;
     const int tid = threadIdx.x;
     // workgroup 0: row 0, and row A/2 when A is even (both self-paired);
     // workgroup k > 0: rows k and A-k
-    const int r0 = jm.y;
     const bool self = r0 == 0;
     const int r1 = self ? A / 2 : A - r0;
     const int nrows = (self && (A & 1)) ? 1 : 2;
@@ -839,21 +832,41 @@ __device__ __forceinline__ void row_kernel_body(const LucasJobDesc* __restrict__
     }
 }
 
+// Column tile of workgroup `wg` of a single-job launch: the tiles are dealt
+// to the 8 XCDs (workgroup 8k + x runs tile k of XCD x's contiguous range)
+// so neighbouring tiles, which share halo lines, share an L2.
+__device__ __forceinline__ int single_col_tile(int wg, int n) {
+    return (n % 8 == 0 && n >= 16) ? (n / 8) * (wg & 7) + (wg >> 3) : wg;
+}
+
 }  // namespace
 
 // One kernel per (workgroup size, wave factor P of its transform): each
 // holds a single wave-factor variant, which keeps the code of a column/row
-// kernel pair within the instruction cache.
+// kernel pair within the instruction cache.  The batched kernels read
+// (job, tile) from a workgroup map and the job's descriptor from an array;
+// the single-job kernels take the descriptor by value (kernel argument
+// segment) and derive the tile from the workgroup id, which takes two
+// dependent memory reads off the start of every workgroup.
 #define LUCAS_KERNEL_PAIR(T, P)                                                                            \
     extern "C" __global__ void __launch_bounds__(T)                                                      \
             lucas_col_kernel_##T##_p##P(const LucasJobDesc* __restrict__ jobs,                            \
                                         const int2* __restrict__ wgmap, int mode) {                       \
-        col_kernel_body<T, P>(jobs, wgmap, mode);                                                        \
+        const int2 jm = wgmap[blockIdx.x];                                                               \
+        col_kernel_body<T, P>(jobs[jm.x], jm.y, mode);                                                   \
     }                                                                                                    \
     extern "C" __global__ void __launch_bounds__(T)                                                      \
             lucas_row_kernel_##T##_p##P(const LucasJobDesc* __restrict__ jobs,                            \
                                         const int2* __restrict__ wgmap) {                                 \
-        row_kernel_body<T, P>(jobs, wgmap);                                                              \
+        const int2 jm = wgmap[blockIdx.x];                                                               \
+        row_kernel_body<T, P>(jobs[jm.x], jm.y);                                                         \
+    }                                                                                                    \
+    extern "C" __global__ void __launch_bounds__(T)                                                      \
+            lucas_col1_kernel_##T##_p##P(const LucasJobDesc J, int mode) {                                 \
+        col_kernel_body<T, P>(J, single_col_tile(blockIdx.x, J.col_wgs), mode);                          \
+    }                                                                                                    \
+    extern "C" __global__ void __launch_bounds__(T) lucas_row1_kernel_##T##_p##P(const LucasJobDesc J) {  \
+        row_kernel_body<T, P>(J, blockIdx.x);                                                            \
     }
 LUCAS_KERNEL_PAIR(256, 0)
 LUCAS_KERNEL_PAIR(256, 1)
@@ -875,7 +888,12 @@ LUCAS_KERNEL_PAIR(1024, 4)
     extern "C" __global__ void __launch_bounds__(1024, 8)                                                  \
             lucas_col_kernel_1024d_p##P(const LucasJobDesc* __restrict__ jobs,                              \
                                         const int2* __restrict__ wgmap, int mode) {                         \
-        col_kernel_body<1024, P>(jobs, wgmap, mode);                                                       \
+        const int2 jm = wgmap[blockIdx.x];                                                                 \
+        col_kernel_body<1024, P>(jobs[jm.x], jm.y, mode);                                                  \
+    }                                                                                                      \
+    extern "C" __global__ void __launch_bounds__(1024, 8)                                                  \
+            lucas_col1_kernel_1024d_p##P(const LucasJobDesc J, int mode) {                                   \
+        col_kernel_body<1024, P>(J, single_col_tile(blockIdx.x, J.col_wgs), mode);                          \
     }
 LUCAS_DENSE_COL(0)
 LUCAS_DENSE_COL(1)
@@ -887,6 +905,8 @@ namespace lucas {
 namespace {
 typedef void (*ColKernel)(const LucasJobDesc*, const int2*, int);
 typedef void (*RowKernel)(const LucasJobDesc*, const int2*);
+typedef void (*Col1Kernel)(const LucasJobDesc, int);
+typedef void (*Row1Kernel)(const LucasJobDesc);
 // [workgroup size 256/512/1024][P 0/1/2/4]
 const ColKernel kColKernels[4][4] = {
         {lucas_col_kernel_256_p0, lucas_col_kernel_256_p1, lucas_col_kernel_256_p2, lucas_col_kernel_256_p4},
@@ -898,6 +918,18 @@ const RowKernel kRowKernels[3][4] = {
         {lucas_row_kernel_256_p0, lucas_row_kernel_256_p1, lucas_row_kernel_256_p2, lucas_row_kernel_256_p4},
         {lucas_row_kernel_512_p0, lucas_row_kernel_512_p1, lucas_row_kernel_512_p2, lucas_row_kernel_512_p4},
         {lucas_row_kernel_1024_p0, lucas_row_kernel_1024_p1, lucas_row_kernel_1024_p2, lucas_row_kernel_1024_p4}};
+const Col1Kernel kCol1Kernels[4][4] = {
+        {lucas_col1_kernel_256_p0, lucas_col1_kernel_256_p1, lucas_col1_kernel_256_p2, lucas_col1_kernel_256_p4},
+        {lucas_col1_kernel_512_p0, lucas_col1_kernel_512_p1, lucas_col1_kernel_512_p2, lucas_col1_kernel_512_p4},
+        {lucas_col1_kernel_1024_p0, lucas_col1_kernel_1024_p1, lucas_col1_kernel_1024_p2,
+         lucas_col1_kernel_1024_p4},
+        {lucas_col1_kernel_1024d_p0, lucas_col1_kernel_1024d_p1, lucas_col1_kernel_1024d_p2,
+         lucas_col1_kernel_1024d_p4}};
+const Row1Kernel kRow1Kernels[3][4] = {
+        {lucas_row1_kernel_256_p0, lucas_row1_kernel_256_p1, lucas_row1_kernel_256_p2, lucas_row1_kernel_256_p4},
+        {lucas_row1_kernel_512_p0, lucas_row1_kernel_512_p1, lucas_row1_kernel_512_p2, lucas_row1_kernel_512_p4},
+        {lucas_row1_kernel_1024_p0, lucas_row1_kernel_1024_p1, lucas_row1_kernel_1024_p2,
+         lucas_row1_kernel_1024_p4}};
 int threads_index(int threads) { return threads == 1024 ? 2 : (threads == 512 ? 1 : 0); }
 int p_index(int p) { return p == 4 ? 3 : p; }
 }  // namespace
@@ -907,8 +939,14 @@ hipError_t configure_kernels(size_t max_lds) {
         for (int q = 0; q < 4; ++q) {
             hipError_t e = hipFuncSetAttribute((const void*)kColKernels[t][q],
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
+            if (e == hipSuccess)
+                e = hipFuncSetAttribute((const void*)kCol1Kernels[t][q], hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)max_lds);
             if (e == hipSuccess && t < 3)
                 e = hipFuncSetAttribute((const void*)kRowKernels[t][q], hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)max_lds);
+            if (e == hipSuccess && t < 3)
+                e = hipFuncSetAttribute((const void*)kRow1Kernels[t][q], hipFuncAttributeMaxDynamicSharedMemorySize,
                                         (int)max_lds);
             if (e != hipSuccess) return e;
         }
@@ -927,6 +965,19 @@ hipError_t launch_row_kernel(int threads, int p, const LucasJobDesc* jobs, const
                              hipStream_t stream) {
     hipLaunchKernelGGL(kRowKernels[threads_index(threads)][p_index(p)], dim3(nwg), dim3(threads), lds, stream, jobs,
                        map);
+    return hipGetLastError();
+}
+
+// Single-job launches: the descriptor travels as the kernel argument.
+hipError_t launch_col1_kernel(int threads, int p, int dense, const LucasJobDesc& job, int nwg, size_t lds, int mode,
+                              hipStream_t stream) {
+    const int ti = threads == 1024 && dense ? 3 : threads_index(threads);
+    hipLaunchKernelGGL(kCol1Kernels[ti][p_index(p)], dim3(nwg), dim3(threads), lds, stream, job, mode);
+    return hipGetLastError();
+}
+
+hipError_t launch_row1_kernel(int threads, int p, const LucasJobDesc& job, int nwg, size_t lds, hipStream_t stream) {
+    hipLaunchKernelGGL(kRow1Kernels[threads_index(threads)][p_index(p)], dim3(nwg), dim3(threads), lds, stream, job);
     return hipGetLastError();
 }
 

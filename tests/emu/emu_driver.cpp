@@ -46,7 +46,14 @@ void run_grid(unsigned nblocks, unsigned nthreads, const std::function<void()>& 
 namespace lucas {
 hipError_t launch_col_kernel(int, int, int, const LucasJobDesc*, const int2*, int, size_t, int, hipStream_t);
 hipError_t launch_row_kernel(int, int, const LucasJobDesc*, const int2*, int, size_t, hipStream_t);
+hipError_t launch_col1_kernel(int, int, int, const LucasJobDesc&, int, size_t, int, hipStream_t);
+hipError_t launch_row1_kernel(int, int, const LucasJobDesc&, int, size_t, hipStream_t);
 }
+
+// 1: launch the single-job kernels (descriptor by value, tile from the
+// workgroup id) instead of the batched ones (map + descriptor array).
+static int g_single = 0;
+extern "C" void emu_set_single(int on) { g_single = on; }
 
 extern "C" int emu_run(int p, int N, int A, int C, int T, const int32_t* digits_in, long long iters,
                        int32_t* digits_out, double* maxerr) {
@@ -69,11 +76,23 @@ extern "C" int emu_run(int p, int N, int A, int C, int T, const int32_t* digits_
     std::vector<int2> cmap, rmap;
     for (int w = 0; w < pl.col_wgs; ++w) cmap.push_back({0, w});
     for (int w = 0; w < pl.row_wgs; ++w) rmap.push_back({0, w});
-    if (iters > 0) {
-        launch_col_kernel(pl.threads, pl.wA / 64, pl.col_dense, &d, cmap.data(), pl.col_wgs, 0, LUCAS_IN_DIGITS, nullptr);
-        for (long long i = 0; i < iters; ++i) {
+    auto col = [&](int mode) {
+        if (g_single)
+            launch_col1_kernel(pl.threads, pl.wA / 64, pl.col_dense, d, pl.col_wgs, 0, mode, nullptr);
+        else
+            launch_col_kernel(pl.threads, pl.wA / 64, pl.col_dense, &d, cmap.data(), pl.col_wgs, 0, mode, nullptr);
+    };
+    auto row = [&]() {
+        if (g_single)
+            launch_row1_kernel(pl.threads, pl.wB / 64, d, pl.row_wgs, 0, nullptr);
+        else
             launch_row_kernel(pl.threads, pl.wB / 64, &d, rmap.data(), pl.row_wgs, 0, nullptr);
-            launch_col_kernel(pl.threads, pl.wA / 64, pl.col_dense, &d, cmap.data(), pl.col_wgs, 0, i + 1 == iters ? LUCAS_OUT_DIGITS : 0, nullptr);
+    };
+    if (iters > 0) {
+        col(LUCAS_IN_DIGITS);
+        for (long long i = 0; i < iters; ++i) {
+            row();
+            col(i + 1 == iters ? LUCAS_OUT_DIGITS : 0);
         }
     }
     std::memcpy(digits_out, dig.data(), sizeof(int32_t) * N);

@@ -50,3 +50,24 @@ def test_emulated_shapes_take_the_wave_path(p, N, A, C, T, wA, wB):
     from gpulucas_amd import lucas
     pl = [x for x in lucas.plan_list(p, N) if x["A"] == A and x["C"] == C and x["threads"] == T][0]
     assert (pl["wave_A"], pl["wave_B"]) == (wA, wB)
+
+
+@pytest.mark.parametrize("p,N,A,C,iters,T", [
+    (1279, 64, 4, 1, 2, 0),          # C < halo, row wrap
+    (9941, 560, 7, 4, 2, 0),         # odd A, radix 7
+    (44497, 2560, 40, 1, 1, 256),    # 32 column tiles: XCD-dealt tile order
+    (294919, 16384, 64, 4, 1, 512),  # wave factors, 32 column tiles
+])
+def test_emulated_single_job_kernels_match_oracle(p, N, A, C, iters, T):
+    """The single-job kernels (descriptor as kernel argument, tile from the
+    workgroup id and XCD dealing) compute the same digits."""
+    x = oracle.s0_digits(N) if iters > 1 else None
+    if x is None:
+        w, _, _, _ = oracle.tables(p, N)
+        rng = np.random.default_rng(7)
+        half = (1 << (w - 1)).astype(np.int64)
+        x = (rng.integers(0, 2 * half) - half).astype(np.int32)
+    got, err = emu.run(p, N, x, iters, A, C, T, single=True)
+    want, _ = oracle.ibdwt_iterate(p, N, x, iters)
+    assert np.array_equal(got, want)
+    assert err < 0.4
