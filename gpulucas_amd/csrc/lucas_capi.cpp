@@ -281,7 +281,8 @@ int lucas_ctx_create_shape(int p, int N, int A, int C, int device, lucas_ctx** o
 int lucas_ctx_create_plan(const lucas_plan_info* plan, int device, lucas_ctx** out) {
     if (!plan || !out) return LUCAS_ERR_INVALID_ARG;
     Plan pl;
-    if (!make_plan_exact(plan->p, plan->N, plan->A, plan->C, plan->threads, plan->radix_pref, &pl).empty())
+    if (!make_plan_exact(plan->p, plan->N, plan->A, plan->C, plan->threads, plan->radix_pref, &pl, plan->wave_A,
+                         plan->wave_B).empty())
         return LUCAS_ERR_INVALID_ARG;
     return ctx_new(pl, device, out);
 }

@@ -126,13 +126,17 @@ static long long model_cost(const Plan& pl) {
     return 64 * (col_t + row_t + coal) + total / 64;
 }
 
-// Wave-register factor W (256, 128 or 64) of a length-L transform over nseq
-// sequences with T work-items: W | L, and the nseq*L/W sub-transforms fit
-// one per wave of the workgroup.  0: none.
-static int wave_factor(int L, int nseq, int T) {
+// Wave-register factors W (256, 128, 64) a length-L transform over nseq
+// sequences with T work-items admits: W | L, and the nseq*L/W sub-transforms
+// fit one per wave of the workgroup; largest first, then 0 (none).  A larger
+// W leaves fewer Stockham stages; a smaller one spreads the transform over
+// more waves with shorter per-wave chains.
+static std::vector<int> wave_factors(int L, int nseq, int T) {
+    std::vector<int> out;
     for (int W : {256, 128, 64})
-        if (L % W == 0 && (long long)nseq * (L / W) <= T / 64) return W;
-    return 0;
+        if (L % W == 0 && (long long)nseq * (L / W) <= T / 64) out.push_back(W);
+    out.push_back(0);
+    return out;
 }
 
 std::vector<Plan> plan_candidates(int p, int N) {
@@ -151,9 +155,12 @@ std::vector<Plan> plan_candidates(int p, int N) {
                 if (tile > (long long)LUCAS_EMAX * T) continue;
                 // a workgroup more than twice the tile would leave most work-items idle
                 if (T > LUCAS_THREADS_SMALL && tile < T / 2) continue;
-                const int wA = wave_factor(A, C + pl.halo, T), wB = wave_factor(B, 2, T);
-                const int A1 = wA ? A / wA : A, B1 = wB ? B / wB : B;
+                const std::vector<int> wAs = wave_factors(A, C + pl.halo, T), wBs = wave_factors(B, 2, T);
+                for (size_t ia = 0; ia < wAs.size(); ++ia)
+                for (size_t ib = 0; ib < wBs.size(); ++ib)
                 for (int pref : {8, 4}) {
+                    const int wA = wAs[ia], wB = wBs[ib];
+                    const int A1 = wA ? A / wA : A, B1 = wB ? B / wB : B;
                     std::vector<int> ra = radix_plan(A1, pref), rb = radix_plan(B1, pref);
                     if (pref == 4 && ra == radix_plan(A1, 8) && rb == radix_plan(B1, 8)) continue;
                     if ((ra.empty() && A1 > 1) || (rb.empty() && B1 > 1)) continue;
@@ -190,13 +197,13 @@ std::vector<Plan> plan_candidates(int p, int N) {
 
 // Measured fastest tilings per signal length (gpulucas_amd/tune.py).
 struct TunedShape {
-    int N, A, C, threads, pref;
+    int N, A, C, threads, pref, wA, wB;
 };
 static const TunedShape kTuned[] = {
 #include "tuned_shapes.inc"
-    {0, 0, 0, 0, 0}};
+    {0, 0, 0, 0, 0, 0, 0}};
 
-std::string make_plan_exact(int p, int N, int A, int C, int threads, int pref, Plan* out) {
+std::string make_plan_exact(int p, int N, int A, int C, int threads, int pref, Plan* out, int wA, int wB) {
     Plan pl;
     std::string why = base_plan(p, N, &pl);
     if (!why.empty()) return why;
@@ -205,6 +212,8 @@ std::string make_plan_exact(int p, int N, int A, int C, int threads, int pref, P
         if (C && c.C != C) continue;
         if (threads && c.threads != threads) continue;
         if (pref && c.radix_pref != pref) continue;
+        if (wA >= 0 && c.wA != wA) continue;
+        if (wB >= 0 && c.wB != wB) continue;
         *out = c;
         return "";
     }
@@ -214,7 +223,8 @@ std::string make_plan_exact(int p, int N, int A, int C, int threads, int pref, P
 std::string make_plan(int p, int N, int forceA, int forceC, Plan* out) {
     if (!forceA && !forceC) {
         for (const TunedShape* t = kTuned; t->N; ++t)
-            if (t->N == N && make_plan_exact(p, N, t->A, t->C, t->threads, t->pref, out).empty()) return "";
+            if (t->N == N && make_plan_exact(p, N, t->A, t->C, t->threads, t->pref, out, t->wA, t->wB).empty())
+                return "";
     }
     return make_plan_exact(p, N, forceA, forceC, 0, 0, out);
 }

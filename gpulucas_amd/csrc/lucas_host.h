@@ -44,8 +44,9 @@ int stage_twiddle_count(int L, const std::vector<int>& rad);
 std::string make_plan(int p, int N, int A, int C, Plan* out);
 // Every admissible (A, C) tiling of (p, N), cheapest modelled first.
 std::vector<Plan> plan_candidates(int p, int N);
-// Plan matching a full description (threads / pref 0 = any); "" on success.
-std::string make_plan_exact(int p, int N, int A, int C, int threads, int pref, Plan* out);
+// Plan matching a full description (A, C, threads, pref 0 = any; wave
+// factors wA, wB < 0 = any); "" on success.
+std::string make_plan_exact(int p, int N, int A, int C, int threads, int pref, Plan* out, int wA = -1, int wB = -1);
 // Default N for p: smallest admissible length under the bits-per-word rule.
 int default_length(int p);
 // Bits per word allowed at length N by the default rule.

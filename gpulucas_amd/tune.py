@@ -61,12 +61,13 @@ def emit(paths):
         with open(path) as f:
             table.update(json.load(f))
     rows = sorted((int(v["N"]), int(v["A"]), int(v["C"]), int(v["threads"]), int(v.get("radix_pref", 8)),
-                   float(v["ms_per_iter"])) for v in table.values())
+                   int(v.get("wave_A", -1)), int(v.get("wave_B", -1)), float(v["ms_per_iter"]))
+                  for v in table.values())
     with open(INC, "w") as f:
         f.write("// Generated by gpulucas_amd/tune.py from GPU measurements (MI355X):\n")
-        f.write("// {N, A, C, threads, radix preference}\n")
-        for N, A, C, T, pref, ms in rows:
-            f.write(f"{{{N}, {A}, {C}, {T}, {pref}}},  // {ms * 1e3:.2f} us/iteration\n")
+        f.write("// {N, A, C, threads, radix preference, column / row wave factor}\n")
+        for N, A, C, T, pref, wa, wb, ms in rows:
+            f.write(f"{{{N}, {A}, {C}, {T}, {pref}, {wa}, {wb}}},  // {ms * 1e3:.2f} us/iteration\n")
     print(f"wrote {len(rows)} entries to {INC}")
 
 
@@ -75,7 +76,7 @@ def main(argv=None):
     ap.add_argument("--out", default=None)
     ap.add_argument("--emit", nargs="*", default=None)
     ap.add_argument("--iters", type=int, default=300)
-    ap.add_argument("--max-candidates", type=int, default=60)
+    ap.add_argument("--max-candidates", type=int, default=250)
     ap.add_argument("--lengths", default=None, help="comma list of p:N pairs")
     ap.add_argument("--no-serving", action="store_true")
     a = ap.parse_args(argv)

@@ -32,15 +32,18 @@ def lib():
             build()
         _lib = C.CDLL(LIB)
         _lib.emu_set_single.argtypes = [C.c_int]
+        _lib.emu_set_waves.argtypes = [C.c_int, C.c_int]
         _lib.emu_run.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_longlong, C.c_void_p,
                                  C.POINTER(C.c_double)]
     return _lib
 
 
-def run(p, N, digits, iters, A=0, C_=0, T=0, single=False):
+def run(p, N, digits, iters, A=0, C_=0, T=0, single=False, waves=(-1, -1)):
     """Digits after `iters` iterations of the emulated kernels; `single` uses
-    the single-job kernels (descriptor by value) instead of the batched ones."""
+    the single-job kernels (descriptor by value) instead of the batched ones;
+    `waves` forces the column / row wave factors (-1: planner's choice)."""
     lib().emu_set_single(1 if single else 0)
+    lib().emu_set_waves(*waves)
     x = np.ascontiguousarray(digits, dtype=np.int32)
     out = np.zeros(N, dtype=np.int32)
     err = C.c_double(0)

@@ -54,12 +54,17 @@ hipError_t launch_row1_kernel(int, int, const LucasJobDesc&, int, size_t, hipStr
 // workgroup id) instead of the batched ones (map + descriptor array).
 static int g_single = 0;
 extern "C" void emu_set_single(int on) { g_single = on; }
+// Wave factors the plan must use (-1: the planner's first choice).
+static int g_wA = -1, g_wB = -1;
+extern "C" void emu_set_waves(int wA, int wB) { g_wA = wA; g_wB = wB; }
 
 extern "C" int emu_run(int p, int N, int A, int C, int T, const int32_t* digits_in, long long iters,
                        int32_t* digits_out, double* maxerr) {
     using namespace lucas;
     Plan pl;
-    if (!(T ? make_plan_exact(p, N, A, C, T, 0, &pl) : make_plan(p, N, A, C, &pl)).empty()) return -1;
+    if (!(T || g_wA >= 0 || g_wB >= 0 ? make_plan_exact(p, N, A, C, T, 0, &pl, g_wA, g_wB)
+                                       : make_plan(p, N, A, C, &pl)).empty())
+        return -1;
     Tables t;
     make_tables(pl, &t);
     std::vector<double2> spec(pl.M), spec2(pl.M);

@@ -71,3 +71,22 @@ def test_emulated_single_job_kernels_match_oracle(p, N, A, C, iters, T):
     want, _ = oracle.ibdwt_iterate(p, N, x, iters)
     assert np.array_equal(got, want)
     assert err < 0.4
+
+
+@pytest.mark.parametrize("p,N,A,C,T,waves", [
+    (164729, 8192, 256, 2, 1024, (64, 0)),    # radix-4 Stockham stage, then 64-point wave factor
+    (164729, 8192, 256, 2, 1024, (128, 0)),   # radix 2, then 128
+    (164729, 8192, 256, 4, 1024, (0, 0)),     # Stockham only
+    (368647, 20480, 16, 4, 1024, (0, 0)),     # rows Stockham only (5 x 128 as radices)
+])
+def test_emulated_smaller_wave_factors_match_oracle(p, N, A, C, T, waves):
+    """Every wave factor the planner offers (not only the largest) computes
+    the oracle's digits."""
+    w, _, _, _ = oracle.tables(p, N)
+    rng = np.random.default_rng(9)
+    half = (1 << (w - 1)).astype(np.int64)
+    x = (rng.integers(0, 2 * half) - half).astype(np.int32)
+    got, err = emu.run(p, N, x, 1, A, C, T, single=True, waves=waves)
+    want, _ = oracle.ibdwt_iterate(p, N, x, 1)
+    assert np.array_equal(got, want)
+    assert err < 0.4
