@@ -61,11 +61,11 @@ void fill_info(const Plan& pl, lucas_plan_info* out) {
     out->device_bytes = job_bytes(pl);
 }
 
-int plan_for(int p, int N, int A, int C, Plan* pl) {
+int plan_for(int p, int N, int A, int C, Plan* pl, bool shared = false) {
     if (p < 7) return LUCAS_ERR_INVALID_ARG;
     if (N == 0) N = default_length(p);
     if (N <= 0) return LUCAS_ERR_INVALID_ARG;
-    return make_plan(p, N, A, C, pl).empty() ? LUCAS_OK : LUCAS_ERR_INVALID_ARG;
+    return make_plan(p, N, A, C, pl, shared).empty() ? LUCAS_OK : LUCAS_ERR_INVALID_ARG;
 }
 
 int ctx_new(const Plan& pl, int device, lucas_ctx** out) {
@@ -557,7 +557,7 @@ int lucas_batch_create(const int* ps, const int* Ns, int njobs, int device, luca
     if (!ps || njobs <= 0 || !out) return LUCAS_ERR_INVALID_ARG;
     std::vector<Plan> plans(njobs);
     for (int i = 0; i < njobs; ++i) {
-        int rc = plan_for(ps[i], Ns ? Ns[i] : 0, 0, 0, &plans[i]);
+        int rc = plan_for(ps[i], Ns ? Ns[i] : 0, 0, 0, &plans[i], true);
         if (rc != LUCAS_OK) return rc;
     }
     lucas_batch* b = new lucas_batch();
@@ -810,6 +810,8 @@ int lucas_server_start(const lucas_server_options* opt, lucas_server** out) {
 int lucas_server_submit(lucas_server* s, int p, int N, long long iterations, long long* id) {
     if (!s || iterations <= 0) return LUCAS_ERR_INVALID_ARG;
     Plan pl;
+    // a served job's latency is what counts (a handful share the device at a
+    // time): the single-job plan, not the batch one
     int rc = plan_for(p, N, 0, 0, &pl);
     if (rc != LUCAS_OK) return rc;
     if (job_bytes(pl) > s->budget) return LUCAS_ERR_BUSY;

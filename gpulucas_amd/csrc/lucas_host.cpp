@@ -220,11 +220,20 @@ std::string make_plan_exact(int p, int N, int A, int C, int threads, int pref, P
     return "no admissible (A, B, C) tiling for this N";
 }
 
-std::string make_plan(int p, int N, int forceA, int forceC, Plan* out) {
+std::string make_plan(int p, int N, int forceA, int forceC, Plan* out, bool shared) {
     if (!forceA && !forceC) {
-        for (const TunedShape* t = kTuned; t->N; ++t)
-            if (t->N == N && make_plan_exact(p, N, t->A, t->C, t->threads, t->pref, out, t->wA, t->wB).empty())
+        for (const TunedShape* t = kTuned; t->N; ++t) {
+            if (t->N != N) continue;
+            if (shared) {
+                // throughput over latency: the widest tile (least halo
+                // recomputation) at the tuned split and workgroup size
+                for (int C : {8, 4})
+                    if (C > t->C && make_plan_exact(p, N, t->A, C, t->threads, t->pref, out).empty()) return "";
+                if (make_plan_exact(p, N, t->A, t->C, t->threads, t->pref, out).empty()) return "";
+            } else if (make_plan_exact(p, N, t->A, t->C, t->threads, t->pref, out, t->wA, t->wB).empty()) {
                 return "";
+            }
+        }
     }
     return make_plan_exact(p, N, forceA, forceC, 0, 0, out);
 }

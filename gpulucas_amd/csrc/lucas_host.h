@@ -41,7 +41,11 @@ std::vector<int> radix_plan(int L, int pref = 8);
 int stage_twiddle_count(int L, const std::vector<int>& rad);
 // Full plan for (p, N) with optional forced A / C (0 = choose).  Returns an
 // empty string on success or the reason for refusal.
-std::string make_plan(int p, int N, int A, int C, Plan* out);
+// `shared`: the plan of a job in a batch, where throughput counts rather
+// than one job's latency: the tuned split and workgroup size with the widest
+// admissible tile (C up to 8, the least halo recomputation) and the largest
+// wave factors.
+std::string make_plan(int p, int N, int A, int C, Plan* out, bool shared = false);
 // Every admissible (A, C) tiling of (p, N), cheapest modelled first.
 std::vector<Plan> plan_candidates(int p, int N);
 // Plan matching a full description (A, C, threads, pref 0 = any; wave
