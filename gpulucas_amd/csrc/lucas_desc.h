@@ -87,7 +87,7 @@ struct LucasJobDesc {
 // carves its LDS in the order listed.
 //   column kernel: (C+H) x A tile; stage twiddles; per-row
 //     {2^(rhi/N), 2^(-rhi/N)/(4M)}; per local digit {2^(rlo/N), 2^(-rlo/N)};
-//     per-row rhi; per local digit rlo; roundoff slots; stage table;
+//     per-row rhi; per local digit rlo; stage table;
 //     omega_A^t, t < A, and the wave-factor table (with a wave factor only);
 //     int8 carries of the own elements, A x C
 //   row kernel: two rows; stage twiddles; omega_B^t, t < B; stage table;
@@ -100,7 +100,7 @@ __host__ __device__ inline int lucas_pad_words(int n) { return (n + 15) & ~15; }
 __host__ __device__ constexpr int lucas_wave_table_words(int P) { return P ? 32 * P + 64 : 0; }
 __host__ __device__ inline int lucas_col_lds_words(int A, int C, int H, int ntws, int nst, int pA) {
     const int ncl = C + H;
-    return lucas_pad_words(A * ncl) + ntws + A + 2 * ncl + (A + 3) / 4 + (2 * ncl + 3) / 4 + 4 + 2 * nst +
+    return lucas_pad_words(A * ncl) + ntws + A + 2 * ncl + (A + 3) / 4 + (2 * ncl + 3) / 4 + 2 * nst +
            (pA ? A : 0) + lucas_wave_table_words(pA) + (A * C + 15) / 16;
 }
 __host__ __device__ inline int lucas_row_lds_words(int B, int ntws, int nst, int pB) {

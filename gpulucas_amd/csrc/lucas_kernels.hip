@@ -484,24 +484,24 @@ __device__ __forceinline__ double2 outer_twiddle(const LucasJobDesc& J, const do
     return cmul(gload2(twA, hi), gload2(twMB, lo));
 }
 
-// Workgroup max of the roundoff: wave64 shuffle reduction, one LDS slot per
-// wave, then a single atomic per workgroup, issued only when the value beats
-// `seen` (the job's max as read at kernel start).  One atomic per wave on a
-// single address serialises across the chip once every wave has a nonzero
-// roundoff.
-template <int T>
-__device__ __forceinline__ void block_max_err(float err, float* red, unsigned int seen, unsigned int* dst) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) err = fmaxf(err, __shfl_xor(err, off, 64));
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = err;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float m = red[0];
-#pragma unroll
-        for (int w = 1; w < T / 64; ++w) m = fmaxf(m, red[w]);
-        const unsigned int bits = __float_as_uint(m);
-        if (bits > seen) atomicMax(dst, bits);
-    }
+// Wave max of the roundoff, folded into the job's maximum.  err >= 0, so
+// float order is the order of the bit patterns: DPP quad permutes and row
+// rotates take the max within each 16-lane row (no LDS crossbar), four lane
+// reads combine the rows.  One atomic per wave, issued only when the wave
+// beats `seen` (the job's max as read at kernel start): after the first
+// iterations a new maximum is rare, so there is no barrier and almost no
+// atomic traffic at the end of the kernel.
+__device__ __forceinline__ void wave_max_err(float err, unsigned int seen, unsigned int* dst) {
+    unsigned int x = __float_as_uint(err);
+    x = max(x, (unsigned int)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, false));   // lane ^ 1
+    x = max(x, (unsigned int)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, false));   // lane ^ 2
+    x = max(x, (unsigned int)__builtin_amdgcn_update_dpp(0, (int)x, 0x124, 0xF, 0xF, false));  // row rotate 4
+    x = max(x, (unsigned int)__builtin_amdgcn_update_dpp(0, (int)x, 0x128, 0xF, 0xF, false));  // row rotate 8
+    const unsigned int m = max(max((unsigned int)__builtin_amdgcn_readlane((int)x, 0),
+                                   (unsigned int)__builtin_amdgcn_readlane((int)x, 16)),
+                               max((unsigned int)__builtin_amdgcn_readlane((int)x, 32),
+                                   (unsigned int)__builtin_amdgcn_readlane((int)x, 48)));
+    if ((threadIdx.x & 63) == 0 && m > seen) atomicMax(dst, m);
 }
 
 // Carry-save restoration of one digit (reference llintToIrrBal<n>,
@@ -605,8 +605,6 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc& J, int tile,
     p += (A + 3) / 4;
     int* const colr = reinterpret_cast<int*>(p);  // rlo per local digit
     p += (2 * ncl + 3) / 4;
-    float* const red = reinterpret_cast<float*>(p);  // per-wave roundoff slots
-    p += 4;
     LucasStage* const lst = reinterpret_cast<LucasStage*>(p);  // column transform stages
     p += 2 * J.nradA;
     double2* const twL = p;  // omega_A^t, t < A (wave factor only)
@@ -635,7 +633,9 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc& J, int tile,
     constexpr int wA = 64 * PA;
     const int A1 = PA ? A / wA : A;
     if (PA) {
-        for (int t = tid; t < A; t += T) twL[t] = gload2(twA, t);
+        // omega_A is read only by the wave factor's outer twiddles (A1 > 1)
+        if (A1 > 1)
+            for (int t = tid; t < A; t += T) twL[t] = gload2(twA, t);
         fill_wave_table<T, PA>(twW, twA, A1, tid);
     }
     for (int t = tid; t < A; t += T) {
@@ -732,7 +732,7 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc& J, int tile,
             lds[pidx(r * ncl + H + lo)] = weighted(r, lo, x);
         }
         if (mode & LUCAS_OUT_DIGITS) {
-            block_max_err<T>(err, red, seen, J.maxerr);
+            wave_max_err(err, seen, J.maxerr);
             return;
         }
     }
@@ -749,7 +749,7 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc& J, int tile,
         const int k = udiv(t, C, mC), lo = t - k * C;
         gstore2(J.spec, (size_t)k * B + c0 + lo, lds[pidx(k * ncl + H + lo)]);
     }
-    block_max_err<T>(err, red, seen, J.maxerr);
+    wave_max_err(err, seen, J.maxerr);
 }
 
 // ---------------------------------------------------------------------------

@@ -4,6 +4,7 @@
 // shared buffer.  It exists so kernel indexing can be checked against the
 // oracle without a GPU; the product is never built against it.
 #pragma once
+#include <algorithm>
 #include <barrier>
 #include <cmath>
 #include <cstring>
@@ -78,6 +79,8 @@ inline int __builtin_amdgcn_ds_swizzle(int src, int pattern) {
     return emu::wave_read(src, (l & 32) | ((((l & 31) & am) | om) ^ xm));
 }
 inline int __builtin_amdgcn_ds_bpermute(int addr, int src) { return emu::wave_read(src, (addr >> 2) & 63); }
+inline int __builtin_amdgcn_readlane(int x, int lane) { return emu::wave_read(x, lane & 63); }
+using std::max;
 inline int __double2loint(double d) { unsigned long long u; std::memcpy(&u, &d, 8); return (int)(u & 0xffffffffu); }
 inline int __double2hiint(double d) { unsigned long long u; std::memcpy(&u, &d, 8); return (int)(u >> 32); }
 inline double __hiloint2double(int hi, int lo) {

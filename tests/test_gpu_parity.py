@@ -86,7 +86,9 @@ def test_digits_bit_exact_vs_oracle_steady_state(p, N, A, C):
         ctx.step(3)
         y, _ = oracle.ibdwt_iterate(p, N, x, 3)
         assert np.array_equal(ctx.digits(), y)
-        assert ctx.max_roundoff() < ROUNDOFF_LIMIT
+        # random full-size digits always leave some FP64 rounding error: a
+        # zero would mean the device-side maximum lost its inputs
+        assert 0.0 < ctx.max_roundoff() < ROUNDOFF_LIMIT
 
 
 @pytest.mark.parametrize("p,N", [(1279, 64), (2203, 120), (4423, 240), (9941, 560), (19937, 1080),
