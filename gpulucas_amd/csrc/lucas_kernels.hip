@@ -423,17 +423,17 @@ __device__ __forceinline__ void wave_dif(double2 (&v)[P], const double2* twW, in
 // omega_L^(n2*k1), runs the W-point DIF in one wave and writes X[k1 + L1*k2]
 // in natural order.  Wave w takes sub-transform w (the planner keeps their
 // number within the workgroup's waves); all reads precede all writes.
-template <int T, int P>
+template <int T, int P, typename Post>
 __device__ __forceinline__ void fft_wave(double2* lds, const SeqGeom& g, int L1, unsigned int mL1,
-                                         const double2* twL, const double2* twW, int tid) {
+                                         const double2* twL, const double2* twW, int tid, Post post) {
     constexpr int W = 64 * P;
     const int lane = tid & 63, sub = tid >> 6;
     const bool active = sub < g.nsq * L1;  // wave-uniform
     double2 v[P];
-    int ob = 0;
+    int ob = 0, q = 0, k1 = 0;
     if (active) {
-        const int q = udiv(sub, L1, mL1);
-        const int k1 = sub - q * L1;
+        q = udiv(sub, L1, mL1);
+        k1 = sub - q * L1;
         const int base = seq_base(g, q);
 #pragma unroll
         for (int m = 0; m < P; ++m) {
@@ -448,21 +448,31 @@ __device__ __forceinline__ void fft_wave(double2* lds, const SeqGeom& g, int L1,
     if (active) {
         const int kb = (int)(__brev((unsigned int)lane) >> 26) * P;
 #pragma unroll
-        for (int m = 0; m < P; ++m) lds[pidx(ob + L1 * (kb + brev_slot<P>(m)) * g.sIdx)] = v[m];
+        for (int m = 0; m < P; ++m) {
+            const int k = L1 * (kb + brev_slot<P>(m));  // output element k + k1 of sequence q
+            lds[pidx(ob + k * g.sIdx)] = post(q, k + k1, v[m]);
+        }
     }
     __syncthreads();
 }
 
+// Identity post-operation of a transform's outputs.
+struct PostNone {
+    __device__ __forceinline__ double2 operator()(int, int, double2 v) const { return v; }
+};
+
 // Forward transform of length L of the sequences of g: the length-L1
 // Stockham passes (stages, twiddles tws) over the sub-sequences of gA, then,
-// when P > 0, the W = 64P wave-register factor.  P = 0: Stockham only over g.
-template <int T, int P>
+// when P > 0, the W = 64P wave-register factor, whose outputs pass through
+// post(sequence, element, value) on their way to LDS.  P = 0: Stockham only
+// over g (post is not applied).
+template <int T, int P, typename Post = PostNone>
 __device__ __forceinline__ void fft_seq(double2* lds, const LucasStage* stages, int nst, const double2* tws,
                                         const SeqGeom& g, const SeqGeom& gA, int L1, unsigned int mL1,
-                                        const double2* twL, const double2* twW, int tid) {
+                                        const double2* twL, const double2* twW, int tid, Post post = Post()) {
     if constexpr (P > 0) {
         fft_lds<T, -1, true>(lds, stages, nst, gA, tws, tid);
-        fft_wave<T, P>(lds, g, L1, mL1, twL, twW, tid);
+        fft_wave<T, P>(lds, g, L1, mL1, twL, twW, tid, post);
     } else {
         fft_lds<T, -1, true>(lds, stages, nst, g, tws, tid);
     }
@@ -671,13 +681,13 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc& J, int tile,
         // inverse column FFT of the tile and halo
         const SeqGeom g{ncl, mncl, 0, 0u, 1, 0, 0, ncl};
         const SeqGeom gA{ncl * wA, J.m_sqA0, 0, 0u, 1, 0, 0, wA * ncl};
-        fft_seq<T, PA>(lds, lst, J.nradA, tws, g, gA, A1, J.m_A1, twL, twW, tid);
-        // unweight, -2 on digit 0, round half-even, roundoff
-        for (int t = tid; t < A * ncl; t += T) {
-            const int r = udiv(t, ncl, mncl), lc = t - r * ncl;
+        // unweight, -2 on digit 0, round half-even, roundoff of element
+        // (r, lc); the rounded products go back as int64 bit patterns (exact:
+        // |c| < 2^53 whenever the roundoff is meaningful)
+        auto unweight = [&](int lc, int r, double2 v) -> double2 {
             int gc = c0 - H + lc;
             if (gc < 0) gc += B;
-            const double2 v = cconj(lds[pidx(t)]);
+            v = cconj(v);
             const int rh = rowr[r];
             const double iwh = roww[r].y;
             const int e = 2 * lc;
@@ -688,10 +698,19 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc& J, int tile,
             if (gc == 0 && r == 0) ve -= 2.0;
             const double re = rint(ve), ro = rint(vo);
             err = fmaxf(err, (float)fmax(fabs(ve - re), fabs(vo - ro)));
-            // |c| < 2^53 whenever the roundoff is meaningful: exact int64
-            lds[pidx(t)] = make_double2(__longlong_as_double((long long)re), __longlong_as_double((long long)ro));
+            return make_double2(__longlong_as_double((long long)re), __longlong_as_double((long long)ro));
+        };
+        if constexpr (PA > 0) {
+            // applied to the wave factor's outputs as they are written
+            fft_seq<T, PA>(lds, lst, J.nradA, tws, g, gA, A1, J.m_A1, twL, twW, tid, unweight);
+        } else {
+            fft_seq<T, PA>(lds, lst, J.nradA, tws, g, gA, A1, J.m_A1, twL, twW, tid);
+            for (int t = tid; t < A * ncl; t += T) {
+                const int r = udiv(t, ncl, mncl), lc = t - r * ncl;
+                lds[pidx(t)] = unweight(lc, r, lds[pidx(t)]);
+            }
+            __syncthreads();
         }
-        __syncthreads();
         // Carry-save restoration + single ripple x_j = d_j + h_{j-1}
         // (loadIntToDoubleIBDWT, gpuLucas.cu:730-766), one work-item per own
         // element, rows in order.  T is a multiple of C, so a pass holds whole
