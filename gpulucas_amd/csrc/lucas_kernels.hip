@@ -529,8 +529,10 @@ __device__ __forceinline__ int restore_digit(const unsigned long long* mag, cons
 #pragma unroll
     for (int i = 0; i <= NC; ++i) {
         if (i > 0) shift += w[i];
-        // fields at or above bit 64 are zero (the reference's shift is undefined there)
-        const int part = shift < 64 ? (int)((unsigned int)(mag[i] >> shift) & mask) : 0;
+        // Fields at or above bit 64 are zero (the reference's shift is
+        // undefined there).  Depths 2 and 3 run at q >= 16 (q <= 20), so their
+        // shifts stay below 3*21 = 63 and need no guard.
+        const int part = (NC <= 3 || shift < 64) ? (int)((unsigned int)(mag[i] >> shift) & mask) : 0;
         sum += neg[i] ? -part : part;
     }
     const int half = 1 << (W - 1);
@@ -850,6 +852,9 @@ __device__ __forceinline__ void row_kernel_body(const LucasJobDesc& J, int r0) {
         gstore2(J.spec2, (size_t)(rl ? r1 : r0) * B + kc, cconj(lds[pidx(t)]));
     }
 }
+
+// The single-job kernels take the descriptor as a kernel argument (4 KiB limit).
+static_assert(sizeof(LucasJobDesc) <= 4096, "LucasJobDesc must fit the kernel argument segment");
 
 // Column tile of workgroup `wg` of a single-job launch: the tiles are dealt
 // to the 8 XCDs (workgroup 8k + x runs tile k of XCD x's contiguous range)
