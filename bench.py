@@ -4,8 +4,10 @@
 One step = one LL iteration s <- s^2 - 2 mod 2^p - 1 of the headline
 workload: p = 2976221, N = 163840 = 2^15*5 (FP64 IBDWT, state resident in
 HBM).  Prints one JSON line with ms per iteration, the roofline of the
-dominant kernel, the batched-vs-plain throughput ratio and the GMP CPU
-baseline timed on this host.
+dominant kernel (with its HBM traffic from the committed rocprofv3 counter
+summary), the per-iteration roofline of a 2^21-point transform
+(p = 37156667), the batched-vs-plain throughput ratios, the serving
+p50/p99 latency and the GMP CPU baseline timed on this host.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
