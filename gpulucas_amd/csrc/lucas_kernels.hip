@@ -545,17 +545,24 @@ __device__ __forceinline__ int restore_digit(const unsigned long long* mag, cons
     return sum - h * (1 << W);
 }
 
+// A rounded convolution output c_j as the carry restoration reads it: sign in
+// bit 63, bit 62 set iff digit j is wide (q+1 bits), |c_j| below (|c| < 2^53
+// whenever the roundoff is meaningful).
+__device__ __forceinline__ double pack_product(double c, bool wide) {
+    const unsigned long long m = (unsigned long long)fabs(c);
+    const unsigned long long bits = m | ((unsigned long long)wide << 62) | (c < 0.0 ? (1ull << 63) : 0ull);
+    return __longlong_as_double((long long)bits);
+}
+
 // Carry-save restoration for element m = (row r, local column lc) of the
 // tile: digits 2m+1 and 2m, and with `first` also digit 2m-1 (carry only).
-// The products of digits 2m+1-k, k = 0 .. NC+2, are the rounded convolution
-// outputs stored as int64 bit patterns in the tile, read top-down; a local
-// column left of global column 0 is the previous row's halo (the cyclic wrap
-// of IrrBaseBalanced.cu:64-70).  r2m = r_{2m}; widths follow from
-// r_{j-1} = r_j + b (mod N).  Returns (d_2m, d_2m+1 + h_2m) and the carries
-// h_2m+1 and h_2m-1.
+// The products of digits 2m+1-k, k = 0 .. NC+2 (pack_product words), are read
+// top-down; a local column left of global column 0 is the previous row's halo
+// (the cyclic wrap of IrrBaseBalanced.cu:64-70).  Returns (d_2m, d_2m+1 + h_2m)
+// and the carries h_2m+1 and h_2m-1.
 template <int NC>
 __device__ __forceinline__ int2 restore_element(const double2* lds, const LucasJobDesc& J, int r, int lc, int c0mH,
-                                                int ncl, int r2m, bool first, int* h1, int* hm1) {
+                                                int ncl, bool first, int* h1, int* hm1) {
     unsigned long long mag[NC + 3];
     bool neg[NC + 3];
     int w[NC + 3];
@@ -564,22 +571,10 @@ __device__ __forceinline__ int2 restore_element(const double2* lds, const LucasJ
         const int ie = k >> 1;  // element offset back
         const int rr = (c0mH + lc - ie < 0) ? (r == 0 ? J.A - 1 : r - 1) : r;
         const double2 v = lds[pidx(rr * ncl + (lc - ie))];
-        const long long c = __double_as_longlong((k & 1) ? v.x : v.y);
-        neg[k] = c < 0;
-        mag[k] = c < 0 ? (unsigned long long)(-c) : (unsigned long long)c;
-    }
-    const int N = J.N, bb = J.b;
-    int rv = r2m - bb;
-    if (rv < 0) rv += N;
-    w[0] = J.q + (rv < bb ? 1 : 0);  // digit 2m+1
-    rv = r2m;
-#pragma unroll
-    for (int k = 1; k < NC + 3; ++k) {
-        if (k > 1) {
-            rv += bb;
-            if (rv >= N) rv -= N;
-        }
-        w[k] = J.q + (rv < bb ? 1 : 0);  // digit 2m+1-k
+        const unsigned long long bits = (unsigned long long)__double_as_longlong((k & 1) ? v.x : v.y);
+        neg[k] = (bits >> 63) != 0;
+        mag[k] = bits & ((1ull << 62) - 1);
+        w[k] = J.q + (int)((bits >> 62) & 1);  // width of digit 2m+1-k
     }
     int h0;
     const int d1 = restore_digit<NC>(mag, neg, w, h1);               // digit 2m+1
@@ -684,8 +679,7 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc& J, int tile,
         const SeqGeom g{ncl, mncl, 0, 0u, 1, 0, 0, ncl};
         const SeqGeom gA{ncl * wA, J.m_sqA0, 0, 0u, 1, 0, 0, wA * ncl};
         // unweight, -2 on digit 0, round half-even, roundoff of element
-        // (r, lc); the rounded products go back as int64 bit patterns (exact:
-        // |c| < 2^53 whenever the roundoff is meaningful)
+        // (r, lc); the rounded products go back as pack_product words
         auto unweight = [&](int lc, int r, double2 v) -> double2 {
             int gc = c0 - H + lc;
             if (gc < 0) gc += B;
@@ -693,14 +687,17 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc& J, int tile,
             const int rh = rowr[r];
             const double iwh = roww[r].y;
             const int e = 2 * lc;
-            const double ie = colw[e].y * iwh * (colr[e] + rh >= N ? 2.0 : 1.0);
-            const double io = colw[e + 1].y * iwh * (colr[e + 1] + rh >= N ? 2.0 : 1.0);
+            // r_j = rlo + rhi (mod N): the wrap halves the weight; wide iff r_j < b
+            const int rje = colr[e] + rh, rjo = colr[e + 1] + rh;
+            const double ie = colw[e].y * iwh * (rje >= N ? 2.0 : 1.0);
+            const double io = colw[e + 1].y * iwh * (rjo >= N ? 2.0 : 1.0);
             double ve = v.x * ie;
             const double vo = v.y * io;
             if (gc == 0 && r == 0) ve -= 2.0;
             const double re = rint(ve), ro = rint(vo);
             err = fmaxf(err, (float)fmax(fabs(ve - re), fabs(vo - ro)));
-            return make_double2(__longlong_as_double((long long)re), __longlong_as_double((long long)ro));
+            return make_double2(pack_product(re, (rje >= N ? rje - N : rje) < J.b),
+                                pack_product(ro, (rjo >= N ? rjo - N : rjo) < J.b));
         };
         if constexpr (PA > 0) {
             // applied to the wave factor's outputs as they are written
@@ -732,14 +729,12 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc& J, int tile,
             int h1 = 0, hm1 = 0;
             if (own) {
                 const int lc = lo + H;
-                int r2m = colr[2 * lc] + rowr[r];
-                if (r2m >= N) r2m -= N;
                 switch (n) {
-                    case 2: x = restore_element<2>(lds, J, r, lc, c0mH, ncl, r2m, lo == 0, &h1, &hm1); break;
-                    case 3: x = restore_element<3>(lds, J, r, lc, c0mH, ncl, r2m, lo == 0, &h1, &hm1); break;
-                    case 4: x = restore_element<4>(lds, J, r, lc, c0mH, ncl, r2m, lo == 0, &h1, &hm1); break;
-                    case 5: x = restore_element<5>(lds, J, r, lc, c0mH, ncl, r2m, lo == 0, &h1, &hm1); break;
-                    default: x = restore_element<6>(lds, J, r, lc, c0mH, ncl, r2m, lo == 0, &h1, &hm1); break;
+                    case 2: x = restore_element<2>(lds, J, r, lc, c0mH, ncl, lo == 0, &h1, &hm1); break;
+                    case 3: x = restore_element<3>(lds, J, r, lc, c0mH, ncl, lo == 0, &h1, &hm1); break;
+                    case 4: x = restore_element<4>(lds, J, r, lc, c0mH, ncl, lo == 0, &h1, &hm1); break;
+                    case 5: x = restore_element<5>(lds, J, r, lc, c0mH, ncl, lo == 0, &h1, &hm1); break;
+                    default: x = restore_element<6>(lds, J, r, lc, c0mH, ncl, lo == 0, &h1, &hm1); break;
                 }
                 hb[t] = (signed char)h1;
             }
