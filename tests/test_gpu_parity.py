@@ -77,6 +77,28 @@ def test_wave_factor_shapes_bit_exact_vs_oracle(p, N, A, C, T):
         assert ctx.max_roundoff() < ROUNDOFF_LIMIT
 
 
+def _tuned_pairs(max_n):
+    from gpulucas_amd import tune
+    return [(p, N) for p, N in tune.standard_lengths() if N <= max_n]
+
+
+@pytest.mark.parametrize("p,N", _tuned_pairs(294912))
+def test_tuned_shapes_bit_exact_vs_oracle(p, N):
+    """Every measured shape of the planner's table up to N = 294912 (the
+    batch and serving lengths, with every wave-factor choice the tuner made)
+    matches the oracle's digits from random steady-state digits."""
+    info = lucas.plan(p, N)
+    print(f"case p={p} N={N} A={info['A']} C={info['C']} T={info['threads']} "
+          f"W={info['wave_A']}/{info['wave_B']}", flush=True)
+    x = random_balanced(p, N, 13)
+    with lucas.Context(p, N) as ctx:
+        ctx.set_digits(x, 0)
+        ctx.step(1)
+        y, _ = oracle.ibdwt_iterate(p, N, x, 1)
+        assert np.array_equal(ctx.digits(), y)
+        assert 0.0 < ctx.max_roundoff() < ROUNDOFF_LIMIT
+
+
 @pytest.mark.parametrize("p,N,A,C", SHAPES[::2])
 def test_digits_bit_exact_vs_oracle_steady_state(p, N, A, C):
     print(f"case p={p} N={N} A={A} C={C}", flush=True)
