@@ -7,7 +7,7 @@ HBM).  Prints one JSON line with ms per iteration, the roofline of the
 dominant kernel (with its HBM traffic from the committed rocprofv3 counter
 summary), the per-iteration roofline of a 2^21-point transform
 (p = 37156667), the batched-vs-plain throughput ratios, the serving
-p50/p99 latency and the GMP CPU baseline timed on this host.
+p50/p99 latency and capacity, and the GMP CPU baseline timed on this host.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -168,6 +168,23 @@ def serving(lucas, wl, count, k_scale, load):
             "all_ok": all(r["status"] == 0 for r in res)}
 
 
+def serving_capacity(lucas, wl, plain_its, iters=2000, count=64, seed=6, lo=100_000, hi=1_200_000):
+    """Aggregate iterations/s of the continuous-batching server when `count`
+    jobs of N <= 64K arrive at once (the same exponents as `batched_small`),
+    against `plain_its`, the rate of running them one after another."""
+    ps = wl.batch_primes(seed=seed, count=count, lo=lo, hi=hi)
+    with lucas.Server(chunk_iters=256) as srv:
+        ids = [srv.submit(p, iters) for p in ps]
+        srv.drain(timeout_s=600)
+        res = [srv.poll(j) for j in ids]
+    span = max(r["finish_s"] for r in res) - min(r["submit_s"] for r in res)
+    its = count * iters / span
+    return {"workload": f"{count} jobs of {iters} iterations submitted at once, primes in [1e5,1.2e6] "
+                        f"(seed {seed}), N <= 64K",
+            "iters_per_s": round(its, 1), "plain_iters_per_s": round(plain_its, 1),
+            "ratio": round(its / plain_its, 3), "all_ok": all(r["status"] == 0 for r in res)}
+
+
 def cpu_baseline(seconds_target):
     import oracle
 
@@ -266,6 +283,8 @@ def main():
         out["batched_small"] = {"workload": "64 primes in [1e5,1.2e6] (seed 6), default N each (N <= 64K)",
                                 "iters_per_s": round(bi, 1), "plain_iters_per_s": round(pi, 1),
                                 "ratio": round(bi / pi, 3)}
+        if not a.no_serve:
+            out["serving_capacity"] = serving_capacity(lucas, wl, pi)
     if not a.no_serve:
         out["serving"] = serving(lucas, wl, a.serve_jobs, 0.1, 0.8)
     if not a.no_cpu:

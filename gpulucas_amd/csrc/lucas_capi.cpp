@@ -807,12 +807,21 @@ int lucas_server_start(const lucas_server_options* opt, lucas_server** out) {
     return LUCAS_OK;
 }
 
+static const long long kServerBusyJobs = 16;
+
 int lucas_server_submit(lucas_server* s, int p, int N, long long iterations, long long* id) {
     if (!s || iterations <= 0) return LUCAS_ERR_INVALID_ARG;
     Plan pl;
-    // a served job's latency is what counts (a handful share the device at a
-    // time): the single-job plan, not the batch one
-    int rc = plan_for(p, N, 0, 0, &pl);
+    // Lightly loaded, a served job's latency is what counts: the single-job
+    // plan.  With kServerBusyJobs or more jobs already waiting or running the
+    // device is shared anyway, and the batch plan's wider tiles (less halo
+    // recomputation) raise the aggregate rate.
+    long long outstanding = 0;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        outstanding = s->outstanding;
+    }
+    int rc = plan_for(p, N, 0, 0, &pl, outstanding >= kServerBusyJobs);
     if (rc != LUCAS_OK) return rc;
     if (job_bytes(pl) > s->budget) return LUCAS_ERR_BUSY;
     std::lock_guard<std::mutex> lk(s->mu);

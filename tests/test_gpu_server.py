@@ -34,6 +34,26 @@ def test_server_full_runs_match_gmp(gmp_fixtures):
         assert st["peak_jobs"] <= 8 and st["iterations"] >= sum(p - 2 for p in ids.values())
 
 
+def test_busy_server_batch_plans_match_gmp(gmp_fixtures):
+    """More than 16 jobs outstanding: later submissions take the batch plan
+    (wider tiles); every result still equals GMP."""
+    primes = [p for p in wl.MERSENNE_PRIME_EXPONENTS if 1279 <= p <= 132049]
+    jobs = primes + SMALL_COMPOSITES
+    assert len(jobs) > 16
+    with lucas.Server(chunk_iters=256) as s:
+        ids = {s.submit(p, p - 2): p for p in jobs}
+        s.drain(timeout_s=300)
+        for jid, p in ids.items():
+            r = s.poll(jid)
+            assert r["done"] == 1 and r["status"] == lucas.OK, p
+            if p in primes:
+                assert r["is_zero"] == 1 and r["res64"] == 0, p
+            else:
+                want, _ = fixture_res64(gmp_fixtures, "small_composite_full", p, p - 2)
+                assert r["res64"] == want, p
+            assert r["max_roundoff"] < 0.4
+
+
 def test_server_admission_by_footprint():
     ps = [86243, 110503, 132049, 216091]
     sizes = sorted(lucas.plan(p)["device_bytes"] for p in ps)
