@@ -79,7 +79,7 @@ def test_wave_factor_shapes_bit_exact_vs_oracle(p, N, A, C, T):
 
 def _tuned_pairs(max_n):
     from gpulucas_amd import tune
-    return [(p, N) for p, N in tune.standard_lengths() if N <= max_n]
+    return sorted((row[7], row[0]) for row in tune.tuned_entries() if row[0] <= max_n)
 
 
 @pytest.mark.parametrize("p,N", _tuned_pairs(294912))

@@ -127,27 +127,14 @@ def test_miller_rabin_against_sieve():
     assert [i for i in range(n) if wl.is_prime(i)] == list(np.nonzero(sieve)[0])
 
 
-def _tuned_table():
-    """Entries of the measured shape table: (N, A, C, threads, pref, wA, wB)."""
-    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
-                        "gpulucas_amd", "csrc", "tuned_shapes.inc")
-    rows = []
-    for line in open(path):
-        m = re.match(r"\{(-?\d+), (-?\d+), (-?\d+), (-?\d+), (-?\d+), (-?\d+), (-?\d+)\}", line)
-        if m:
-            rows.append(tuple(int(x) for x in m.groups()))
-    return rows
-
-
 def test_tuned_table_entries_are_the_plans_chosen():
     """Every measured shape is admissible and is what the planner returns for
-    the standard workloads' exponents at that length."""
+    the exponent it was measured at."""
     from gpulucas_amd import tune
-    table = {row[0]: row for row in _tuned_table()}
+    table = tune.tuned_entries()
     assert len(table) >= 40
-    pairs = dict((N, p) for p, N in tune.standard_lengths())
-    assert set(table) <= set(pairs)
-    for N, (_, A, C, T, pref, wA, wB) in table.items():
-        pl = lucas.plan(pairs[N], N)
+    assert len({row[0] for row in table}) == len(table)
+    for N, A, C, T, pref, wA, wB, p in table:
+        pl = lucas.plan(p, N)
         assert (pl["A"], pl["C"], pl["threads"], pl["radix_pref"]) == (A, C, T, pref), N
         assert (pl["wave_A"], pl["wave_B"]) == (wA, wB), N
