@@ -16,6 +16,8 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(PKG, "libgpulucas.so")
+# the same library with the kernels' phase stamps compiled in (profiling only)
+PHASES_LIB = os.path.join(PKG, "libgpulucas_phases.so")
 CLI = os.path.join(PKG, "gpulucas")
 
 SOURCES = ["lucas_kernels.hip", "lucas_host.cpp", "lucas_engine.cpp", "lucas_capi.cpp"]
@@ -50,10 +52,17 @@ def _run(cmd, verbose):
 
 
 def build_library(verbose: bool = False, jobs: int = 4) -> str:
-    """Compile every source (in parallel) and link libgpulucas.so."""
+    """Compile every source (in parallel) and link libgpulucas.so, and
+    libgpulucas_phases.so from the same host objects and the kernels built
+    with LUCAS_PHASE_STAMPS."""
     objdir = os.path.join(PKG, "_build")
     os.makedirs(objdir, exist_ok=True)
     objs, procs = [], []
+    phases_obj = os.path.join(objdir, "lucas_kernels_phases.o")
+    cmd = [HIPCC] + HIP_FLAGS + ["-DLUCAS_PHASE_STAMPS", "-c", os.path.join(CSRC, "lucas_kernels.hip"), "-o", phases_obj]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
     for src in SOURCES:
         obj = os.path.join(objdir, src.rsplit(".", 1)[0] + ".o")
         objs.append(obj)
@@ -79,6 +88,10 @@ def build_library(verbose: bool = False, jobs: int = 4) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"], verbose)
     os.replace(tmp, LIB)
+    tmp = PHASES_LIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, phases_obj] + objs[1:] + ["-lpthread"],
+         verbose)
+    os.replace(tmp, PHASES_LIB)
     build_cli(verbose)
     return LIB
 

@@ -179,6 +179,7 @@ const char* lucas_status_string(int status) {
         case LUCAS_ERR_IO: return "checkpoint I/O error";
         case LUCAS_ERR_NO_DEVICE: return "no usable GPU";
         case LUCAS_ERR_BUSY: return "busy";
+        case LUCAS_ERR_UNSUPPORTED: return "not supported by this build";
         default: return "unknown status";
     }
 }
@@ -346,6 +347,12 @@ int lucas_ctx_max_roundoff(lucas_ctx* ctx, int reset, double* out) {
 int lucas_ctx_time_kernels(lucas_ctx* ctx, int iters, double* col_us, double* row_us) {
     if (!ctx || !col_us || !row_us) return LUCAS_ERR_INVALID_ARG;
     return ctx->engine->time_kernels(iters, col_us, row_us);
+}
+
+int lucas_ctx_phase_stamps(lucas_ctx* ctx, int iters, unsigned long long* col_stamps,
+                           unsigned long long* row_stamps, int max_workgroups, int* n_col, int* n_row) {
+    if (!ctx || !col_stamps || !row_stamps || !n_col || !n_row || max_workgroups <= 0) return LUCAS_ERR_INVALID_ARG;
+    return ctx->engine->phase_stamps(iters, col_stamps, row_stamps, max_workgroups, n_col, n_row);
 }
 
 int lucas_ctx_time_iterations(lucas_ctx* ctx, long long iters, double* ms_per_iter) {

@@ -28,6 +28,8 @@ hipError_t launch_col1_kernel(int threads, int p, int dense, const LucasJobDesc&
                               hipStream_t stream);
 hipError_t launch_row1_kernel(int threads, int p, const LucasJobDesc& job, int nwg, size_t lds, hipStream_t stream);
 hipError_t configure_kernels(size_t max_lds);
+bool phase_stamps_built();
+hipError_t read_phase_stamps(unsigned long long* col, int ncol, unsigned long long* row, int nrow);
 
 static const int kGraphIters = 32;
 
@@ -398,6 +400,30 @@ int Engine::time_kernels(int iters, double* col_us, double* row_us) {
     if (rc != LUCAS_OK) return rc;
     *row_us = 1000.0 * sum_row / iters;
     *col_us = 1000.0 * sum_col / iters;
+    return LUCAS_OK;
+}
+
+int Engine::phase_stamps(int iters, unsigned long long* col, unsigned long long* row, int max_wgs, int* ncol,
+                         int* nrow) {
+    if (!phase_stamps_built()) return LUCAS_ERR_UNSUPPORTED;
+    if (iters <= 0 || iters > 4096 || jobs_.size() != 1) return LUCAS_ERR_INVALID_ARG;
+    hipSetDevice(device_);
+    int rc = prepare();
+    if (rc != LUCAS_OK) return rc;
+    int nc = 0, nr = 0;
+    for (const Group& g : colg_) nc += g.n;
+    for (const Group& g : rowg_) nr += g.n;
+    if (nc > max_wgs || nr > max_wgs) return LUCAS_ERR_INVALID_ARG;
+    rc = launch_col(LUCAS_IN_DIGITS);  // valid spectra; digits untouched
+    for (int i = 0; i < iters && rc == LUCAS_OK; ++i) {
+        rc = launch_row();
+        if (rc == LUCAS_OK) rc = launch_col(0);
+    }
+    if (rc == LUCAS_OK && hipStreamSynchronize(stream_) != hipSuccess) rc = LUCAS_ERR_DEVICE;
+    if (rc == LUCAS_OK && read_phase_stamps(col, nc, row, nr) != hipSuccess) rc = LUCAS_ERR_DEVICE;
+    if (rc != LUCAS_OK) return rc;
+    *ncol = nc;
+    *nrow = nr;
     return LUCAS_OK;
 }
 

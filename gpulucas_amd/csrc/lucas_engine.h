@@ -47,6 +47,11 @@ public:
     // Mean microseconds of each kernel launch over `iters` iterations in the
     // step() order, from events after every launch (digits are not touched).
     int time_kernels(int iters, double* col_us, double* row_us);
+    // One job only, profiling library only: runs `iters` iterations in the
+    // step() order (digits are not touched) and copies the phase stamps of
+    // the last column and row launch (8 words per workgroup).
+    int phase_stamps(int iters, unsigned long long* col, unsigned long long* row, int max_wgs, int* ncol,
+                     int* nrow);
     int sync();
 
 private:

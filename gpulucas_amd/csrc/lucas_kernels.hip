@@ -53,6 +53,35 @@ __device__ __forceinline__ void gstore2i(int* p, size_t i, int2 x) {
     ((LUCAS_GLOBAL lucas_v2i*)p)[i] = v;
 }
 
+// Phase stamps (built only with LUCAS_PHASE_STAMPS, into the separate
+// profiling library): work-item 0 of each of the first LUCAS_STAMP_WGS
+// workgroups records the 100 MHz real-time counter at the phase boundaries of
+// the steady-state (mode 0) column launches and of the row launches, six
+// stamps per workgroup, into a per-device array the host reads back.  The
+// default build compiles the marks to nothing.
+#define LUCAS_STAMP_WGS 8192
+#ifdef LUCAS_PHASE_STAMPS
+__device__ unsigned long long lucas_stamps_col[LUCAS_STAMP_WGS * 8];
+__device__ unsigned long long lucas_stamps_row[LUCAS_STAMP_WGS * 8];
+#define LUCAS_STAMP_COL(k)                                                                 \
+    do {                                                                                   \
+        if (mode == 0 && threadIdx.x == 0 && blockIdx.x < LUCAS_STAMP_WGS)                 \
+            lucas_stamps_col[blockIdx.x * 8 + (k)] = __builtin_amdgcn_s_memrealtime();     \
+    } while (0)
+#define LUCAS_STAMP_ROW(k)                                                                 \
+    do {                                                                                   \
+        if (threadIdx.x == 0 && blockIdx.x < LUCAS_STAMP_WGS)                              \
+            lucas_stamps_row[blockIdx.x * 8 + (k)] = __builtin_amdgcn_s_memrealtime();     \
+    } while (0)
+#else
+#define LUCAS_STAMP_COL(k) \
+    do {                   \
+    } while (0)
+#define LUCAS_STAMP_ROW(k) \
+    do {                   \
+    } while (0)
+#endif
+
 // Dynamic LDS of the running workgroup (sized per launch).
 __device__ __forceinline__ double2* lucas_dynamic_lds() {
     extern __shared__ double2 lucas_lds[];
@@ -593,6 +622,7 @@ __device__ __forceinline__ int2 restore_element(const double2* lds, const LucasJ
 // ---------------------------------------------------------------------------
 template <int T, int PA>
 __device__ __forceinline__ void col_kernel_body(const LucasJobDesc& J, int tile, int mode) {
+    LUCAS_STAMP_COL(0);
     double2* const lds = lucas_dynamic_lds();
     const int A = J.A, B = J.B, C = J.C, H = J.halo, N = J.N;
     const int n = J.ncarry;
@@ -657,6 +687,7 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc& J, int tile,
         colr[t] = gload(J.rlo, jj);
     }
     __syncthreads();
+    LUCAS_STAMP_COL(1);  // tile and tables loaded
 
     // weight: (y_2m, y_2m+1) = 2^(r_j/N) (x_2m, x_2m+1) for own element (r, lo)
     auto weighted = [&](int r, int lo, int2 x) {
@@ -719,6 +750,7 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc& J, int tile,
         // publishes h_2m+1 in hb; after the barrier stage 2 adds h_2m-1 from
         // the element to the left and overwrites its own element with the
         // weighted new digits (or stores them).
+        LUCAS_STAMP_COL(2);  // inverse FFT and unweight done
         const int c0mH = c0 - H;
         const int nper = (A * C + T - 1) / T;
         for (int i = 0; i < nper; ++i) {
@@ -753,18 +785,21 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc& J, int tile,
         }
     }
     __syncthreads();
+    LUCAS_STAMP_COL(3);  // carry done
     // forward column FFT of the own columns
     {
         const SeqGeom g{C, mC, 0, 0u, 1, 0, H, ncl};
         const SeqGeom gA{C * wA, J.m_sqA1, C, mC, 1, ncl, H, wA * ncl};
         fft_seq<T, PA>(lds, lst, J.nradA, tws, g, gA, A1, J.m_A1, twL, twW, tid);
     }
+    LUCAS_STAMP_COL(4);  // forward FFT done
     // store row-major; the outer twiddle omega_M^(c*k) is applied by the row
     // kernel as it loads
     for (int t = tid; t < A * C; t += T) {
         const int k = udiv(t, C, mC), lo = t - k * C;
         gstore2(J.spec, (size_t)k * B + c0 + lo, lds[pidx(k * ncl + H + lo)]);
     }
+    LUCAS_STAMP_COL(5);  // stores issued
     wave_max_err(err, seen, J.maxerr);
 }
 
@@ -781,6 +816,7 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc& J, int tile,
 // ---------------------------------------------------------------------------
 template <int T, int PB>
 __device__ __forceinline__ void row_kernel_body(const LucasJobDesc& J, int r0) {
+    LUCAS_STAMP_ROW(0);
     double2* const lds = lucas_dynamic_lds();
     const int A = J.A, B = J.B;
     const int tid = threadIdx.x;
@@ -814,7 +850,9 @@ __device__ __forceinline__ void row_kernel_body(const LucasJobDesc& J, int r0) {
     // forward row FFT; square; inverse row FFT (conjugated input).  Two call
     // sites: a loop over the two transforms costs registers across the loop.
     __syncthreads();
+    LUCAS_STAMP_ROW(1);  // rows and tables loaded
     fft_seq<T, PB>(lds, lst, J.nradB, tws, g, gA, B1, J.m_B1, twu, twW, tid);
+    LUCAS_STAMP_ROW(2);  // forward FFT done
     {
         // Pairs (kc in row r0) <-> (pc in row r1) for rows k, A-k: kc < B,
         // pc = B-1-kc.  Workgroup 0 pairs within each row instead: row 0 has
@@ -841,11 +879,14 @@ __device__ __forceinline__ void row_kernel_body(const LucasJobDesc& J, int r0) {
         }
     }
     __syncthreads();
+    LUCAS_STAMP_ROW(3);  // square done
     fft_seq<T, PB>(lds, lst, J.nradB, tws, g, gA, B1, J.m_B1, twu, twW, tid);
+    LUCAS_STAMP_ROW(4);  // inverse FFT done
     for (int t = tid; t < nrows * B; t += T) {
         const int rl = t >= B ? 1 : 0, kc = t - rl * B;
         gstore2(J.spec2, (size_t)(rl ? r1 : r0) * B + kc, cconj(lds[pidx(t)]));
     }
+    LUCAS_STAMP_ROW(5);  // stores issued
 }
 
 // The single-job kernels take the descriptor as a kernel argument (4 KiB limit).
@@ -970,6 +1011,30 @@ hipError_t configure_kernels(size_t max_lds) {
             if (e != hipSuccess) return e;
         }
     return hipSuccess;
+}
+
+bool phase_stamps_built() {
+#ifdef LUCAS_PHASE_STAMPS
+    return true;
+#else
+    return false;
+#endif
+}
+
+// Copies the stamps of the first ncol column and nrow row workgroups of the
+// current device (eight words per workgroup, six used) after the device
+// has finished the launches that wrote them.
+hipError_t read_phase_stamps(unsigned long long* col, int ncol, unsigned long long* row, int nrow) {
+#ifdef LUCAS_PHASE_STAMPS
+    if (ncol < 0 || nrow < 0 || ncol > LUCAS_STAMP_WGS || nrow > LUCAS_STAMP_WGS) return hipErrorInvalidValue;
+    hipError_t e = hipMemcpyFromSymbol(col, HIP_SYMBOL(lucas_stamps_col), sizeof(unsigned long long) * 8 * ncol);
+    if (e == hipSuccess)
+        e = hipMemcpyFromSymbol(row, HIP_SYMBOL(lucas_stamps_row), sizeof(unsigned long long) * 8 * nrow);
+    return e;
+#else
+    (void)col, (void)ncol, (void)row, (void)nrow;
+    return hipErrorNotSupported;
+#endif
 }
 
 // dense: the 64-VGPR variant (1024 work-items only)

@@ -32,6 +32,7 @@ ERR_DEVICE = -4
 ERR_IO = -5
 ERR_NO_DEVICE = -6
 ERR_BUSY = -7
+ERR_UNSUPPORTED = -8
 DEFAULT_ERROR_LIMIT = 0.30
 
 
@@ -112,6 +113,8 @@ _SIGS = {
     "lucas_ctx_res64": (C.c_int, [C.c_void_p, _P(C.c_uint64), _P(C.c_int)]),
     "lucas_ctx_max_roundoff": (C.c_int, [C.c_void_p, C.c_int, _P(C.c_double)]),
     "lucas_ctx_time_kernels": (C.c_int, [C.c_void_p, C.c_int, _P(C.c_double), _P(C.c_double)]),
+    "lucas_ctx_phase_stamps": (C.c_int, [C.c_void_p, C.c_int, _P(C.c_ulonglong), _P(C.c_ulonglong), C.c_int,
+                                         _P(C.c_int), _P(C.c_int)]),
     "lucas_ctx_time_iterations": (C.c_int, [C.c_void_p, C.c_longlong, _P(C.c_double)]),
     "lucas_ctx_save": (C.c_int, [C.c_void_p, C.c_char_p]),
     "lucas_ctx_load": (C.c_int, [C.c_char_p, C.c_int, _P(C.c_void_p)]),
@@ -300,6 +303,19 @@ class Context:
         a, b = C.c_double(0), C.c_double(0)
         _check(_lib().lucas_ctx_time_kernels(self._h, iters, C.byref(a), C.byref(b)), "time_kernels")
         return a.value, b.value
+
+    def phase_stamps(self, iters: int = 20, max_workgroups: int = 8192) -> tuple[np.ndarray, np.ndarray]:
+        """Real-time stamps (100 MHz ticks, int64) of the last column and row
+        launch after `iters` iterations: arrays of shape (workgroups, 6).  Needs
+        the profiling library (libgpulucas_phases.so); the default library
+        raises LucasError with status ERR_UNSUPPORTED."""
+        col = np.zeros((max_workgroups, 8), np.uint64)
+        row = np.zeros((max_workgroups, 8), np.uint64)
+        nc, nr = C.c_int(0), C.c_int(0)
+        _check(_lib().lucas_ctx_phase_stamps(self._h, iters, col.ctypes.data_as(_P(C.c_ulonglong)),
+                                             row.ctypes.data_as(_P(C.c_ulonglong)), max_workgroups,
+                                             C.byref(nc), C.byref(nr)), "phase_stamps")
+        return col[:nc.value, :6].astype(np.int64), row[:nr.value, :6].astype(np.int64)
 
     def time_iterations(self, iters: int) -> float:
         v = C.c_double(0)

@@ -37,6 +37,7 @@ extern "C" {
 #define LUCAS_ERR_IO (-5)              /* checkpoint read/write failure (gpuLucas.cu:1569-1665) */
 #define LUCAS_ERR_NO_DEVICE (-6)       /* no usable GPU */
 #define LUCAS_ERR_BUSY (-7)            /* server queue full / admission refused */
+#define LUCAS_ERR_UNSUPPORTED (-8)     /* not available in this build of the library */
 
 /* Reference ERROR_LIMIT (gpuLucas.cu:176). */
 #define LUCAS_DEFAULT_ERROR_LIMIT 0.30
@@ -137,6 +138,18 @@ int lucas_ctx_max_roundoff(lucas_ctx* ctx, int reset, double* out);
  * recorded on the context's stream after every launch (microseconds; each
  * interval includes that launch's dispatch gap).  Max iters 4096. */
 int lucas_ctx_time_kernels(lucas_ctx* ctx, int iters, double* col_us, double* row_us);
+/* Phase profile (no counterpart in the reference).  Only the profiling build
+ * libgpulucas_phases.so implements it; libgpulucas.so returns
+ * LUCAS_ERR_UNSUPPORTED.  Runs `iters` iterations in the lucas_ctx_step order
+ * (digits are not touched) and copies, for each workgroup of the last column
+ * and row launch, 8 words of which the first 6 are 100 MHz real-time stamps:
+ * column: start, tile loaded, inverse FFT + unweight done, carry done,
+ * forward FFT done, stores issued; row: start, rows loaded, forward FFT done,
+ * square done, inverse FFT done, stores issued.  *n_col, *n_row receive the
+ * workgroup counts (at most max_workgroups each, else LUCAS_ERR_INVALID_ARG).
+ * Single-exponent contexts only. */
+int lucas_ctx_phase_stamps(lucas_ctx* ctx, int iters, unsigned long long* col_stamps,
+                           unsigned long long* row_stamps, int max_workgroups, int* n_col, int* n_row);
 /* Mean wall time of `iters` iterations (graph-launched), in ms/iteration. */
 int lucas_ctx_time_iterations(lucas_ctx* ctx, long long iters, double* ms_per_iter);
 /* Checkpoint (own versioned format; rotates path -> path.bak like

@@ -40,6 +40,7 @@ extern emu_idx blockDim;
 typedef int hipError_t;
 typedef void* hipStream_t;
 #define hipSuccess 0
+#define hipErrorNotSupported 801
 inline hipError_t hipGetLastError() { return 0; }
 #define hipFuncAttributeMaxDynamicSharedMemorySize 0
 inline hipError_t hipFuncSetAttribute(const void*, int, int) { return 0; }

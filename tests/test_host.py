@@ -27,6 +27,37 @@ def test_library_exports_every_declared_function():
         assert hasattr(lib, n), n
     # the ctypes mirror binds exactly the declared set
     assert lucas.exported_symbols() == names
+    # the profiling build exports the same set
+    from gpulucas_amd import build
+    plib = ctypes.CDLL(build.PHASES_LIB)
+    for n in names:
+        assert hasattr(plib, n), n
+
+
+def test_phase_stamps_need_the_profiling_build():
+    # the default library refuses before touching a device; bad handles first
+    f = lucas._lib().lucas_ctx_phase_stamps
+    assert f(None, 1, None, None, 0, None, None) == lucas.ERR_INVALID_ARG
+    assert lucas._lib().lucas_status_string(lucas.ERR_UNSUPPORTED) == b"not supported by this build"
+
+
+def test_phase_summary_of_stamps():
+    import numpy as np
+    from gpulucas_amd import phases
+    # two workgroups; ticks of 10 ns
+    st = np.array([[100, 300, 500, 600, 900, 910],
+                   [150, 350, 600, 700, 1000, 1030]], np.int64)
+    s = phases.summarize(st, phases.COL_PHASES)
+    assert s["workgroups"] == 2
+    assert s["span_us"] == 9.3 and s["start_spread_us"] == 0.5
+    assert s["workgroup_us_mean"] == 8.45 and s["workgroup_us_max"] == 8.8
+    assert s["phase_us_mean"]["load"] == 2.0
+    assert s["phase_us_mean"]["inverse FFT + unweight"] == 2.25
+    assert s["phase_us_max"]["store"] == 0.3
+    bad = st.copy()
+    bad[0, 3] = 400  # carry ends before it starts
+    with pytest.raises(ValueError):
+        phases.summarize(bad, phases.COL_PHASES)
 
 
 def test_version_and_status_strings():
