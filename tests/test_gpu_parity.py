@@ -197,6 +197,21 @@ def test_long_transforms_vs_gmp(p, k, gmp_fixtures):
         assert ctx.max_roundoff() < ROUNDOFF_LIMIT
 
 
+@pytest.mark.parametrize("p,N,k", [(20996011, 1572864, 100), (20996011, 1572864, 2000),
+                                   (37156667, 2097152, 100), (37156667, 2097152, 2000)])
+def test_dense_column_kernel_vs_gmp(p, N, k, gmp_fixtures):
+    """Lengths whose column grid has more 1024-item tiles than CUs run the
+    dense (64-VGPR, two workgroups per CU) column kernel: Res64 against GMP."""
+    info = lucas.plan(p, N)
+    assert info["col_dense"] == 1 and info["threads"] == 1024 and info["col_wgs"] > 256
+    print(f"case p={p} N={N} k={k}", flush=True)
+    with lucas.Context(p, N) as ctx:
+        ctx.step(k)
+        want, zero = fixture_res64(gmp_fixtures, "large", p, k)
+        assert ctx.res64() == (want, zero)
+        assert ctx.max_roundoff() < ROUNDOFF_LIMIT
+
+
 def test_config3_near3m_primes(gmp_fixtures):
     from gpulucas_amd import workloads as wl
     for p in wl.near3m_primes():
