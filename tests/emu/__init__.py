@@ -15,10 +15,14 @@ _lib = None
 
 
 def build():
+    # each process builds into its own file and renames it into place, so
+    # parallel test workers never load a half-written library
+    tmp = f"{LIB}.{os.getpid()}.tmp"
     cmd = ["g++", "-std=c++20", "-O2", "-fPIC", "-shared", "-pthread", "-I" + HERE,
            "-I" + os.path.join(ROOT, "gpulucas_amd", "csrc"), "-I" + os.path.join(ROOT, "include"),
-           "-x", "c++", SOURCES[0], "-x", "c++", SOURCES[1], SOURCES[2], "-o", LIB]
+           "-x", "c++", SOURCES[0], "-x", "c++", SOURCES[1], SOURCES[2], "-o", tmp]
     subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
 
 
 def lib():
