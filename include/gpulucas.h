@@ -222,6 +222,9 @@ int lucas_server_poll(lucas_server* s, long long id, lucas_job_result* out);
 int lucas_server_drain(lucas_server* s, double timeout_s);
 /* Aggregate counters: iterations done, busy seconds, peak jobs resident. */
 int lucas_server_stats(lucas_server* s, long long* iterations, double* busy_s, int* peak_jobs);
+/* Stop the worker and free the server.  Jobs still queued or resident are
+ * dropped with it.  No other call on `s` may run concurrently with it or
+ * follow it (a thread blocked in lucas_server_drain must have returned). */
 int lucas_server_stop(lucas_server* s);
 
 #ifdef __cplusplus
