@@ -243,6 +243,7 @@ int lucas_plan_shape(int p, int N, int A, int C, lucas_plan_info* out) {
 int lucas_plan_list(int p, int N, lucas_plan_info* out, int cap, int* count) {
     if (!count || cap < 0 || (cap > 0 && !out)) return LUCAS_ERR_INVALID_ARG;
     std::vector<Plan> c = plan_candidates(p, N);
+    if (c.empty()) c = plan_candidates(p, N, true);
     if (c.empty()) return LUCAS_ERR_INVALID_ARG;
     *count = (int)c.size();
     for (int i = 0; i < cap && i < (int)c.size(); ++i) fill_info(c[i], &out[i]);
@@ -282,8 +283,10 @@ int lucas_ctx_create_shape(int p, int N, int A, int C, int device, lucas_ctx** o
 int lucas_ctx_create_plan(const lucas_plan_info* plan, int device, lucas_ctx** out) {
     if (!plan || !out) return LUCAS_ERR_INVALID_ARG;
     Plan pl;
-    if (!make_plan_exact(plan->p, plan->N, plan->A, plan->C, plan->threads, plan->radix_pref, &pl, plan->wave_A,
-                         plan->wave_B).empty())
+    const bool ordinary = make_plan_exact(plan->p, plan->N, plan->A, plan->C, plan->threads, plan->radix_pref, &pl,
+                                          plan->wave_A, plan->wave_B).empty();
+    if (!ordinary && !make_plan_exact(plan->p, plan->N, plan->A, plan->C, plan->threads, plan->radix_pref, &pl,
+                                      plan->wave_A, plan->wave_B, true).empty())
         return LUCAS_ERR_INVALID_ARG;
     return ctx_new(pl, device, out);
 }
@@ -529,6 +532,7 @@ int lucas_autotune_shape(int p, int N, int iters, int max_candidates, const luca
     if (!out || p < 7) return LUCAS_ERR_INVALID_ARG;
     if (N == 0) N = default_length(p);
     std::vector<Plan> cands = plan_candidates(p, N);
+    if (cands.empty()) cands = plan_candidates(p, N, true);
     if (cands.empty()) return LUCAS_ERR_INVALID_ARG;
     if (max_candidates > 0 && (int)cands.size() > max_candidates) cands.resize(max_candidates);
     if (iters < 10) iters = 10;

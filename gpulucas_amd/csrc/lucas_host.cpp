@@ -139,7 +139,15 @@ static std::vector<int> wave_factors(int L, int nseq, int T) {
     return out;
 }
 
-std::vector<Plan> plan_candidates(int p, int N) {
+// Words per work-item a transform with these Stockham stages can hold: a
+// stage of radix R runs ceil(EMAX/R) butterflies of R words per work-item.
+static int stage_capacity(const std::vector<int>& rad) {
+    int cap = 8;
+    for (int R : rad) cap = std::min(cap, (LUCAS_EMAX + R - 1) / R * R);
+    return cap;
+}
+
+std::vector<Plan> plan_candidates(int p, int N, bool wide) {
     std::vector<Plan> out;
     Plan pl;
     if (!base_plan(p, N, &pl).empty()) return out;
@@ -152,7 +160,8 @@ std::vector<Plan> plan_candidates(int p, int N) {
             if (B % C) continue;
             const long long tile = std::max<long long>((long long)A * (C + pl.halo), 2LL * B);
             for (int T : {LUCAS_THREADS_SMALL, LUCAS_THREADS_LARGE, LUCAS_THREADS_MAX}) {
-                if (tile > (long long)LUCAS_EMAX * T) continue;
+                if (tile > (long long)(wide ? 8 : LUCAS_EMAX) * T) continue;
+                if (wide && tile <= (long long)LUCAS_EMAX * T) continue;  // ordinary
                 // a workgroup more than twice the tile would leave most work-items idle
                 if (T > LUCAS_THREADS_SMALL && tile < T / 2) continue;
                 const std::vector<int> wAs = wave_factors(A, C + pl.halo, T), wBs = wave_factors(B, 2, T);
@@ -165,6 +174,9 @@ std::vector<Plan> plan_candidates(int p, int N) {
                     if (pref == 4 && ra == radix_plan(A1, 8) && rb == radix_plan(B1, 8)) continue;
                     if ((ra.empty() && A1 > 1) || (rb.empty() && B1 > 1)) continue;
                     if ((int)ra.size() > LUCAS_MAX_RADICES || (int)rb.size() > LUCAS_MAX_RADICES) continue;
+                    if (wide && ((long long)A * (C + pl.halo) > (long long)stage_capacity(ra) * T ||
+                                 2LL * B > (long long)stage_capacity(rb) * T))
+                        continue;
                     const size_t lds_col = 16 * (size_t)lucas_col_lds_words(
                             A, C, pl.halo, stage_twiddle_count(A1, ra), (int)ra.size(), wA / 64);
                     const size_t lds_row =
@@ -203,11 +215,12 @@ static const TunedShape kTuned[] = {
 #include "tuned_shapes.inc"
     {0, 0, 0, 0, 0, 0, 0}};
 
-std::string make_plan_exact(int p, int N, int A, int C, int threads, int pref, Plan* out, int wA, int wB) {
+std::string make_plan_exact(int p, int N, int A, int C, int threads, int pref, Plan* out, int wA, int wB,
+                            bool wide) {
     Plan pl;
     std::string why = base_plan(p, N, &pl);
     if (!why.empty()) return why;
-    for (const Plan& c : plan_candidates(p, N)) {
+    for (const Plan& c : plan_candidates(p, N, wide)) {
         if (A && c.A != A) continue;
         if (C && c.C != C) continue;
         if (threads && c.threads != threads) continue;
@@ -242,7 +255,9 @@ std::string make_plan(int p, int N, int forceA, int forceC, Plan* out, bool shar
             }
         }
     }
-    return make_plan_exact(p, N, forceA, forceC, 0, 0, out);
+    const std::string why = make_plan_exact(p, N, forceA, forceC, 0, 0, out);
+    if (why.empty() || make_plan_exact(p, N, forceA, forceC, 0, 0, out, -1, -1, true).empty()) return "";
+    return why;
 }
 
 // Bits per word the default length rule allows at length N.  Calibrated on
@@ -272,15 +287,23 @@ static double length_cost(long long N) {
     return (double)N * w;
 }
 
-int default_length(int p) {
-    if (p < 7) return 0;
+// Default length among the lengths with an ordinary tiling, or (`wide`)
+// among those with a wide one; 0 if none.
+static int default_length_of(int p, bool wide) {
+    auto admissible = [&](long long N) {
+        Plan pl;
+        return make_plan_exact(p, (int)N, 0, 0, 0, 0, &pl, -1, -1, wide).empty();
+    };
+    // No tiling exists past M = A*B with 2B and 3A (C + H >= 3) within the
+    // words a 1024-item workgroup holds (4 per item, 8 for wide tilings).
+    const long long words = (wide ? 8LL : (long long)LUCAS_EMAX) * LUCAS_THREADS_MAX;
+    const long long max_M = (words / 2) * (words / 3);
     // smallest admissible length under the bits rule
     long long nmin = 0;
-    for (long long N = 4; N <= (long long)p; N += 2) {
+    for (long long N = 4; N <= (long long)p && N / 2 <= max_M; N += 2) {
         if (!valid_length(N)) continue;
         if ((double)p / (double)N > max_bits_per_word((int)N)) continue;
-        Plan pl;
-        if (make_plan(p, (int)N, 0, 0, &pl).empty()) {
+        if (admissible(N)) {
             nmin = N;
             break;
         }
@@ -293,12 +316,17 @@ int default_length(int p) {
         if (!valid_length(N)) continue;
         const double c = length_cost(N);
         if (c >= best_cost) continue;
-        Plan pl;
-        if (!make_plan(p, (int)N, 0, 0, &pl).empty()) continue;
+        if (!admissible(N)) continue;
         best = N;
         best_cost = c;
     }
     return (int)best;
+}
+
+int default_length(int p) {
+    if (p < 7) return 0;
+    const int N = default_length_of(p, false);
+    return N ? N : default_length_of(p, true);
 }
 
 unsigned int magic_u32(unsigned int d) {

@@ -47,10 +47,17 @@ int stage_twiddle_count(int L, const std::vector<int>& rad);
 // wave factors.
 std::string make_plan(int p, int N, int A, int C, Plan* out, bool shared = false);
 // Every admissible (A, C) tiling of (p, N), cheapest modelled first.
-std::vector<Plan> plan_candidates(int p, int N);
+// Ordinary tilings hold at most LUCAS_EMAX*T words per workgroup.  `wide`
+// lists instead the larger tilings only transforms of radix 3, 5, 7 and 8
+// stages can run (a stage of radix R holds ceil(EMAX/R)*R words per
+// work-item, up to 8); they are offered where no ordinary tiling exists,
+// which extends the longest lengths.
+std::vector<Plan> plan_candidates(int p, int N, bool wide = false);
 // Plan matching a full description (A, C, threads, pref 0 = any; wave
-// factors wA, wB < 0 = any); "" on success.
-std::string make_plan_exact(int p, int N, int A, int C, int threads, int pref, Plan* out, int wA = -1, int wB = -1);
+// factors wA, wB < 0 = any) among the ordinary (or with `wide`, the wide)
+// tilings; "" on success.
+std::string make_plan_exact(int p, int N, int A, int C, int threads, int pref, Plan* out, int wA = -1, int wB = -1,
+                            bool wide = false);
 // Default N for p: smallest admissible length under the bits-per-word rule.
 int default_length(int p);
 // Bits per word allowed at length N by the default rule.

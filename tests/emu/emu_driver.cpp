@@ -63,7 +63,8 @@ extern "C" int emu_run(int p, int N, int A, int C, int T, const int32_t* digits_
     using namespace lucas;
     Plan pl;
     if (!(T || g_wA >= 0 || g_wB >= 0 ? make_plan_exact(p, N, A, C, T, 0, &pl, g_wA, g_wB)
-                                       : make_plan(p, N, A, C, &pl)).empty())
+                                       : make_plan(p, N, A, C, &pl)).empty() &&
+        !make_plan_exact(p, N, A, C, T, 0, &pl, g_wA, g_wB, true).empty())
         return -1;
     Tables t;
     make_tables(pl, &t);

@@ -101,6 +101,35 @@ def test_default_length(p):
     assert N >= p / 20.2
 
 
+@pytest.mark.parametrize("p,N", [(100000007, 6291456), (136279841, 8064000)])
+def test_wide_tilings_extend_the_longest_lengths(p, N):
+    """Past the ordinary tilings (at most 4 words per work-item) the planner
+    offers wide ones, whose Stockham stages of radix 3, 5, 7 and 8 hold up to
+    8 words per work-item: exponents up to M52 = 2^136279841 - 1."""
+    assert lucas.default_N(p) == N
+    plans = lucas.plan_list(p, N)
+    assert plans
+    for info in plans:
+        T = info["threads"]
+        tile = max(info["A"] * (info["C"] + info["halo"]), 2 * info["B"])
+        assert 4 * T < tile <= 8 * T
+        # a wave factor holds at most 4 words per work-item: wide column tiles have none
+        if info["A"] * (info["C"] + info["halo"]) > 4 * T:
+            assert info["wave_A"] == 0
+        if 2 * info["B"] > 4 * T:
+            assert info["wave_B"] == 0
+    assert lucas.plan(p, N) in plans
+
+
+def test_longest_supported_exponent():
+    # beyond N = 8064000 no two-pass tiling fits: refused, not mis-planned
+    assert lucas.default_N(150000001) == 0
+    with pytest.raises(lucas.LucasError):
+        lucas.plan(150000001, 0)
+    # the ordinary range keeps its lengths
+    assert lucas.default_N(82589933) == 4915200
+
+
 @pytest.mark.parametrize("p,N", [(86243, 4608), (2976221, 163840), (1257787, 65536), (4423, 240)])
 def test_digit_tables_match_oracle(p, N):
     w, a, ai = lucas.digit_tables(p, N)

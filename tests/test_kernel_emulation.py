@@ -90,3 +90,21 @@ def test_emulated_smaller_wave_factors_match_oracle(p, N, A, C, T, waves):
     want, _ = oracle.ibdwt_iterate(p, N, x, 1)
     assert np.array_equal(got, want)
     assert err < 0.4
+
+
+@pytest.mark.parametrize("p,N,A,C", [
+    (294919, 16384, 512, 1),  # column tile 1536 words on 256 items: radix 8.8.8 stages, 6 words each
+    (180181, 10240, 320, 2),  # column tile 1280: radix 8.8.5, 5 words per item
+    (322561, 18432, 16, 32),  # row pair 1152 words: radix 8.8.3.3, 6 words per item
+])
+def test_emulated_wide_tiles_match_oracle(p, N, A, C):
+    """Wide tilings (more than 4 words per work-item, which only radix 3, 5,
+    7 and 8 stages hold) compute the oracle's digits."""
+    from gpulucas_amd import lucas
+    info = lucas.plan(p, N, A=A, C_=C)  # not an ordinary tiling at 256 items
+    assert max(A * (C + info["halo"]), 2 * info["B"]) > 4 * 256
+    x = oracle.s0_digits(N)
+    got, err = emu.run(p, N, x, 2, A, C, 256, single=True, waves=(0, 0))
+    want, _ = oracle.ibdwt_iterate(p, N, x, 2)
+    assert np.array_equal(got, want)
+    assert err < 0.4
