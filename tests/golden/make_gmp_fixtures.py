@@ -49,9 +49,9 @@ def plan():
     # Long transforms (N = 2^19 .. 2.3M): the top of the serving range.
     runs["large"] = [(9600037, [1000, 5000]), (20996011, [100, 2000]), (37156667, [100, 2000]),
                      (37732501, [2000])]
-    # Lengths past the ordinary tilings (wide tiles, N up to 8064000): 100
-    # iterations of M52 = 2^136279841 - 1 and of a depth-4 exponent.
-    runs["wide"] = [(100000007, [100]), (136279841, [100])]
+    # The longest lengths: M48 (N = 3670016) and, past the ordinary tilings
+    # (wide tiles, N up to 8064000), a depth-4 exponent and M52.
+    runs["longest"] = [(57885161, [100]), (100000007, [100]), (136279841, [100])]
     return runs
 
 

@@ -212,17 +212,18 @@ def test_dense_column_kernel_vs_gmp(p, N, k, gmp_fixtures):
         assert ctx.max_roundoff() < ROUNDOFF_LIMIT
 
 
-@pytest.mark.parametrize("p,N", [(100000007, 6291456), (136279841, 8064000)])
-def test_wide_tiles_longest_lengths_vs_gmp(p, N, gmp_fixtures):
-    """The longest lengths, past the ordinary tilings (up to 8 words per
-    work-item in radix 3/5/7/8 stages), up to M52 = 2^136279841 - 1: Res64
-    after 100 iterations against GMP."""
+@pytest.mark.parametrize("p,N", [(57885161, 3670016), (100000007, 6291456), (136279841, 8064000)])
+def test_longest_lengths_vs_gmp(p, N, gmp_fixtures):
+    """The longest lengths: M48 at its tuned shape and, past the ordinary
+    tilings (wide tiles, up to 8 words per work-item in radix 3/5/7/8
+    stages), up to M52 = 2^136279841 - 1: Res64 after 100 iterations
+    against GMP."""
     assert lucas.default_N(p) == N
     info = lucas.plan(p, N)
     print(f"case p={p} N={N} A={info['A']} B={info['B']} C={info['C']} T={info['threads']}", flush=True)
     with lucas.Context(p, N) as ctx:
         ctx.step(100)
-        want, zero = fixture_res64(gmp_fixtures, "wide", p, 100)
+        want, zero = fixture_res64(gmp_fixtures, "longest", p, 100)
         assert ctx.res64() == (want, zero)
         assert ctx.max_roundoff() < ROUNDOFF_LIMIT
 
