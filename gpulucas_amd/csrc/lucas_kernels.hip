@@ -649,7 +649,9 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc& J, int tile,
     double2* const twW = p;  // wave-factor twiddles (fill_wave_table)
     p += lucas_wave_table_words(PA);
     signed char* const hb = reinterpret_cast<signed char*>(p);  // carries h_2m+1 of own elements, [A][C]
-    const unsigned int seen = *reinterpret_cast<volatile unsigned int*>(J.maxerr);
+    // the job's maximum so far: a relaxed device-coherent load, first needed
+    // by the wave reductions at the end, so no wait follows it here
+    const unsigned int seen = __hip_atomic_load(J.maxerr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const double2* const twA = J.twA;
     const double2* const twMB = J.twMB;
     const bool in_digits = (mode & LUCAS_IN_DIGITS) != 0;

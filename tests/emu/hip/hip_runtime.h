@@ -107,6 +107,12 @@ inline unsigned __brev(unsigned x) {
 }
 inline unsigned __umulhi(unsigned a, unsigned b) { return (unsigned)(((unsigned long long)a * b) >> 32); }
 inline unsigned __float_as_uint(float f) { unsigned u; std::memcpy(&u, &f, 4); return u; }
+#define __HIP_MEMORY_SCOPE_AGENT 3
+template <typename T>
+inline T __hip_atomic_load(T* p, int, int) {
+    std::lock_guard<std::mutex> lk(emu::atom_mu);
+    return *p;
+}
 inline unsigned atomicMax(unsigned* p, unsigned v) {
     std::lock_guard<std::mutex> lk(emu::atom_mu);
     unsigned old = *p;
