@@ -156,11 +156,60 @@ int read_checkpoint_file(const char* path, int* p, int* N, long long* iter, std:
     return rc;
 }
 
-// primary, then ".bak" (readCheckpoint, gpuLucas.cu:1569-1591)
+// A checkpoint written by the reference (writeCheckpoint, gpuLucas.cu:1641-1665):
+//   char name[16] = "gpuLucas"; char version[16]; int32 p, N, iter;
+//   double signal[N]
+// The signal is the next squaring's FFT input, the weighted digits
+// x_j * 2^(r_j/N) (loadIntToDoubleIBDWT, gpuLucas.cu:730-766), and holds
+// s_(iter-1): the reference resumes at loop index iter + 1
+// (readCheckpoint, gpuLucas.cu:1632-1634; one squaring per index from
+// index 2 = s_1).  Each digit is recovered by dividing out its weight and
+// rounding; a value further than 1/4 from an integer, or a digit past its
+// width, marks the file as not a valid state.
+int read_reference_checkpoint_file(const char* path, int* p, int* N, long long* iter,
+                                   std::vector<int32_t>* digits) {
+    FILE* f = fopen(path, "rb");
+    if (!f) return LUCAS_ERR_IO;
+    char name[16], vers[16];
+    int32_t hp = 0, hN = 0, hiter = 0;
+    int rc = LUCAS_ERR_IO;
+    if (fread(name, 1, 16, f) == 16 && fread(vers, 1, 16, f) == 16 && memcmp(name, "gpuLucas", 9) == 0 &&
+        fread(&hp, 4, 1, f) == 1 && fread(&hN, 4, 1, f) == 1 && fread(&hiter, 4, 1, f) == 1 && hp >= 7 &&
+        hN <= (1 << 28) && valid_length(hN) && hN <= hp && hiter >= 2 && hiter <= hp) {
+        std::vector<double> sig((size_t)hN);
+        if (fread(sig.data(), sizeof(double), (size_t)hN, f) == (size_t)hN) {
+            std::vector<int> w((size_t)hN);
+            digit_widths(hp, hN, w.data());
+            digits->resize((size_t)hN);
+            rc = LUCAS_OK;
+            for (int j = 0; j < hN && rc == LUCAS_OK; ++j) {
+                const double a = (double)exp2l((long double)r_of(j, hp, hN) / (long double)hN);
+                const double x = sig[(size_t)j] / a, xr = rint(x);
+                if (!(fabs(x - xr) <= 0.25) || !(fabs(xr) < ldexp(1.0, w[(size_t)j] + 1)))
+                    rc = LUCAS_ERR_IO;
+                else
+                    (*digits)[(size_t)j] = (int32_t)xr;
+            }
+            if (rc == LUCAS_OK) {
+                *p = hp;
+                *N = hN;
+                *iter = hiter - 1;
+            }
+        }
+    }
+    fclose(f);
+    return rc;
+}
+
+// primary, then ".bak" (readCheckpoint, gpuLucas.cu:1569-1591); each in this
+// library's format or the reference's
 int read_checkpoint(const char* path, int* p, int* N, long long* iter, std::vector<int32_t>* digits) {
-    if (read_checkpoint_file(path, p, N, iter, digits) == LUCAS_OK) return LUCAS_OK;
-    std::string bak = std::string(path) + ".bak";
-    return read_checkpoint_file(bak.c_str(), p, N, iter, digits);
+    const std::string bak = std::string(path) + ".bak";
+    for (const char* f : {path, bak.c_str()}) {
+        if (read_checkpoint_file(f, p, N, iter, digits) == LUCAS_OK) return LUCAS_OK;
+        if (read_reference_checkpoint_file(f, p, N, iter, digits) == LUCAS_OK) return LUCAS_OK;
+    }
+    return LUCAS_ERR_IO;
 }
 
 }  // namespace

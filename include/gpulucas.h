@@ -153,7 +153,11 @@ int lucas_ctx_phase_stamps(lucas_ctx* ctx, int iters, unsigned long long* col_st
 /* Mean wall time of `iters` iterations (graph-launched), in ms/iteration. */
 int lucas_ctx_time_iterations(lucas_ctx* ctx, long long iters, double* ms_per_iter);
 /* Checkpoint (own versioned format; rotates path -> path.bak like
- * writeCheckpoint / readCheckpoint, gpuLucas.cu:1569-1665). */
+ * writeCheckpoint / readCheckpoint, gpuLucas.cu:1569-1665).  lucas_ctx_load
+ * tries path, then path.bak, each in this format or in the reference's
+ * (program name, version, p, N, loop index, weighted double signal), so a
+ * run checkpointed by gpuLucas resumes here.  Unreadable or inconsistent
+ * files return LUCAS_ERR_IO before any device work. */
 int lucas_ctx_save(lucas_ctx* ctx, const char* path);
 int lucas_ctx_load(const char* path, int device, lucas_ctx** out);
 

@@ -249,6 +249,30 @@ def test_error_test_contract():
         assert lucas.errorTest(1310719, 65536, 200) == -1.0
 
 
+def test_reference_checkpoint_resume_vs_gmp(tmp_path, gmp_fixtures):
+    """A run checkpointed in the reference's format (as gpuLucas writes it
+    after loop index 1001, i.e. s_1000) resumes here: GMP's residue at
+    10000, also when only the backup file is left."""
+    from refckpt import write_reference_checkpoint
+    p, N = 86243, 4608
+    with lucas.Context(p, N) as ctx:
+        ctx.step(1000)
+        x = ctx.digits()
+    path = str(tmp_path / "86243.chk")
+    write_reference_checkpoint(path, p, N, 1001, x)
+    want, zero = fixture_res64(gmp_fixtures, "c1_p86243", p, 10000)
+    with lucas.Context.load(path) as ctx:
+        assert ctx.iteration == 1000 and ctx.N == N
+        ctx.step(9000)
+        assert ctx.res64() == (want, zero)
+    import os
+    os.rename(path, path + ".bak")
+    with lucas.Context.load(path) as ctx:
+        assert ctx.iteration == 1000
+        ctx.step(9000)
+        assert ctx.res64() == (want, zero)
+
+
 def test_checkpoint_resume_identical(tmp_path):
     p, N = 86243, 4608
     path = str(tmp_path / "m86243.chk")

@@ -121,6 +121,45 @@ def test_wide_tilings_extend_the_longest_lengths(p, N):
     assert lucas.plan(p, N) in plans
 
 
+def test_reference_checkpoint_format_is_read(tmp_path):
+    """A checkpoint written by the reference (weighted double signal) is
+    parsed back to its digits; damaged ones are refused before any device
+    work."""
+    from refckpt import write_reference_checkpoint
+    p, N = 86243, 4608
+    x, _ = oracle.ibdwt_iterate(p, N, oracle.s0_digits(N), 5)
+    good = str(tmp_path / "86243.chk")
+    write_reference_checkpoint(good, p, N, 6, x)  # loop index 6 holds s_5
+    try:
+        ctx = lucas.Context.load(good)
+    except lucas.LucasError as e:
+        assert e.status == lucas.ERR_NO_DEVICE  # parsed; no GPU in this process
+    else:
+        with ctx:
+            assert ctx.iteration == 5 and np.array_equal(ctx.digits(), x)
+    bad = str(tmp_path / "bad.chk")
+    # truncated signal
+    with open(good, "rb") as f:
+        blob = f.read()
+    with open(bad, "wb") as f:
+        f.write(blob[:-8])
+    with pytest.raises(lucas.LucasError) as e:
+        lucas.Context.load(bad)
+    assert e.value.status == lucas.ERR_IO
+    # a value between two digits is not a state
+    y = x.astype(np.float64)
+    y[7] += 0.5
+    write_reference_checkpoint(bad, p, N, 6, y)
+    with pytest.raises(lucas.LucasError) as e:
+        lucas.Context.load(bad)
+    assert e.value.status == lucas.ERR_IO
+    # another program's file
+    write_reference_checkpoint(bad, p, N, 6, x, name=b"CUDALucas")
+    with pytest.raises(lucas.LucasError) as e:
+        lucas.Context.load(bad)
+    assert e.value.status == lucas.ERR_IO
+
+
 def test_longest_supported_exponent():
     # beyond N = 8064000 no two-pass tiling fits: refused, not mis-planned
     assert lucas.default_N(150000001) == 0
