@@ -124,6 +124,24 @@ def test_full_run_mersenne_primes(p, N):
     assert r["max_roundoff"] < ROUNDOFF_LIMIT
 
 
+@pytest.mark.parametrize("p", [61, 89, 107, 127, 521, 607])
+def test_smallest_mersenne_primes(p):
+    """The bottom of the range (N = 16 .. 30, carry depth up to 6): the
+    Mersenne primes end at residue 0."""
+    r = lucas.mersenneTest(p, 0)
+    assert r["status"] == lucas.OK and r["is_prime"] == 1 and r["res64"] == 0
+    assert r["iterations"] == p - 2
+
+
+@pytest.mark.parametrize("p", [37, 41, 43, 47, 53, 59, 67, 71, 101, 509])
+def test_smallest_composites_vs_gmp(p):
+    """The smallest exponents with a tiling: full runs, Res64 against GMP."""
+    r = lucas.mersenneTest(p, 0)
+    want, zero = oracle.gmp_res64(p, [p - 2])[0]
+    assert (r["res64"], bool(r["is_prime"])) == (want, zero) and not zero
+    assert r["max_roundoff"] < ROUNDOFF_LIMIT
+
+
 @pytest.mark.parametrize("p", [1009, 2003, 4001, 9973, 10007, 20011, 30011, 44501, 50021])
 def test_full_run_composites_vs_gmp(p, gmp_fixtures):
     print(f"case p={p}", flush=True)

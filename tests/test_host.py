@@ -160,6 +160,14 @@ def test_reference_checkpoint_format_is_read(tmp_path):
     assert e.value.status == lucas.ERR_IO
 
 
+def test_smallest_supported_exponent():
+    # below p = 32 no length leaves room for the carry restoration and the halo
+    assert lucas.default_N(31) == 0
+    with pytest.raises(lucas.LucasError):
+        lucas.plan(31, 0)
+    assert lucas.default_N(37) == 16
+
+
 def test_longest_supported_exponent():
     # beyond N = 8064000 no two-pass tiling fits: refused, not mis-planned
     assert lucas.default_N(150000001) == 0
