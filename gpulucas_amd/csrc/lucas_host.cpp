@@ -151,7 +151,10 @@ std::vector<Plan> plan_candidates(int p, int N, bool wide) {
     std::vector<Plan> out;
     Plan pl;
     if (!base_plan(p, N, &pl).empty()) return out;
-    for (int A = 2; A <= pl.M; ++A) {
+    // every tile below holds at most 8 words per item of a 1024-item
+    // workgroup, and C + H >= 3: A <= 8192/3 and B = M/A <= 4096
+    const int a_lo = std::max(2, (pl.M + 4095) / 4096), a_hi = std::min(pl.M, 8192 / 3);
+    for (int A = a_lo; A <= a_hi; ++A) {
         if (pl.M % A) continue;
         const int B = pl.M / A;
         if (B < pl.halo) continue;
