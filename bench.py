@@ -86,17 +86,19 @@ def long_transform(lucas, p, N, steps, warmup):
 
 
 def batched_ratio(lucas, wl, iters, count, seed=4, lo=1_000_000, hi=5_000_000):
-    """Aggregate iterations/s: all exponents in one batch vs one at a time."""
+    """Aggregate iterations/s: all exponents in one batch vs one at a time.
+    Each side first runs `iters` untimed iterations, which builds its replay
+    graphs, so neither timed region includes graph instantiation."""
     ps = wl.batch_primes(seed=seed, count=count, lo=lo, hi=hi)
     b = lucas.Batch(ps)
-    b.step(2)
+    b.step(iters)
     ms_batch = b.time_iterations(iters)
     b.close()
     batched_its = count * 1e3 / ms_batch
     total_ms = 0.0
     for p in ps:
         c = lucas.Context(p, 0)
-        c.step(2)
+        c.step(iters)
         total_ms += c.time_iterations(iters) * iters
         c.close()
     plain_its = count * iters * 1e3 / total_ms
@@ -201,7 +203,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20000)
-    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=600)
     ap.add_argument("--batch-iters", type=int, default=300)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -273,7 +275,8 @@ def main():
         out["roofline"]["traffic"] = tr.get(key)
         out["roofline"]["traffic_by_kernel"] = tr
     if not a.no_long:
-        out["long_transform"] = long_transform(lucas, P_LONG, N_LONG, a.long_steps, 64)
+        # 600 untimed iterations build both replay graphs (128 and 32 iterations)
+        out["long_transform"] = long_transform(lucas, P_LONG, N_LONG, a.long_steps, 600)
     if not a.no_batch:
         bi, pi = batched_ratio(lucas, wl, a.batch_iters, 64)
         out["batched"] = {"workload": "64 primes in [1e6,5e6] (seed 4), default N each",

@@ -31,7 +31,13 @@ hipError_t configure_kernels(size_t max_lds);
 bool phase_stamps_built();
 hipError_t read_phase_stamps(unsigned long long* col, int ncol, unsigned long long* row, int nrow);
 
-static const int kGraphIters = 32;
+// Iterations per replayed graph: a step of at least kLongStep iterations runs
+// graphs of 128 (measured 0.8 % faster per iteration than graphs of 32 at
+// N = 163840), which amortises their instantiation; its remainder, and every
+// shorter step (a server's chunks, whose job mix changes often), runs graphs
+// of 32, then single launches.
+static const int kGraphIters[2] = {128, 32};
+static const long long kLongStep = 512;
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
@@ -166,8 +172,7 @@ Engine::Engine(int device) : device_(device) {}
 
 Engine::~Engine() {
     hipSetDevice(device_);
-    if (gexec_) hipGraphExecDestroy(gexec_);
-    if (graph_) hipGraphDestroy(graph_);
+    destroy_graphs();
     if (d_desc_) hipFree(d_desc_);
     for (Group* gs : {colg_, rowg_})
         for (int i = 0; i < kGroups; ++i)
@@ -199,14 +204,7 @@ void Engine::set_jobs(const std::vector<Job*>& jobs) {
 int Engine::prepare() {
     if (!dirty_) return LUCAS_OK;
     hipSetDevice(device_);
-    if (gexec_) {
-        hipGraphExecDestroy(gexec_);
-        gexec_ = nullptr;
-    }
-    if (graph_) {
-        hipGraphDestroy(graph_);
-        graph_ = nullptr;
-    }
+    destroy_graphs();
     std::vector<LucasJobDesc> desc;
     std::vector<int2> cmap[kGroups], rmap[kGroups];
     static const int kThreads[4] = {LUCAS_THREADS_SMALL, LUCAS_THREADS_LARGE, LUCAS_THREADS_MAX, LUCAS_THREADS_MAX};
@@ -301,30 +299,38 @@ int Engine::launch_row() {
     return LUCAS_OK;
 }
 
-int Engine::build_graph() {
-    if (gexec_) return LUCAS_OK;
+void Engine::destroy_graphs() {
+    for (int l = 0; l < 2; ++l) {
+        if (gexec_[l]) hipGraphExecDestroy(gexec_[l]);
+        if (graph_[l]) hipGraphDestroy(graph_[l]);
+        gexec_[l] = nullptr;
+        graph_[l] = nullptr;
+    }
+}
+
+int Engine::build_graph(int level) {
+    if (gexec_[level]) return LUCAS_OK;
     hipError_t e = hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return LUCAS_ERR_DEVICE;
     int rc = LUCAS_OK;
-    for (int i = 0; i < kGraphIters && rc == LUCAS_OK; ++i) {
+    for (int i = 0; i < kGraphIters[level] && rc == LUCAS_OK; ++i) {
         rc = launch_row();
         if (rc == LUCAS_OK) rc = launch_col(0);
     }
-    e = hipStreamEndCapture(stream_, &graph_);
+    e = hipStreamEndCapture(stream_, &graph_[level]);
     if (rc != LUCAS_OK || e != hipSuccess) {
-        if (graph_) hipGraphDestroy(graph_);
-        graph_ = nullptr;
+        if (graph_[level]) hipGraphDestroy(graph_[level]);
+        graph_[level] = nullptr;
         (void)hipGetLastError();
         return LUCAS_ERR_DEVICE;
     }
-    e = hipGraphInstantiate(&gexec_, graph_, nullptr, nullptr, 0);
+    e = hipGraphInstantiate(&gexec_[level], graph_[level], nullptr, nullptr, 0);
     if (e != hipSuccess) {
-        hipGraphDestroy(graph_);
-        graph_ = nullptr;
-        gexec_ = nullptr;
+        hipGraphDestroy(graph_[level]);
+        graph_[level] = nullptr;
+        gexec_[level] = nullptr;
         return LUCAS_ERR_DEVICE;
     }
-    graph_iters_ = kGraphIters;
     return LUCAS_OK;
 }
 
@@ -336,11 +342,12 @@ int Engine::step(long long iters) {
     if (rc != LUCAS_OK) return rc;
     rc = launch_col(LUCAS_IN_DIGITS);
     long long rem = iters - 1;
-    if (rc == LUCAS_OK && rem >= kGraphIters) {
-        rc = build_graph();
-        while (rc == LUCAS_OK && rem >= graph_iters_) {
-            if (hipGraphLaunch(gexec_, stream_) != hipSuccess) rc = LUCAS_ERR_DEVICE;
-            rem -= graph_iters_;
+    for (int l = 0; l < 2; ++l) {
+        if (rc != LUCAS_OK || rem < kGraphIters[l] || (l == 0 && iters < kLongStep)) continue;
+        rc = build_graph(l);
+        while (rc == LUCAS_OK && rem >= kGraphIters[l]) {
+            if (hipGraphLaunch(gexec_[l], stream_) != hipSuccess) rc = LUCAS_ERR_DEVICE;
+            rem -= kGraphIters[l];
         }
     }
     while (rc == LUCAS_OK && rem > 0) {

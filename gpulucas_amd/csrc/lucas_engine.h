@@ -58,7 +58,7 @@ private:
     int prepare();
     int launch_col(int mode);
     int launch_row();
-    int build_graph();
+    int build_graph(int level);
 
     // Jobs whose kernel has the same workgroup size and wave factor share
     // one launch; column and row launches are grouped separately.
@@ -82,9 +82,11 @@ private:
     bool single_ = false;  // one job: launch the by-value-descriptor kernels with one_
     LucasJobDesc one_;
     Group colg_[kGroups], rowg_[kGroups];
-    hipGraph_t graph_ = nullptr;
-    hipGraphExec_t gexec_ = nullptr;
-    int graph_iters_ = 0;
+    // steady-state iterations replayed as graphs of kGraphIters[level]
+    // iterations, built on first use
+    hipGraph_t graph_[2] = {nullptr, nullptr};
+    hipGraphExec_t gexec_[2] = {nullptr, nullptr};
+    void destroy_graphs();
 };
 
 }  // namespace lucas
