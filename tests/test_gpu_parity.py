@@ -162,6 +162,21 @@ def test_config1_p86243_n4608(gmp_fixtures):
         assert ctx.max_roundoff() < ROUNDOFF_LIMIT
 
 
+def test_step_sizes_across_replay_graph_boundaries(gmp_fixtures):
+    """One step of 1000 iterations (graphs of 128 and 32, then single
+    launches) and the same 1000 split at every graph-size boundary reach the
+    same digits and GMP's residue."""
+    p, N = 86243, 4608
+    want, zero = fixture_res64(gmp_fixtures, "c1_p86243", p, 1000)
+    with lucas.Context(p, N) as a, lucas.Context(p, N) as b:
+        a.step(1000)
+        for k in (1, 31, 32, 33, 127, 128, 129, 519):
+            b.step(k)
+        assert a.iteration == b.iteration == 1000
+        assert np.array_equal(a.digits(), b.digits())
+        assert a.res64() == (want, zero)
+
+
 def test_config2_p1257787_n65536_full(gmp_fixtures):
     p, N = 1257787, 65536
     with lucas.Context(p, N) as ctx:
