@@ -48,6 +48,16 @@ static void friendly(char* buf, size_t n, double secs) {
         snprintf(buf, n, "%02ld:%02ld:%02ld", t / 3600, (t % 3600) / 60, t % 60);
 }
 
+// An exponent argument: decimal digits only, at most 2^31 - 1 (the
+// reference's check, gpuLucas.cu:437-441).  0 = ok, 1 = not a number,
+// 2 = too large.
+static int parse_exponent(const char* s, int* p) {
+    if (!*s || strlen(s) != strspn(s, "0123456789")) return 1;
+    if (strlen(s) > 10 || strtoll(s, nullptr, 10) > 2147483647LL) return 2;
+    *p = (int)strtoll(s, nullptr, 10);
+    return 0;
+}
+
 int main(int argc, char** argv) {
     int p = 0, N = 0, device = 0, verbose = 0, quiet = 0;
     long long ckfreq = 50000;
@@ -58,13 +68,29 @@ int main(int argc, char** argv) {
             case 'v': verbose = 1; break;
             case 'q': quiet = 1; break;
             case 'f': N = atoi(optarg); break;
-            case 'n': p = atoi(optarg); break;
+            case 'n':
+                if (parse_exponent(optarg, &p) == 2) {
+                    fprintf(stderr, "testPrime too large, aborting\n");
+                    return 2;
+                }
+                break;
             case 'd': device = atoi(optarg); break;
             case 'c': ckfreq = atoll(optarg); break;
             default: usage(argv[0]); return 2;
         }
     }
-    if (optind < argc) p = atoi(argv[optind]);
+    if (optind < argc) {
+        const int e = parse_exponent(argv[optind], &p);
+        if (e == 2) {
+            fprintf(stderr, "testPrime too large, aborting\n");
+            return 2;
+        }
+        if (e == 1) {
+            usage(argv[0]);
+            fprintf(stderr, "Non-option argument %s\n", argv[optind]);
+            return 2;
+        }
+    }
     if (p < 7 || ckfreq <= 0) {
         usage(argv[0]);
         return 2;

@@ -19,3 +19,13 @@ def test_help_and_usage_errors():
     assert run("-x", "86243").returncode == 2
     assert run("-c", "0", "86243").returncode == 2
     assert os.access(CLI, os.X_OK)
+
+
+def test_exponent_argument_checks():
+    # the reference's bound on the exponent (gpuLucas.cu:437-441)
+    for args in (("3000000000",), ("-n", "99999999999"), ("2147483648",)):
+        r = run(*args)
+        assert r.returncode == 2 and "testPrime too large" in r.stderr, args
+    r = run("86243x")
+    assert r.returncode == 2 and "Non-option argument 86243x" in r.stderr
+    assert run("-n", "abc").returncode == 2
