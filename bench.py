@@ -33,33 +33,40 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def kernel_times(ctx, iters, us_per_iter):
-    """Mean duration of each kernel in the iteration sequence.  HIP events
-    recorded after every launch (row, col, row, col, ...) on the engine's
-    stream give one interval per launch; the markers themselves lengthen each
-    interval, so the excess of (row + col) over the graph-launched iteration
-    time is taken off in equal halves.  Returns (col_us, row_us, raw)."""
-    col_i, row_i = ctx.time_kernels(iters)
-    excess = max(0.0, col_i + row_i - us_per_iter)
-    raw = {"col_interval_us": round(col_i, 3), "row_interval_us": round(row_i, 3),
-           "marker_excess_us": round(excess, 3)}
-    return col_i - excess / 2, row_i - excess / 2, raw
+def kernel_times(ctx, iters):
+    """Mean duration of each kernel in the steady-state iteration.
+
+    The iteration time comes from `iters` graph-launched iterations (graphs
+    built beforehand), the same launches a step makes.  HIP events recorded
+    after every launch of the row/col sequence on the engine's stream give
+    each kernel's share of the iteration; the markers lengthen every
+    interval, so only their ratio is used: col = t_iter * col_i / (col_i +
+    row_i).  Returns (col_us, row_us, t_iter_us, raw)."""
+    t_iter = 1e3 * ctx.time_iterations(iters)
+    col_i, row_i = ctx.time_kernels(200)
+    share = col_i / (col_i + row_i)
+    raw = {"iteration_us": round(t_iter, 3), "col_interval_us": round(col_i, 3),
+           "row_interval_us": round(row_i, 3), "graph_iterations": iters,
+           "method": "graph-launched iteration time split by the ratio of event-timed launch intervals"}
+    return t_iter * share, t_iter * (1 - share), t_iter, raw
 
 
 def headline(lucas, steps, warmup):
     ctx = lucas.Context(P_HEAD, N_HEAD)
     info = ctx.info
     log(f"plan {info}")
-    ctx.step(max(warmup, 1))  # untimed: fills s to p bits, builds the graph
+    ctx.step(max(warmup, 1))  # untimed warm-up iterations: s fills all p bits
+    ctx.prepare(steps)  # set-up: build the replay graphs of a `steps` step (runs nothing)
     ctx.max_roundoff(reset=True)
     t0 = time.perf_counter()
     ctx.step(steps)  # ends with a device synchronisation
     t1 = time.perf_counter()
     ms = 1e3 * (t1 - t0) / steps
     err = ctx.max_roundoff()
-    col_us, row_us, raw = kernel_times(ctx, 200, 1e3 * ms)
+    bits = ctx.product_bits()
+    col_us, row_us, t_iter, raw = kernel_times(ctx, 2000)
     ctx.close()
-    return ms, err, col_us, row_us, info, raw
+    return ms, err, bits, col_us, row_us, info, raw
 
 
 def long_transform(lucas, p, N, steps, warmup):
@@ -68,13 +75,14 @@ def long_transform(lucas, p, N, steps, warmup):
     with lucas.Context(p, N) as ctx:
         info = ctx.info
         ctx.step(max(warmup, 1))
+        ctx.prepare(steps)
         ctx.max_roundoff(reset=True)
         t0 = time.perf_counter()
         ctx.step(steps)
         t1 = time.perf_counter()
         err = ctx.max_roundoff()
         us = 1e6 * (t1 - t0) / steps
-        col_us, row_us, _ = kernel_times(ctx, 50, us)
+        col_us, row_us, _, _ = kernel_times(ctx, 1000)
     gbs = 32 * N / (us * 1e-6) / 1e9
     return {"workload": f"LL iteration p={p} N={N}, FP64 IBDWT", "p": p, "N": N,
             "A": info["A"], "B": info["B"], "C": info["C"], "steps": steps,
@@ -123,50 +131,58 @@ def measured_traffic(info):
     return {k: int(v) for k, v in vals.items() if k != "plan"}
 
 
-def t_iter_model_us(N):
-    """Solo ms/iter model of the engine (fit to tuned MI355X measurements:
-    13.5 us at N=4608, 21.9 us at 163840, 90.6 us at 2286144)."""
-    return 16.7 + 32.0 * N / 1e6
-
-
-def serving(lucas, wl, count, k_scale, load):
-    """Poisson arrivals of (p, K) jobs into the continuous-batching server.
-
-    Bounded sample of the serving workload: the first `count` jobs of seed 5
-    (p log-uniform in [1e5, 4e7]), K scaled by `k_scale`; the arrival rate is
-    set so that running the jobs one after another alone would keep the GPU
-    `load` busy."""
-    import threading
-
-    jobs = wl.serving_jobs(seed=5, count=count)
-    ks = [max(500, int(k * k_scale)) for _, _, k in jobs]
-    ns = [lucas.default_N(p) for _, p, _ in jobs]
-    solo = [k * t_iter_model_us(n) * 1e-6 for k, n in zip(ks, ns)]
-    mean_gap = (sum(solo) / len(solo)) / load
-    scale = mean_gap / (jobs[-1][0] / len(jobs))
-    arrivals = [t * scale for t, _, _ in jobs]
+def run_jobs(lucas, jobs, scale):
+    """Submit (t, p, K) jobs to a continuous-batching server at times
+    t * scale (all at once for scale 0); returns (results, stats)."""
     ids = {}
     with lucas.Server(chunk_iters=256) as srv:
         t0 = time.perf_counter()
-        for (t, p, _), k, a in zip(jobs, ks, arrivals):
-            dt = a - (time.perf_counter() - t0)
+        for t, p, k in jobs:
+            dt = t * scale - (time.perf_counter() - t0)
             if dt > 0:
                 time.sleep(dt)
             ids[srv.submit(p, k)] = (p, k)
-        srv.drain(timeout_s=600)
+        srv.drain(timeout_s=900)
         res = [srv.poll(j) for j in ids]
         st = srv.stats()
+    return res, st
+
+
+def serving(lucas, wl, count, load):
+    """The serving workload at its stated size: the first `count` Poisson
+    jobs of seed 5 (p log-uniform in [1e5, 4e7], K uniform in [5000, 50000],
+    each at its full K).  The server's capacity is measured first, as the
+    algorithmic HBM bytes (32 N per iteration) per second it sustains with
+    every job submitted at once; the Poisson arrivals are then spaced so
+    that the offered load is `load` x that capacity."""
+    jobs = wl.serving_jobs(seed=5, count=count)
+    ns = [lucas.default_N(p) for _, p, _ in jobs]
+    work = sum(32.0 * n * k for n, (_, _, k) in zip(ns, jobs))
+    log(f"serving: {count} jobs, {sum(k for _, _, k in jobs)} iterations, {work / 1e12:.2f} TB algorithmic")
+    res, st = run_jobs(lucas, jobs, 0.0)
+    span_cap = max(r["finish_s"] for r in res) - min(r["submit_s"] for r in res)
+    capacity = work / span_cap  # bytes/s
+    log(f"serving capacity: {span_cap:.2f} s for all jobs at once = {capacity / 1e12:.3f} TB/s")
+    span_arr = work / (load * capacity)
+    res, st = run_jobs(lucas, jobs, span_arr / jobs[-1][0])
     lat = sorted(r["finish_s"] - r["submit_s"] for r in res)
     span = max(r["finish_s"] for r in res) - min(r["submit_s"] for r in res)
 
     def pct(q):
         return lat[min(len(lat) - 1, int(round(q * (len(lat) - 1))))]
 
-    return {"workload": f"{count} Poisson jobs (seed 5), p log-uniform [1e5,4e7], K/{int(1 / k_scale)} "
-                        f"in [500,5000], offered load {load:.0%} of solo service",
-            "jobs": len(res), "p50_s": round(pct(0.5), 4), "p99_s": round(pct(0.99), 4),
-            "max_s": round(lat[-1], 4), "iters_per_s": round(sum(ks) / span, 1),
-            "peak_jobs": st["peak_jobs"], "max_roundoff": round(max(r["max_roundoff"] for r in res), 4),
+    offered = load * capacity
+    return {"workload": f"{count} Poisson jobs (seed 5), p log-uniform [1e5,4e7], N {min(ns)}..{max(ns)}, "
+                        f"K in [5000,50000] at full size",
+            "jobs": len(res), "iterations": sum(k for _, _, k in jobs),
+            "p50_s": round(pct(0.5), 4), "p99_s": round(pct(0.99), 4), "max_s": round(lat[-1], 4),
+            "iters_per_s": round(sum(k for _, _, k in jobs) / span, 1),
+            "capacity": {"all_at_once_s": round(span_cap, 3), "algorithmic_TBps": round(capacity / 1e12, 4)},
+            "offered_load": {"of_capacity": load, "algorithmic_TBps": round(offered / 1e12, 4),
+                             "of_hbm_peak": round(offered / (HBM_PEAK_GBS * 1e9), 4)},
+            "arrival_span_s": round(span_arr, 3), "peak_jobs": st["peak_jobs"],
+            "busy_s": round(st["busy_s"], 3), "span_s": round(span, 3),
+            "max_roundoff": round(max(r["max_roundoff"] for r in res), 4),
             "all_ok": all(r["status"] == 0 for r in res)}
 
 
@@ -187,23 +203,58 @@ def serving_capacity(lucas, wl, plain_its, iters=2000, count=64, seed=6, lo=100_
             "ratio": round(its / plain_its, 3), "all_ok": all(r["status"] == 0 for r in res)}
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(seconds_target):
+    """GMP Lucas-Lehmer (the oracle's squaring + Mersenne fold) at the
+    headline exponent on this host's cores, as child processes of the oracle
+    binary, each pinned to one core: one process alone (ms/iteration), then
+    one per core at once (aggregate iterations/s).  Runs before any GPU
+    call; bounded to about `seconds_target` seconds per phase."""
+    import subprocess
+
     import oracle
 
-    # calibrate on a short run, then time a bounded sample (~seconds_target s)
-    ms = oracle.gmp_time_ms(P_HEAD, 64, 20)
+    gmp = os.path.join(ROOT, "oracle", "gmp_ll")
+    if not os.path.exists(gmp):
+        oracle.build()
+    cores = sorted(os.sched_getaffinity(0))[:16]  # the box's CPU share for one GPU is 16
+
+    def run(core, warm, iters):
+        return subprocess.Popen([gmp, "--time", str(P_HEAD), str(warm), str(iters)], stdout=subprocess.PIPE,
+                                text=True, preexec_fn=lambda: os.sched_setaffinity(0, {core}))
+
+    # calibrate on a short run, then a bounded sample on one core
+    ms = float(run(cores[0], 64, 20).communicate()[0])
     iters = max(50, int(seconds_target * 1e3 / ms))
-    ms = oracle.gmp_time_ms(P_HEAD, 0, iters)
-    return {"value": round(ms, 4), "unit": "ms/LL-iter", "cores": 1, "kind": "port",
-            "sample": f"GMP {oracle.gmp_version()} LL squaring+fold at p={P_HEAD}, {iters} iterations "
-                      f"after 64 warm-up, 1 thread"}
+    ms1 = float(run(cores[0], 64, iters).communicate()[0])
+    # every core at once, same sample each
+    procs = [run(c, 64, iters) for c in cores]
+    per = [float(p.communicate()[0]) for p in procs]
+    agg = sum(1e3 / m for m in per)
+    return {"value": round(ms1, 4), "unit": "ms/LL-iter", "cores": 1, "kind": "port",
+            "sample": f"GMP {oracle.gmp_version()} LL squaring+fold at p={P_HEAD}, {iters} iterations after "
+                      f"64 warm-up, 1 pinned process",
+            "cpu_model": cpu_model(),
+            "aggregate": {"cores": len(cores), "iters_per_s": round(agg, 2),
+                          "ms_per_iter_per_core": round(sum(per) / len(per), 4),
+                          "sample": f"{len(cores)} pinned processes, one per core, {iters} iterations each"}}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20000)
-    ap.add_argument("--warmup", type=int, default=600)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--batch-iters", type=int, default=300)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -224,6 +275,13 @@ def main():
               "process per GPU and exponents shard across GPUs without communication.", file=sys.stderr)
         sys.exit(2)
 
+    # the CPU baseline runs first, before the engine's library is loaded or
+    # any GPU call is made (its workers are child processes pinned to cores)
+    cpu = None
+    if not (a.no_cpu or a.copy_calibration or a.profile_run):
+        cpu = cpu_baseline(a.cpu_seconds)
+        log(f"cpu baseline {cpu}")
+
     from gpulucas_amd import lucas, workloads as wl
 
     if a.copy_calibration:
@@ -237,7 +295,7 @@ def main():
             print(json.dumps({"profile_run": ctx.info, "max_roundoff": ctx.max_roundoff()}), flush=True)
         return
 
-    ms, err, col_us, row_us, info, raw = headline(lucas, a.steps, a.warmup)
+    ms, err, bits, col_us, row_us, info, raw = headline(lucas, a.steps, a.warmup)
     N = N_HEAD
     kern_bytes = 16 * N  # one read + one write of the 8N-byte state per kernel
     dom_name, dom_us = ("lucas_col_kernel", col_us) if col_us >= row_us else ("lucas_row_kernel", row_us)
@@ -260,12 +318,12 @@ def main():
                    "parallelism": "single exponent, 1 GPU"},
         "iters_per_s": round(1e3 / ms, 1),
         "max_roundoff": round(err, 5),
+        "product_bits": {"max_bit_length": bits[0], "max_leading_zeros": bits[1]},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": None,
                      "kernel_us": {"lucas_col_kernel": round(col_us, 3), "lucas_row_kernel": round(row_us, 3)},
-                     "kernel_timing": dict(raw, method="HIP events after each launch in the row/col sequence, "
-                                                       "marker excess over the iteration time removed"),
+                     "kernel_timing": raw,
                      "algorithmic_bytes_per_launch": kern_bytes},
         "fp64_tflops": round(5 * N * __import__("math").log2(N) / (ms * 1e-3) / 1e12, 4),
     }
@@ -275,8 +333,7 @@ def main():
         out["roofline"]["traffic"] = tr.get(key)
         out["roofline"]["traffic_by_kernel"] = tr
     if not a.no_long:
-        # 600 untimed iterations build both replay graphs (128 and 32 iterations)
-        out["long_transform"] = long_transform(lucas, P_LONG, N_LONG, a.long_steps, 600)
+        out["long_transform"] = long_transform(lucas, P_LONG, N_LONG, a.long_steps, 100)
     if not a.no_batch:
         bi, pi = batched_ratio(lucas, wl, a.batch_iters, 64)
         out["batched"] = {"workload": "64 primes in [1e6,5e6] (seed 4), default N each",
@@ -289,9 +346,9 @@ def main():
         if not a.no_serve:
             out["serving_capacity"] = serving_capacity(lucas, wl, pi)
     if not a.no_serve:
-        out["serving"] = serving(lucas, wl, a.serve_jobs, 0.1, 0.8)
-    if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
+        out["serving"] = serving(lucas, wl, a.serve_jobs, 0.8)
+    if cpu:
+        out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
 
 

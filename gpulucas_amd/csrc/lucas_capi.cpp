@@ -83,6 +83,35 @@ int ctx_new(const Plan& pl, int device, lucas_ctx** out) {
     return LUCAS_OK;
 }
 
+// The lengths findSignalSize tries (gpuLucas.cu:1072-1077): from p/20 up to
+// the smallest power of two >= p/16, but not past p/14.
+void autotune_window(int p, long long* lo, long long* hi) {
+    *lo = ((long long)p + 19) / 20;
+    long long pow2 = 1;
+    while (pow2 < (long long)p / 16) pow2 <<= 1;
+    *hi = std::min(pow2, (long long)p / 14);
+    if (*hi < *lo) *hi = *lo;
+}
+
+// Candidate lengths in [lo, hi]: every admissible N (2^a 3^b 5^c 7^d with a
+// tiling) at or under the calibrated bits-per-word line, plus the default
+// length of p, ascending.
+std::vector<int> autotune_candidates(int p, long long lo, long long hi) {
+    std::vector<int> c;
+    for (long long N = std::max(4LL, lo + (lo & 1)); N <= hi && N <= (long long)p; N += 2) {
+        if (!valid_length(N)) continue;
+        if ((double)p / (double)N > max_bits_per_word((int)N)) continue;
+        Plan pl;
+        if (make_plan(p, (int)N, 0, 0, &pl).empty()) c.push_back((int)N);
+    }
+    const int dn = default_length(p);
+    if (dn > 0 && std::find(c.begin(), c.end(), dn) == c.end()) {
+        c.push_back(dn);
+        std::sort(c.begin(), c.end());
+    }
+    return c;
+}
+
 double error_limit_of(const lucas_options* opt) {
     return (opt && opt->error_limit > 0) ? opt->error_limit : LUCAS_DEFAULT_ERROR_LIMIT;
 }
@@ -278,6 +307,8 @@ int lucas_device_copy_bench(size_t bytes, int reps, int device, double* us_per_c
 
 int lucas_default_N(int p) { return default_length(p); }
 
+double lucas_max_bits_per_word(int N) { return N > 0 ? max_bits_per_word(N) : 0.0; }
+
 int lucas_plan(int p, int N, lucas_plan_info* out) { return lucas_plan_shape(p, N, 0, 0, out); }
 
 int lucas_plan_shape(int p, int N, int A, int C, lucas_plan_info* out) {
@@ -396,6 +427,18 @@ int lucas_ctx_max_roundoff(lucas_ctx* ctx, int reset, double* out) {
     return job_read_maxerr(ctx->job, ctx->engine->stream(), reset, out);
 }
 
+int lucas_ctx_product_bits(lucas_ctx* ctx, int reset, int* max_bits, int* max_clz) {
+    if (!ctx) return LUCAS_ERR_INVALID_ARG;
+    return job_read_pbits(ctx->job, ctx->engine->stream(), reset, max_bits, max_clz);
+}
+
+int lucas_ctx_prepare(lucas_ctx* ctx, long long iters) {
+    if (!ctx || iters < 0) return LUCAS_ERR_INVALID_ARG;
+    int rc = ctx->engine->prepare_graphs(iters);
+    if (rc == LUCAS_OK) rc = ctx->engine->sync();
+    return rc;
+}
+
 int lucas_ctx_time_kernels(lucas_ctx* ctx, int iters, double* col_us, double* row_us) {
     if (!ctx || !col_us || !row_us) return LUCAS_ERR_INVALID_ARG;
     return ctx->engine->time_kernels(iters, col_us, row_us);
@@ -409,7 +452,8 @@ int lucas_ctx_phase_stamps(lucas_ctx* ctx, int iters, unsigned long long* col_st
 
 int lucas_ctx_time_iterations(lucas_ctx* ctx, long long iters, double* ms_per_iter) {
     if (!ctx || iters <= 0 || !ms_per_iter) return LUCAS_ERR_INVALID_ARG;
-    int rc = ctx->engine->sync();
+    int rc = ctx->engine->prepare_graphs(iters);  // graph builds stay outside the timed region
+    if (rc == LUCAS_OK) rc = ctx->engine->sync();
     if (rc != LUCAS_OK) return rc;
     const double t0 = now_s();
     rc = ctx->engine->step(iters);
@@ -449,6 +493,7 @@ int lucas_mersenne_test(int p, int N, const lucas_options* opt, lucas_result* ou
     if (!out || p < 7) return LUCAS_ERR_INVALID_ARG;
     memset(out, 0, sizeof(*out));
     const int device = opt ? opt->device : 0;
+    const double limit = error_limit_of(opt);
     lucas_ctx* ctx = nullptr;
     int rc = LUCAS_ERR_IO;
     if (opt && opt->resume && opt->checkpoint_path) {
@@ -465,30 +510,43 @@ int lucas_mersenne_test(int p, int N, const lucas_options* opt, lucas_result* ou
     const long long total = (long long)p - 2;
     const long long start = ctx->job->iter;
     const long long every = (opt && opt->checkpoint_every > 0 && opt->checkpoint_path) ? opt->checkpoint_every : 0;
+    // The roundoff gate is read after every chunk (the reference checked every
+    // p/1000 iterations and only printed a breach, gpuLucas.cu:1401-1407): a
+    // breach stops the run, and no checkpoint is written past it.
+    const long long gate = std::max(1000LL, std::min(1LL << 16, (long long)p / 1000));
+    double err = 0.0;
+    bool breach = false;
     const double t0 = now_s();
     while (rc == LUCAS_OK && ctx->job->iter < total) {
-        long long chunk = total - ctx->job->iter;
+        long long chunk = std::min(total - ctx->job->iter, gate - ctx->job->iter % gate);
         if (every) chunk = std::min(chunk, every - (ctx->job->iter % every));
-        chunk = std::min(chunk, 1LL << 20);
         rc = lucas_ctx_step(ctx, chunk);
+        if (rc == LUCAS_OK) rc = lucas_ctx_max_roundoff(ctx, 0, &err);
+        if (rc == LUCAS_OK && err >= limit) {
+            breach = true;
+            break;
+        }
         if (rc == LUCAS_OK && every && ctx->job->iter % every == 0 && ctx->job->iter < total)
             rc = lucas_ctx_save(ctx, opt->checkpoint_path);
     }
     const double t1 = now_s();
-    double err = 0.0;
     if (rc == LUCAS_OK) rc = lucas_ctx_max_roundoff(ctx, 0, &err);
     uint64_t r = 0;
     int zero = 0;
     if (rc == LUCAS_OK) rc = lucas_ctx_res64(ctx, &r, &zero);
+    int bits = 0, clz = 0;
+    if (rc == LUCAS_OK) rc = lucas_ctx_product_bits(ctx, 0, &bits, &clz);
     out->p = p;
     out->N = ctx->job->plan.N;
     out->res64 = r;
-    out->is_prime = zero;
+    out->is_prime = breach ? 0 : zero;
     out->iterations = ctx->job->iter - start;
     out->max_roundoff = err;
     out->ms_per_iter = out->iterations > 0 ? 1e3 * (t1 - t0) / (double)out->iterations : 0.0;
+    out->max_product_bits = bits;
+    out->max_product_clz = clz;
     lucas_ctx_destroy(ctx);
-    if (rc == LUCAS_OK && err >= error_limit_of(opt)) rc = LUCAS_ERR_ROUNDOFF;
+    if (rc == LUCAS_OK && (breach || err >= limit)) rc = LUCAS_ERR_ROUNDOFF;
     return rc;
 }
 
@@ -501,12 +559,15 @@ int lucas_error_test(int p, int N, int iterations, const lucas_options* opt, luc
     int rc = lucas_ctx_create(p, N, opt ? opt->device : 0, &ctx);
     if (rc != LUCAS_OK) return rc;
     double ms = 0.0, err = 0.0;
-    // a short warm-up launch outside the timed region builds the graph
+    // the replay graphs are built before the timed region (time_iterations),
+    // so ms/iteration covers the iterations only
     rc = lucas_ctx_time_iterations(ctx, iterations, &ms);
     if (rc == LUCAS_OK) rc = lucas_ctx_max_roundoff(ctx, 0, &err);
     uint64_t r = 0;
     int zero = 0;
     if (rc == LUCAS_OK) rc = lucas_ctx_res64(ctx, &r, &zero);
+    int bits = 0, clz = 0;
+    if (rc == LUCAS_OK) rc = lucas_ctx_product_bits(ctx, 0, &bits, &clz);
     out->p = p;
     out->N = ctx->job->plan.N;
     out->res64 = r;
@@ -514,6 +575,8 @@ int lucas_error_test(int p, int N, int iterations, const lucas_options* opt, luc
     out->iterations = iterations;
     out->max_roundoff = err;
     out->ms_per_iter = ms;
+    out->max_product_bits = bits;
+    out->max_product_clz = clz;
     lucas_ctx_destroy(ctx);
     if (rc == LUCAS_OK && err >= error_limit_of(opt)) rc = LUCAS_ERR_ROUNDOFF;
     return rc;
@@ -521,58 +584,67 @@ int lucas_error_test(int p, int N, int iterations, const lucas_options* opt, luc
 
 int lucas_autotune_N(int p, int iters, const lucas_options* opt, lucas_plan_info* out) {
     if (!out || p < 7) return LUCAS_ERR_INVALID_ARG;
-    if (iters < 20) iters = 50;
+    if (iters < 300) iters = 300;
     const double limit = error_limit_of(opt);
     const int device = opt ? opt->device : 0;
-    // candidates: admissible N between 15.5 and 20.5 bits per word, smallest first
-    std::vector<int> cands;
-    for (long long N = std::max(4LL, (long long)(p / 20.5)); N <= (long long)(p / 15.5) && cands.size() < 8; ++N) {
-        if (!valid_length(N)) continue;
-        Plan pl;
-        if (make_plan(p, (int)N, 0, 0, &pl).empty()) cands.push_back((int)N);
-    }
-    if (cands.empty()) return LUCAS_ERR_INVALID_ARG;
+    long long lo = 0, hi = 0;
+    autotune_window(p, &lo, &hi);
     double best_ms = 1e30;
     Plan best;
     int last_rc = LUCAS_ERR_ROUNDOFF;
-    for (int N : cands) {
-        Plan pl;
-        make_plan(p, N, 0, 0, &pl);
-        lucas_ctx* ctx = nullptr;
-        int rc = ctx_new(pl, device, &ctx);
-        if (rc != LUCAS_OK) {
-            last_rc = rc;
-            continue;
+    // findSignalSize widens the window by 8 x 256 words up to 10 times when
+    // no length passes (gpuLucas.cu:1166-1181); here by an eighth.
+    for (int retry = 0; retry <= 10 && best.N == 0; ++retry) {
+        for (int N : autotune_candidates(p, lo, hi)) {
+            Plan pl;
+            if (!make_plan(p, N, 0, 0, &pl).empty()) continue;
+            lucas_ctx* ctx = nullptr;
+            int rc = ctx_new(pl, device, &ctx);
+            if (rc != LUCAS_OK) {
+                last_rc = rc;
+                continue;
+            }
+            // steady-state roundoff from a pseudo-random balanced start
+            std::vector<int> w(N);
+            digit_widths(p, N, w.data());
+            std::vector<int32_t> d(N);
+            uint64_t s = 0x9E3779B97F4A7C15ULL ^ (uint64_t)p ^ ((uint64_t)N << 32);
+            for (int j = 0; j < N; ++j) {
+                s ^= s << 13;
+                s ^= s >> 7;
+                s ^= s << 17;
+                const int32_t half = 1 << (w[j] - 1);
+                d[j] = (int32_t)(s % (uint64_t)(2 * half)) - half;
+            }
+            double ms = 0, err = 0;
+            rc = lucas_ctx_set_digits(ctx, d.data(), 0);
+            if (rc == LUCAS_OK) rc = lucas_ctx_time_iterations(ctx, iters, &ms);
+            if (rc == LUCAS_OK) rc = lucas_ctx_max_roundoff(ctx, 0, &err);
+            lucas_ctx_destroy(ctx);
+            if (rc != LUCAS_OK) {
+                last_rc = rc;
+                continue;
+            }
+            if (err < limit && ms < best_ms) {
+                best_ms = ms;
+                best = pl;
+            }
         }
-        // steady-state roundoff from a pseudo-random balanced start
-        std::vector<int> w(N);
-        digit_widths(p, N, w.data());
-        std::vector<int32_t> d(N);
-        uint64_t s = 0x9E3779B97F4A7C15ULL ^ (uint64_t)p;
-        for (int j = 0; j < N; ++j) {
-            s ^= s << 13;
-            s ^= s >> 7;
-            s ^= s << 17;
-            const int32_t half = 1 << (w[j] - 1);
-            d[j] = (int32_t)(s % (uint64_t)(2 * half)) - half;
-        }
-        double ms = 0, err = 0;
-        rc = lucas_ctx_set_digits(ctx, d.data(), 0);
-        if (rc == LUCAS_OK) rc = lucas_ctx_time_iterations(ctx, 4, &ms);  // warm-up
-        if (rc == LUCAS_OK) rc = lucas_ctx_time_iterations(ctx, iters, &ms);
-        if (rc == LUCAS_OK) rc = lucas_ctx_max_roundoff(ctx, 0, &err);
-        lucas_ctx_destroy(ctx);
-        if (rc != LUCAS_OK) {
-            last_rc = rc;
-            continue;
-        }
-        if (err < limit && ms < best_ms) {
-            best_ms = ms;
-            best = pl;
-        }
+        lo = hi + 1;
+        hi += std::max(8LL * 256, hi / 8);
     }
     if (best.N == 0) return last_rc;
     fill_info(best, out);
+    return LUCAS_OK;
+}
+
+int lucas_autotune_candidates(int p, int* out, int cap, int* count) {
+    if (p < 7 || !count || cap < 0 || (cap > 0 && !out)) return LUCAS_ERR_INVALID_ARG;
+    long long lo = 0, hi = 0;
+    autotune_window(p, &lo, &hi);
+    const std::vector<int> c = autotune_candidates(p, lo, hi);
+    *count = (int)c.size();
+    for (int i = 0; i < cap && i < (int)c.size(); ++i) out[i] = c[i];
     return LUCAS_OK;
 }
 
@@ -682,7 +754,8 @@ int lucas_batch_get_digits(lucas_batch* b, int job, int32_t* digits) {
 
 int lucas_batch_time_iterations(lucas_batch* b, long long iters, double* ms_per_iter) {
     if (!b || iters <= 0 || !ms_per_iter) return LUCAS_ERR_INVALID_ARG;
-    int rc = b->engine->sync();
+    int rc = b->engine->prepare_graphs(iters);  // graph builds stay outside the timed region
+    if (rc == LUCAS_OK) rc = b->engine->sync();
     if (rc != LUCAS_OK) return rc;
     const double t0 = now_s();
     rc = b->engine->step(iters);
@@ -734,12 +807,17 @@ struct lucas_server {
     int worker_rc = LUCAS_OK;
 
     void run();
+    // <dir>/<p>_<id>.chk: two resident jobs of one exponent keep separate files
+    std::string checkpoint_path(int p, long long id) const {
+        return ckdir + "/" + std::to_string(p) + "_" + std::to_string(id) + ".chk";
+    }
 };
 
 void lucas_server::run() {
     hipSetDevice(opt.device);
     const int max_jobs = opt.max_jobs > 0 ? opt.max_jobs : 256;
     const long long chunk_max = opt.chunk_iters > 0 ? opt.chunk_iters : 256;
+    const double limit = opt.error_limit > 0 ? opt.error_limit : LUCAS_DEFAULT_ERROR_LIMIT;
     for (;;) {
         std::vector<Pending> admit;
         {
@@ -799,7 +877,7 @@ void lucas_server::run() {
                 if (!ckdir.empty() && opt.checkpoint_every > 0 && a.job->iter % opt.checkpoint_every < chunk) {
                     std::vector<int32_t> d(a.job->plan.N);
                     if (job_get_digits(a.job, engine->stream(), d.data()) == LUCAS_OK) {
-                        std::string path = ckdir + "/" + std::to_string(a.job->plan.p) + ".chk";
+                        std::string path = checkpoint_path(a.job->plan.p, a.id);
                         write_checkpoint(path.c_str(), a.job->plan.p, a.job->plan.N, a.job->iter, d.data());
                     }
                 }
@@ -814,6 +892,8 @@ void lucas_server::run() {
                 res.status = job_get_digits(a.job, engine->stream(), d.data());
                 if (res.status == LUCAS_OK) normalize_digits(a.job->plan.p, a.job->plan.N, d.data(), nullptr, &res.res64, &res.is_zero);
                 if (res.status == LUCAS_OK) res.status = job_read_maxerr(a.job, engine->stream(), 0, &res.max_roundoff);
+                // the gate of errorTrial / mersenneTest (gpuLucas.cu:904-907, 1239-1242)
+                if (res.status == LUCAS_OK && res.max_roundoff >= limit) res.status = LUCAS_ERR_ROUNDOFF;
             }
             const size_t bytes = job_bytes(a.job->plan);
             job_destroy(a.job);

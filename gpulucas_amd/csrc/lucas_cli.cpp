@@ -6,7 +6,10 @@
 //   3. error trial of clamp(p/10000, 100, 2500) iterations; refuse N when the
 //      roundoff reaches the limit (529-568)
 //   4. full test with periodic checkpoints and progress lines (1399-1484);
-//      SIGINT/SIGQUIT/SIGTERM/SIGTSTP checkpoint and exit (365-369, 598-606)
+//      SIGINT/SIGQUIT/SIGTERM/SIGTSTP checkpoint and exit (365-369, 598-606);
+//      the roundoff gate is read after every slice: a breach prints "Error
+//      limit exceeded" (also under -q), writes no result and exits 4 (the
+//      reference only printed it, 1404-1407)
 //   5. result line on stdout and appended to results.txt (1514-1547)
 #include <getopt.h>
 #include <signal.h>
@@ -115,7 +118,7 @@ int main(int argc, char** argv) {
             lucas_options o;
             memset(&o, 0, sizeof(o));
             o.device = device;
-            rc = lucas_autotune_N(p, 50, &o, &best);
+            rc = lucas_autotune_N(p, 300, &o, &best);
             N = rc == LUCAS_OK ? best.N : lucas_default_N(p);
             if (!quiet) printf("Signal length N = %d (%s)\n", N, rc == LUCAS_OK ? "measured" : "default rule");
         }
@@ -164,12 +167,21 @@ int main(int argc, char** argv) {
         long long next = std::min(total, (it / ckfreq + 1) * ckfreq);
         next = std::min(next, it + slice);
         rc = lucas_ctx_step(ctx, next - it);
+        double gate_err = 0;
+        if (rc == LUCAS_OK) rc = lucas_ctx_max_roundoff(ctx, 0, &gate_err);
         if (rc != LUCAS_OK) {
             fprintf(stderr, "step: %s\n", lucas_status_string(rc));
             lucas_ctx_destroy(ctx);
             return 1;
         }
         const long long now = lucas_ctx_iteration(ctx);
+        if (gate_err >= LUCAS_DEFAULT_ERROR_LIMIT) {
+            // no result line and no checkpoint past the breach
+            fprintf(stderr, "Error limit exceeded: max roundoff %.4f >= %.2f at iteration %lld (N = %d); "
+                    "no result written, rerun with a longer -f\n", gate_err, LUCAS_DEFAULT_ERROR_LIMIT, now, info.N);
+            lucas_ctx_destroy(ctx);
+            return 4;
+        }
         if (now % ckfreq == 0 && now < total) {
             if (lucas_ctx_save(ctx, ck.c_str()) != LUCAS_OK) fprintf(stderr, "warning: checkpoint write failed\n");
             if (!quiet) {
@@ -182,10 +194,12 @@ int main(int argc, char** argv) {
                 printf("[%4.1f%%] Iter %lld: %.4f ms/iter, roundoff %.4f, ETA %s", 100.0 * (double)now / (double)p,
                        now, ms, err, eta);
                 if (verbose) {
+                    // the reference's verbose "max Bit Vector" (gpuLucas.cu:911-916)
                     uint64_t r = 0;
-                    int z = 0;
+                    int z = 0, bits = 0, clz = 0;
                     lucas_ctx_res64(ctx, &r, &z);
-                    printf(", 0x%016llx", (unsigned long long)r);
+                    lucas_ctx_product_bits(ctx, 0, &bits, &clz);
+                    printf(", 0x%016llx, product bits %d, max Bit Vector %d", (unsigned long long)r, bits, clz);
                 }
                 printf("\n");
                 fflush(stdout);
@@ -215,7 +229,6 @@ int main(int argc, char** argv) {
         snprintf(line, sizeof(line), "M( %d )C, 0x%016llx, n = %d, %s", p, (unsigned long long)res, info.N,
                  lucas_version());
     printf("%s\n", line);
-    if (!quiet && err >= LUCAS_DEFAULT_ERROR_LIMIT) printf("warning: max roundoff %.4f\n", err);
     FILE* f = fopen("results.txt", "a");
     if (f) {
         fprintf(f, "%s\n", line);

@@ -81,6 +81,9 @@ struct LucasJobDesc {
     double2* spec2;
     int* digits;
     unsigned int* maxerr;   // max |x - rint(x)| since last reset, as float bits
+    // product-size diagnostic since last reset: [0] max bit length of the
+    // rounded products |c_j|, [1] max 64-bit leading-zero count of |c_j|
+    unsigned int* pbits;
 };
 
 // Dynamic LDS of the two kernels for one job, in 16-byte words.  Each kernel

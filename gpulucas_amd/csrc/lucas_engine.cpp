@@ -31,12 +31,16 @@ hipError_t configure_kernels(size_t max_lds);
 bool phase_stamps_built();
 hipError_t read_phase_stamps(unsigned long long* col, int ncol, unsigned long long* row, int nrow);
 
-// Iterations per replayed graph: a step of at least kLongStep iterations runs
-// graphs of 128 (measured 0.8 % faster per iteration than graphs of 32 at
-// N = 163840), which amortises their instantiation; its remainder, and every
-// shorter step (a server's chunks, whose job mix changes often), runs graphs
-// of 32, then single launches.
-static const int kGraphIters[2] = {128, 32};
+// Replay graphs.  A step of k iterations is the head graph (column kernel,
+// digits in), k - 1 steady-state [row ; col] pairs and the tail graph (row,
+// column kernel with digits out).  The pairs run as graphs of 2^l pairs,
+// l = 0 .. kLevels - 1: graphs of 128 while at least 128 remain in a step of
+// kLongStep or more (measured 0.8 % faster per iteration than graphs of 32 at
+// N = 163840), otherwise the binary digits of the remainder, largest first.
+// So every iteration of every step is graph-launched; a step of k < 128 pairs
+// launches at most 2 + log2(k) graphs.  Graphs are built on first use, or
+// ahead of a timed region by prepare_graphs().
+static const int kLevels = 8;  // 1, 2, 4, .., 128 pairs
 static const long long kLongStep = 512;
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -125,6 +129,7 @@ int job_create(const Plan& plan, int device, hipStream_t stream, Job** out) {
     d.twsA = (const double2*)(m + o_twsA);
     d.twsB = (const double2*)(m + o_twsB);
     d.maxerr = (unsigned int*)(m + o_err);
+    d.pbits = d.maxerr + 1;  // same 256-byte block, zeroed with it
     job->iter = 0;
     *out = job;
     return LUCAS_OK;
@@ -149,6 +154,21 @@ int job_set_digits(Job* job, hipStream_t stream, const int32_t* digits) {
     hipError_t e = hipMemcpyAsync(job->desc.digits, digits, sizeof(int) * job->plan.N, hipMemcpyHostToDevice, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     return e == hipSuccess ? LUCAS_OK : LUCAS_ERR_DEVICE;
+}
+
+int job_read_pbits(Job* job, hipStream_t stream, int reset, int* max_bits, int* max_clz) {
+    unsigned int v[2] = {0, 0};
+    hipError_t e = hipMemcpyAsync(v, job->desc.pbits, sizeof(v), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e == hipSuccess && reset) {
+        const unsigned int zero[2] = {0, 0};
+        e = hipMemcpyAsync(job->desc.pbits, zero, sizeof(zero), hipMemcpyHostToDevice, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    }
+    if (e != hipSuccess) return LUCAS_ERR_DEVICE;
+    if (max_bits) *max_bits = (int)v[0];
+    if (max_clz) *max_clz = (int)v[1];
+    return LUCAS_OK;
 }
 
 int job_read_maxerr(Job* job, hipStream_t stream, int reset, double* out) {
@@ -300,7 +320,7 @@ int Engine::launch_row() {
 }
 
 void Engine::destroy_graphs() {
-    for (int l = 0; l < 2; ++l) {
+    for (int l = 0; l < kGraphSlots; ++l) {
         if (gexec_[l]) hipGraphExecDestroy(gexec_[l]);
         if (graph_[l]) hipGraphDestroy(graph_[l]);
         gexec_[l] = nullptr;
@@ -308,30 +328,63 @@ void Engine::destroy_graphs() {
     }
 }
 
-int Engine::build_graph(int level) {
-    if (gexec_[level]) return LUCAS_OK;
+// Slot l < kLevels: 2^l steady-state [row ; col] pairs; kHead: the column
+// kernel reading digits; kTail: row + column kernel writing digits.
+int Engine::build_graph(int slot) {
+    if (gexec_[slot]) return LUCAS_OK;
     hipError_t e = hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return LUCAS_ERR_DEVICE;
     int rc = LUCAS_OK;
-    for (int i = 0; i < kGraphIters[level] && rc == LUCAS_OK; ++i) {
+    if (slot == kHead) {
+        rc = launch_col(LUCAS_IN_DIGITS);
+    } else if (slot == kTail) {
         rc = launch_row();
-        if (rc == LUCAS_OK) rc = launch_col(0);
+        if (rc == LUCAS_OK) rc = launch_col(LUCAS_OUT_DIGITS);
+    } else {
+        for (int i = 0; i < (1 << slot) && rc == LUCAS_OK; ++i) {
+            rc = launch_row();
+            if (rc == LUCAS_OK) rc = launch_col(0);
+        }
     }
-    e = hipStreamEndCapture(stream_, &graph_[level]);
+    e = hipStreamEndCapture(stream_, &graph_[slot]);
     if (rc != LUCAS_OK || e != hipSuccess) {
-        if (graph_[level]) hipGraphDestroy(graph_[level]);
-        graph_[level] = nullptr;
+        if (graph_[slot]) hipGraphDestroy(graph_[slot]);
+        graph_[slot] = nullptr;
         (void)hipGetLastError();
         return LUCAS_ERR_DEVICE;
     }
-    e = hipGraphInstantiate(&gexec_[level], graph_[level], nullptr, nullptr, 0);
+    e = hipGraphInstantiate(&gexec_[slot], graph_[slot], nullptr, nullptr, 0);
     if (e != hipSuccess) {
-        hipGraphDestroy(graph_[level]);
-        graph_[level] = nullptr;
-        gexec_[level] = nullptr;
+        hipGraphDestroy(graph_[slot]);
+        graph_[slot] = nullptr;
+        gexec_[slot] = nullptr;
         return LUCAS_ERR_DEVICE;
     }
     return LUCAS_OK;
+}
+
+// Graph slots a step of `iters` iterations launches, in order.
+static void step_plan(long long iters, std::vector<int>* slots) {
+    slots->clear();
+    slots->push_back(Engine::kHead);
+    long long rem = iters - 1;
+    const int top = kLevels - 1;
+    if (iters >= kLongStep)
+        for (; rem >= (1LL << top); rem -= 1LL << top) slots->push_back(top);
+    for (int l = top; l >= 0; --l)
+        for (; rem >= (1LL << l); rem -= 1LL << l) slots->push_back(l);
+    slots->push_back(Engine::kTail);
+}
+
+int Engine::prepare_graphs(long long iters) {
+    if (iters <= 0) return iters == 0 ? LUCAS_OK : LUCAS_ERR_INVALID_ARG;
+    if (jobs_.empty()) return LUCAS_OK;
+    hipSetDevice(device_);
+    int rc = prepare();
+    std::vector<int> slots;
+    step_plan(iters, &slots);
+    for (size_t i = 0; i < slots.size() && rc == LUCAS_OK; ++i) rc = build_graph(slots[i]);
+    return rc;
 }
 
 int Engine::step(long long iters) {
@@ -339,24 +392,12 @@ int Engine::step(long long iters) {
     if (jobs_.empty()) return LUCAS_OK;
     hipSetDevice(device_);
     int rc = prepare();
-    if (rc != LUCAS_OK) return rc;
-    rc = launch_col(LUCAS_IN_DIGITS);
-    long long rem = iters - 1;
-    for (int l = 0; l < 2; ++l) {
-        if (rc != LUCAS_OK || rem < kGraphIters[l] || (l == 0 && iters < kLongStep)) continue;
-        rc = build_graph(l);
-        while (rc == LUCAS_OK && rem >= kGraphIters[l]) {
-            if (hipGraphLaunch(gexec_[l], stream_) != hipSuccess) rc = LUCAS_ERR_DEVICE;
-            rem -= kGraphIters[l];
-        }
+    std::vector<int> slots;
+    step_plan(iters, &slots);
+    for (size_t i = 0; i < slots.size() && rc == LUCAS_OK; ++i) {
+        rc = build_graph(slots[i]);
+        if (rc == LUCAS_OK && hipGraphLaunch(gexec_[slots[i]], stream_) != hipSuccess) rc = LUCAS_ERR_DEVICE;
     }
-    while (rc == LUCAS_OK && rem > 0) {
-        rc = launch_row();
-        if (rc == LUCAS_OK) rc = launch_col(0);
-        --rem;
-    }
-    if (rc == LUCAS_OK) rc = launch_row();
-    if (rc == LUCAS_OK) rc = launch_col(LUCAS_OUT_DIGITS);
     if (rc == LUCAS_OK)
         for (Job* j : jobs_) j->iter += iters;
     return rc;

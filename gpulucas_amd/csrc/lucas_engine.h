@@ -28,6 +28,8 @@ void job_destroy(Job* job);
 int job_get_digits(Job* job, hipStream_t stream, int32_t* digits);
 int job_set_digits(Job* job, hipStream_t stream, const int32_t* digits);
 int job_read_maxerr(Job* job, hipStream_t stream, int reset, double* out);
+// Product-size diagnostic since the last reset (see LucasJobDesc::pbits).
+int job_read_pbits(Job* job, hipStream_t stream, int reset, int* max_bits, int* max_clz);
 
 // A set of jobs advanced together: one launch of each kernel per iteration
 // covers every job (workgroup -> (job, tile) maps built on the host).
@@ -43,7 +45,13 @@ public:
     const std::vector<Job*>& jobs() const { return jobs_; }
 
     // Advances every job by `iters` (>= 1) iterations; digits in, digits out.
+    // Every launch is a replay-graph launch.
     int step(long long iters);
+    // Builds (without running) every replay graph a step of `iters`
+    // iterations launches, so that a later timed step builds nothing.
+    int prepare_graphs(long long iters);
+    // Graph slots: 0 .. kLevels-1 hold 2^slot steady-state iterations.
+    static const int kHead = 8, kTail = 9, kGraphSlots = 10;
     // Mean microseconds of each kernel launch over `iters` iterations in the
     // step() order, from events after every launch (digits are not touched).
     int time_kernels(int iters, double* col_us, double* row_us);
@@ -58,7 +66,7 @@ private:
     int prepare();
     int launch_col(int mode);
     int launch_row();
-    int build_graph(int level);
+    int build_graph(int slot);
 
     // Jobs whose kernel has the same workgroup size and wave factor share
     // one launch; column and row launches are grouped separately.
@@ -82,10 +90,9 @@ private:
     bool single_ = false;  // one job: launch the by-value-descriptor kernels with one_
     LucasJobDesc one_;
     Group colg_[kGroups], rowg_[kGroups];
-    // steady-state iterations replayed as graphs of kGraphIters[level]
-    // iterations, built on first use
-    hipGraph_t graph_[2] = {nullptr, nullptr};
-    hipGraphExec_t gexec_[2] = {nullptr, nullptr};
+    // replay graphs by slot (see kHead, kTail), built on first use
+    hipGraph_t graph_[kGraphSlots] = {};
+    hipGraphExec_t gexec_[kGraphSlots] = {};
     void destroy_graphs();
 };
 

@@ -523,24 +523,49 @@ __device__ __forceinline__ double2 outer_twiddle(const LucasJobDesc& J, const do
     return cmul(gload2(twA, hi), gload2(twMB, lo));
 }
 
-// Wave max of the roundoff, folded into the job's maximum.  err >= 0, so
-// float order is the order of the bit patterns: DPP quad permutes and row
+// Maximum over a wave of a non-negative value: DPP quad permutes and row
 // rotates take the max within each 16-lane row (no LDS crossbar), four lane
-// reads combine the rows.  One atomic per wave, issued only when the wave
-// beats `seen` (the job's max as read at kernel start): after the first
-// iterations a new maximum is rare, so there is no barrier and almost no
-// atomic traffic at the end of the kernel.
-__device__ __forceinline__ void wave_max_err(float err, unsigned int seen, unsigned int* dst) {
-    unsigned int x = __float_as_uint(err);
+// reads combine the rows.  The result is wave-uniform.
+__device__ __forceinline__ unsigned int wave_max_u32(unsigned int x) {
     x = max(x, (unsigned int)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, false));   // lane ^ 1
     x = max(x, (unsigned int)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, false));   // lane ^ 2
     x = max(x, (unsigned int)__builtin_amdgcn_update_dpp(0, (int)x, 0x124, 0xF, 0xF, false));  // row rotate 4
     x = max(x, (unsigned int)__builtin_amdgcn_update_dpp(0, (int)x, 0x128, 0xF, 0xF, false));  // row rotate 8
-    const unsigned int m = max(max((unsigned int)__builtin_amdgcn_readlane((int)x, 0),
-                                   (unsigned int)__builtin_amdgcn_readlane((int)x, 16)),
-                               max((unsigned int)__builtin_amdgcn_readlane((int)x, 32),
-                                   (unsigned int)__builtin_amdgcn_readlane((int)x, 48)));
+    return max(max((unsigned int)__builtin_amdgcn_readlane((int)x, 0), (unsigned int)__builtin_amdgcn_readlane((int)x, 16)),
+               max((unsigned int)__builtin_amdgcn_readlane((int)x, 32), (unsigned int)__builtin_amdgcn_readlane((int)x, 48)));
+}
+
+// Wave max of the roundoff, folded into the job's maximum.  err >= 0, so
+// float order is the order of the bit patterns.  One atomic per wave, issued
+// only when the wave beats `seen` (the job's max as read at kernel start):
+// after the first iterations a new maximum is rare, so there is no barrier
+// and almost no atomic traffic at the end of the kernel.
+__device__ __forceinline__ void wave_max_err(float err, unsigned int seen, unsigned int* dst) {
+    const unsigned int m = wave_max_u32(__float_as_uint(err));
     if ((threadIdx.x & 63) == 0 && m > seen) atomicMax(dst, m);
+}
+
+// Bit length of a non-negative integer-valued double (0 for 0), capped at 64.
+__device__ __forceinline__ unsigned int bit_length(double v) {
+    const unsigned int e = ((unsigned int)__double2hiint(v) >> 20) & 0x7ff;
+    return e ? min(e - 1022u, 64u) : 0u;
+}
+
+// Product-size diagnostic of the rounded convolution outputs c_j (the
+// reference's computeMaxBitVector, gpuLucas.cu:826-836, reduced by
+// findMaxErrorHOST, 811-819): dst[0] = max bit length of |c_j|; dst[1] = max
+// over j of the 64-bit leading-zero count of |c_j|, the figure the reference
+// prints as "max Bit Vector" (64 minus the smallest bit length).  pmax and
+// pmin are the work-item's largest and smallest |c_j|; seen0, seen1 the
+// job's values as read at kernel start.
+__device__ __forceinline__ void wave_max_bits(double pmax, double pmin, unsigned int seen0, unsigned int seen1,
+                                              unsigned int* dst) {
+    const unsigned int b = wave_max_u32(bit_length(pmax));
+    const unsigned int z = wave_max_u32(64u - bit_length(pmin));
+    if ((threadIdx.x & 63) == 0) {
+        if (b > seen0) atomicMax(dst, b);
+        if (z > seen1) atomicMax(dst + 1, z);
+    }
 }
 
 // Carry-save restoration of one digit (reference llintToIrrBal<n>,
@@ -652,10 +677,13 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc& J, int tile,
     // the job's maximum so far: a relaxed device-coherent load, first needed
     // by the wave reductions at the end, so no wait follows it here
     const unsigned int seen = __hip_atomic_load(J.maxerr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned int seen_b = __hip_atomic_load(J.pbits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned int seen_z = __hip_atomic_load(J.pbits + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const double2* const twA = J.twA;
     const double2* const twMB = J.twMB;
     const bool in_digits = (mode & LUCAS_IN_DIGITS) != 0;
     float err = 0.0f;
+    double pmax = 0.0, pmin = 0x1p60;  // largest / smallest |c_j| of this work-item
 
     if (!in_digits) {
         // load the tile with the inverse outer twiddle omega_M^(-c*r), conjugated
@@ -729,6 +757,8 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc& J, int tile,
             if (gc == 0 && r == 0) ve -= 2.0;
             const double re = rint(ve), ro = rint(vo);
             err = fmaxf(err, (float)fmax(fabs(ve - re), fabs(vo - ro)));
+            pmax = fmax(pmax, fmax(fabs(re), fabs(ro)));
+            pmin = fmin(pmin, fmin(fabs(re), fabs(ro)));
             return make_double2(pack_product(re, (rje >= N ? rje - N : rje) < J.b),
                                 pack_product(ro, (rjo >= N ? rjo - N : rjo) < J.b));
         };
@@ -783,6 +813,7 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc& J, int tile,
         }
         if (mode & LUCAS_OUT_DIGITS) {
             wave_max_err(err, seen, J.maxerr);
+            wave_max_bits(pmax, pmin, seen_b, seen_z, J.pbits);
             return;
         }
     }
@@ -803,6 +834,7 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc& J, int tile,
     }
     LUCAS_STAMP_COL(5);  // stores issued
     wave_max_err(err, seen, J.maxerr);
+    if (!in_digits) wave_max_bits(pmax, pmin, seen_b, seen_z, J.pbits);
 }
 
 // ---------------------------------------------------------------------------

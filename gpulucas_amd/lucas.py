@@ -56,7 +56,8 @@ class PlanInfo(C.Structure):
 
 class Result(C.Structure):
     _fields_ = [("p", C.c_int), ("N", C.c_int), ("is_prime", C.c_int), ("res64", C.c_uint64),
-                ("iterations", C.c_longlong), ("max_roundoff", C.c_double), ("ms_per_iter", C.c_double)]
+                ("iterations", C.c_longlong), ("max_roundoff", C.c_double), ("ms_per_iter", C.c_double),
+                ("max_product_bits", C.c_int), ("max_product_clz", C.c_int)]
 
     def as_dict(self):
         d = {f: getattr(self, f) for f, _ in self._fields_}
@@ -71,7 +72,8 @@ class Options(C.Structure):
 
 class ServerOptions(C.Structure):
     _fields_ = [("device", C.c_int), ("hbm_budget_bytes", C.c_size_t), ("max_jobs", C.c_int),
-                ("chunk_iters", C.c_int), ("checkpoint_dir", C.c_char_p), ("checkpoint_every", C.c_longlong)]
+                ("chunk_iters", C.c_int), ("checkpoint_dir", C.c_char_p), ("checkpoint_every", C.c_longlong),
+                ("error_limit", C.c_double)]
 
 
 class JobResult(C.Structure):
@@ -93,6 +95,7 @@ _SIGS = {
     "lucas_status_string": (C.c_char_p, [C.c_int]),
     "lucas_device_count": (C.c_int, [_P(C.c_int)]),
     "lucas_default_N": (C.c_int, [C.c_int]),
+    "lucas_max_bits_per_word": (C.c_double, [C.c_int]),
     "lucas_device_copy_bench": (C.c_int, [C.c_size_t, C.c_int, C.c_int, _P(C.c_double)]),
     "lucas_plan": (C.c_int, [C.c_int, C.c_int, _P(PlanInfo)]),
     "lucas_plan_shape": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _P(PlanInfo)]),
@@ -112,6 +115,8 @@ _SIGS = {
     "lucas_ctx_set_digits": (C.c_int, [C.c_void_p, C.c_void_p, C.c_longlong]),
     "lucas_ctx_res64": (C.c_int, [C.c_void_p, _P(C.c_uint64), _P(C.c_int)]),
     "lucas_ctx_max_roundoff": (C.c_int, [C.c_void_p, C.c_int, _P(C.c_double)]),
+    "lucas_ctx_product_bits": (C.c_int, [C.c_void_p, C.c_int, _P(C.c_int), _P(C.c_int)]),
+    "lucas_ctx_prepare": (C.c_int, [C.c_void_p, C.c_longlong]),
     "lucas_ctx_time_kernels": (C.c_int, [C.c_void_p, C.c_int, _P(C.c_double), _P(C.c_double)]),
     "lucas_ctx_phase_stamps": (C.c_int, [C.c_void_p, C.c_int, _P(C.c_ulonglong), _P(C.c_ulonglong), C.c_int,
                                          _P(C.c_int), _P(C.c_int)]),
@@ -121,6 +126,7 @@ _SIGS = {
     "lucas_mersenne_test": (C.c_int, [C.c_int, C.c_int, _P(Options), _P(Result)]),
     "lucas_error_test": (C.c_int, [C.c_int, C.c_int, C.c_int, _P(Options), _P(Result)]),
     "lucas_autotune_N": (C.c_int, [C.c_int, C.c_int, _P(Options), _P(PlanInfo)]),
+    "lucas_autotune_candidates": (C.c_int, [C.c_int, C.c_void_p, C.c_int, _P(C.c_int)]),
     "lucas_batch_create": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, _P(C.c_void_p)]),
     "lucas_batch_destroy": (C.c_int, [C.c_void_p]),
     "lucas_batch_step": (C.c_int, [C.c_void_p, C.c_longlong]),
@@ -185,6 +191,11 @@ def device_copy_bench(nbytes: int, reps: int = 20, device: int = 0) -> float:
 
 def default_N(p: int) -> int:
     return _lib().lucas_default_N(p)
+
+
+def max_bits_per_word(N: int) -> float:
+    """Calibrated bits-per-word line of the default length rule at length N."""
+    return _lib().lucas_max_bits_per_word(N)
 
 
 def plan(p: int, N: int = 0, A: int = 0, C_: int = 0) -> dict:
@@ -299,6 +310,18 @@ class Context:
         _check(_lib().lucas_ctx_max_roundoff(self._h, int(reset), C.byref(v)), "max_roundoff")
         return v.value
 
+    def product_bits(self, reset: bool = False) -> tuple[int, int]:
+        """(largest bit length of a rounded product |c_j|, the reference's
+        "max Bit Vector" = largest 64-bit leading-zero count of |c_j|) over
+        every iteration since the last reset."""
+        b, z = C.c_int(0), C.c_int(0)
+        _check(_lib().lucas_ctx_product_bits(self._h, int(reset), C.byref(b), C.byref(z)), "product_bits")
+        return b.value, z.value
+
+    def prepare(self, iters: int):
+        """Build the replay graphs a step of `iters` iterations launches."""
+        _check(_lib().lucas_ctx_prepare(self._h, iters), f"prepare({iters})")
+
     def time_kernels(self, iters: int = 100) -> tuple[float, float]:
         a, b = C.c_double(0), C.c_double(0)
         _check(_lib().lucas_ctx_time_kernels(self._h, iters, C.byref(a), C.byref(b)), "time_kernels")
@@ -365,7 +388,16 @@ def errorTest(p: int, N: int = 0, iterations: int = 0, **kw) -> float:  # noqa: 
     return -1.0 if d["status"] == ERR_ROUNDOFF else d["ms_per_iter"]
 
 
-def autotune_N(p: int, iters: int = 50, device: int = 0, error_limit: float = 0.0) -> dict:
+def autotune_candidates(p: int) -> list[int]:
+    """Lengths the first window of autotune_N measures (host only)."""
+    n = C.c_int(0)
+    _check(_lib().lucas_autotune_candidates(p, None, 0, C.byref(n)), f"autotune_candidates({p})")
+    arr = np.zeros(n.value, dtype=np.int32)
+    _check(_lib().lucas_autotune_candidates(p, _ptr(arr), n.value, C.byref(n)), "autotune_candidates")
+    return [int(x) for x in arr]
+
+
+def autotune_N(p: int, iters: int = 300, device: int = 0, error_limit: float = 0.0) -> dict:
     info = PlanInfo()
     opt = _options(device, error_limit)
     _check(_lib().lucas_autotune_N(p, iters, C.byref(opt), C.byref(info)), f"autotune_N({p})")
@@ -435,9 +467,9 @@ class Server:
     """Continuous-batching request loop (device-resident state per job)."""
 
     def __init__(self, device: int = 0, hbm_budget_bytes: int = 0, max_jobs: int = 0, chunk_iters: int = 0,
-                 checkpoint_dir: str | None = None, checkpoint_every: int = 0):
+                 checkpoint_dir: str | None = None, checkpoint_every: int = 0, error_limit: float = 0.0):
         opt = ServerOptions(device, hbm_budget_bytes, max_jobs, chunk_iters,
-                            checkpoint_dir.encode() if checkpoint_dir else None, checkpoint_every)
+                            checkpoint_dir.encode() if checkpoint_dir else None, checkpoint_every, error_limit)
         self._h = C.c_void_p()
         _check(_lib().lucas_server_start(C.byref(opt), C.byref(self._h)), "server_start")
 

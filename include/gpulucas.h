@@ -70,6 +70,13 @@ typedef struct {
     long long iterations;  /* iterations performed in this call (from s_0 unless resumed) */
     double max_roundoff;   /* over every iteration of the call */
     double ms_per_iter;    /* wall-clock mean */
+    /* Product-size diagnostic over every iteration of the call
+     * (computeMaxBitVector, gpuLucas.cu:826-836, printed at 913-916): the
+     * largest bit length of a rounded convolution output |c_j|, and the
+     * reference's "max Bit Vector", the largest 64-bit leading-zero count of
+     * |c_j| (64 minus the smallest bit length). */
+    int max_product_bits;
+    int max_product_clz;
 } lucas_result;
 
 typedef struct {
@@ -89,6 +96,10 @@ int lucas_device_count(int* count);
 /* Default signal length for p (replaces findSignalSize, gpuLucas.cu:1064-1187,
  * with a table rule; lucas_autotune_N measures on the GPU). */
 int lucas_default_N(int p);
+/* Bits per word the default rule allows at length N: the calibrated line on
+ * which the roundoff from a random residue stays near 0.12 (no reference
+ * counterpart; the reference's window was p/20 .. p/14, gpuLucas.cu:1072-1077). */
+double lucas_max_bits_per_word(int N);
 /* Shape of (p, N); N == 0 chooses lucas_default_N(p).  Replaces the setup of
  * initConstantSymbols / setSliceAndDice (gpuLucas.cu:222-237, 323-343). */
 int lucas_plan(int p, int N, lucas_plan_info* out);
@@ -133,6 +144,18 @@ int lucas_ctx_set_digits(lucas_ctx* ctx, const int32_t* digits, long long iter);
 int lucas_ctx_res64(lucas_ctx* ctx, uint64_t* res64, int* is_zero);
 /* Max roundoff since the last reset (findMaxErrorHOST, gpuLucas.cu:811-819). */
 int lucas_ctx_max_roundoff(lucas_ctx* ctx, int reset, double* out);
+/* Product-size diagnostic since the last reset, over every iteration
+ * (computeMaxBitVector, gpuLucas.cu:826-836, reduced by findMaxErrorHOST,
+ * 811-819): *max_bits = largest bit length of a rounded convolution output
+ * |c_j| (the margin to the 53-bit FP64 mantissa); *max_clz = largest 64-bit
+ * leading-zero count of |c_j|, the value the reference prints as "max Bit
+ * Vector".  Either pointer may be NULL. */
+int lucas_ctx_product_bits(lucas_ctx* ctx, int reset, int* max_bits, int* max_clz);
+/* Builds the replay graphs a step of `iters` iterations launches without
+ * running them (no reference counterpart: the reference launched every
+ * kernel from the host, gpuLucas.cu:845-942), so that a later timed step of
+ * that length instantiates nothing. */
+int lucas_ctx_prepare(lucas_ctx* ctx, long long iters);
 /* Mean time per kernel launch over `iters` iterations launched as in
  * lucas_ctx_step (row kernel, column kernel, alternating), from HIP events
  * recorded on the context's stream after every launch (microseconds; each
@@ -164,20 +187,33 @@ int lucas_ctx_load(const char* path, int device, lucas_ctx** out);
 /* ---- reference entry points ------------------------------------------- */
 /* Full test: p-2 iterations from s_0 = 4 (or from a checkpoint), roundoff
  * tracked on every iteration, Res64 + prime flag (mersenneTest,
- * gpuLucas.cu:1358-1551).  Returns LUCAS_OK, or LUCAS_ERR_ROUNDOFF after the
- * run when the gate was exceeded (the reference only printed it,
- * gpuLucas.cu:1404-1407); `out` is filled in both cases. */
+ * gpuLucas.cu:1358-1551).  The roundoff gate is read after every chunk of
+ * max(1000, min(65536, p/1000)) iterations (the reference read it every p/1000
+ * and only printed a breach, gpuLucas.cu:1401-1407): a breach stops the run
+ * and returns LUCAS_ERR_ROUNDOFF with is_prime = 0 and `iterations` the
+ * count reached; no checkpoint is written past a breach.  Otherwise
+ * LUCAS_OK.  `out` is filled in both cases. */
 int lucas_mersenne_test(int p, int N, const lucas_options* opt, lucas_result* out);
 /* Timing + roundoff trial of `iterations` (clamped to >= 100) from s_0 = 4
- * (errorTrial, gpuLucas.cu:1192-1282).  out->ms_per_iter is the mean time;
- * returns LUCAS_ERR_ROUNDOFF when max roundoff >= error_limit (the reference
- * returned -1).  iterations <= 0 uses clamp(p/10000, 100, 2500)
+ * (errorTrial, gpuLucas.cu:1192-1282).  out->ms_per_iter is the mean time
+ * of the iterations alone (the replay graphs are built before the timed
+ * region); returns LUCAS_ERR_ROUNDOFF when max roundoff >= error_limit (the
+ * reference returned -1).  iterations <= 0 uses clamp(p/10000, 100, 2500)
  * (gpuLucas.cu:529-541). */
 int lucas_error_test(int p, int N, int iterations, const lucas_options* opt, lucas_result* out);
-/* Measured choice of N for p (findSignalSize, gpuLucas.cu:1064-1187): times
- * `iters` iterations per candidate and keeps the fastest with roundoff below
- * the limit.  Writes the chosen N and its plan. */
+/* Measured choice of N for p (findSignalSize, gpuLucas.cu:1064-1187).  The
+ * candidates are the reference's window, N from p/20 to the smallest power of
+ * two >= p/16 but at most p/14 (1072-1077), restricted to lengths with a
+ * tiling whose bits per word stay at or under the calibrated line, plus
+ * lucas_default_N(p).  Each runs max(iters, 300) iterations from
+ * pseudo-random balanced digits (graphs built beforehand); the fastest whose
+ * roundoff stays below the limit wins.  When none passes, the window widens
+ * past its top, up to 10 times (1166-1181).  Writes the chosen N and its
+ * plan. */
 int lucas_autotune_N(int p, int iters, const lucas_options* opt, lucas_plan_info* out);
+/* The first window's candidate lengths of lucas_autotune_N (host only):
+ * fills up to `cap` entries, ascending, and writes the total into *count. */
+int lucas_autotune_candidates(int p, int* out, int cap, int* count);
 
 /* Measured choice of tiling (radix split A x B, tile width C, workgroup
  * size) for (p, N): times `iters` iterations of up to `max_candidates`
@@ -202,8 +238,9 @@ typedef struct {
     size_t hbm_budget_bytes;  /* admission limit on resident job state (0 = 80% of free) */
     int max_jobs;             /* slots (0 = 256) */
     int chunk_iters;          /* iterations per scheduling tick (0 = 256) */
-    const char* checkpoint_dir; /* NULL: no checkpoints */
+    const char* checkpoint_dir; /* NULL: no checkpoints; else <dir>/<p>_<id>.chk per job */
     long long checkpoint_every;
+    double error_limit;       /* roundoff gate per job; <= 0: LUCAS_DEFAULT_ERROR_LIMIT */
 } lucas_server_options;
 
 typedef struct {
@@ -211,7 +248,8 @@ typedef struct {
     int p, N;
     long long iterations;
     int done;               /* 1 when finished */
-    int status;             /* LUCAS_OK or error */
+    int status;             /* LUCAS_OK, LUCAS_ERR_ROUNDOFF (max_roundoff reached
+                               the limit: the residue is not trustworthy), or error */
     uint64_t res64;
     int is_zero;
     double max_roundoff;

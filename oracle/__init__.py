@@ -64,6 +64,8 @@ def _ibd():
         lib.ibdwt_widths_double.restype = None
         lib.ibdwt_tables.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         lib.ibdwt_carry_depth.argtypes = [C.c_int]
+        lib.ibdwt_product_bits.argtypes = [C.c_int, C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        lib.ibdwt_product_bits.restype = None
         _IBD = lib
     return _IBD
 
@@ -138,6 +140,15 @@ def ibdwt_conv_fft(p: int, N: int, x):
     err = C.c_double(0)
     assert _ibd().ibdwt_conv_fft(p, N, _p(xx), _p(c), C.byref(err)) == 0
     return c, err.value
+
+
+def product_bits(c) -> tuple[int, int]:
+    """(largest bit length of |c_j|, largest 64-bit leading-zero count of
+    |c_j|) -- computeMaxBitVector + findMaxErrorHOST."""
+    cc = np.ascontiguousarray(c, dtype=np.int64)
+    b, z = C.c_int(0), C.c_int(0)
+    _ibd().ibdwt_product_bits(len(cc), _p(cc), C.byref(b), C.byref(z))
+    return b.value, z.value
 
 
 def ibdwt_restore(p: int, N: int, c):

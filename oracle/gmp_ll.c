@@ -155,8 +155,13 @@ double gmp_ll_time(int p, int warm, int iters) {
 const char *gmp_ll_version(void) { return gmp_version; }
 
 #ifdef GMP_LL_MAIN
-/* CLI: gmp_ll p k1 [k2 ...]  -> one line "k res64hex zero" per k, flushed. */
+/* CLI: gmp_ll p k1 [k2 ...]  -> one line "k res64hex zero" per k, flushed.
+ *      gmp_ll --time p warm iters -> "ms_per_iter" (gmp_ll_time), one line. */
 int main(int argc, char **argv) {
+    if (argc == 5 && strcmp(argv[1], "--time") == 0) {
+        printf("%.6f\n", gmp_ll_time(atoi(argv[2]), atoi(argv[3]), atoi(argv[4])));
+        return 0;
+    }
     if (argc < 3) {
         fprintf(stderr, "usage: %s p k1 [k2 ...]\n", argv[0]);
         return 2;

@@ -376,3 +376,23 @@ int ibdwt_tables(int p, int N, int *w, uint8_t *bpw8, double *a, double *ainv) {
     plan_free(&P);
     return 0;
 }
+
+/*
+ * computeMaxBitVector (gpuLucas.cu:826-836) reduced by findMaxErrorHOST
+ * (gpuLucas.cu:811-819): per product the 64-bit leading-zero count of |c_j|
+ * as a float (__clzll(0) is 64), maximum over j -- the reference's "max Bit
+ * Vector".  Also the largest bit length of |c_j|, the figure that bounds the
+ * products against the 53-bit FP64 mantissa.
+ */
+void ibdwt_product_bits(int N, const int64_t *c, int *max_bits, int *max_clz) {
+    float mz = 0.0f;
+    int mb = 0;
+    for (int j = 0; j < N; j++) {
+        uint64_t m = c[j] >= 0 ? (uint64_t)c[j] : (uint64_t)0 - (uint64_t)c[j];
+        int z = m ? __builtin_clzll(m) : 64;
+        if ((float)z > mz) mz = (float)z;
+        if (64 - z > mb) mb = 64 - z;
+    }
+    *max_bits = mb;
+    *max_clz = (int)mz;
+}

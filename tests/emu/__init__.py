@@ -39,6 +39,7 @@ def lib():
         _lib.emu_set_waves.argtypes = [C.c_int, C.c_int]
         _lib.emu_run.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_longlong, C.c_void_p,
                                  C.POINTER(C.c_double)]
+        _lib.emu_product_bits.argtypes = [C.c_void_p]
     return _lib
 
 
@@ -55,3 +56,11 @@ def run(p, N, digits, iters, A=0, C_=0, T=0, single=False, waves=(-1, -1)):
                        C.byref(err))
     assert rc == 0, "plan refused"
     return out, err.value
+
+
+def product_bits():
+    """(max bit length, max 64-bit leading-zero count) of the rounded
+    products over the last run's iterations (the kernels' diagnostic)."""
+    out = np.zeros(2, dtype=np.int32)
+    lib().emu_product_bits(out.ctypes.data_as(C.c_void_p))
+    return int(out[0]), int(out[1])

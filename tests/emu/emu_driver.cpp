@@ -58,6 +58,14 @@ extern "C" void emu_set_single(int on) { g_single = on; }
 static int g_wA = -1, g_wB = -1;
 extern "C" void emu_set_waves(int wA, int wB) { g_wA = wA; g_wB = wB; }
 
+// Product-size diagnostic of the last emu_run: max bit length, max 64-bit
+// leading-zero count of the rounded products.
+static unsigned g_pbits[2] = {0, 0};
+extern "C" void emu_product_bits(int* out) {
+    out[0] = (int)g_pbits[0];
+    out[1] = (int)g_pbits[1];
+}
+
 extern "C" int emu_run(int p, int N, int A, int C, int T, const int32_t* digits_in, long long iters,
                        int32_t* digits_out, double* maxerr) {
     using namespace lucas;
@@ -79,6 +87,8 @@ extern "C" int emu_run(int p, int N, int A, int C, int T, const int32_t* digits_
     d.rlo = t.rlo.data(); d.rhi = t.rhi.data();
     d.wlo = t.wlo.data(); d.whi = t.whi.data(); d.iwlo = t.iwlo.data(); d.iwhi = t.iwhi.data();
     d.spec = spec.data(); d.spec2 = spec2.data(); d.digits = dig.data(); d.maxerr = &err;
+    g_pbits[0] = g_pbits[1] = 0;
+    d.pbits = g_pbits;
     std::vector<int2> cmap, rmap;
     for (int w = 0; w < pl.col_wgs; ++w) cmap.push_back({0, w});
     for (int w = 0; w < pl.row_wgs; ++w) rmap.push_back({0, w});

@@ -82,6 +82,7 @@ inline int __builtin_amdgcn_ds_swizzle(int src, int pattern) {
 inline int __builtin_amdgcn_ds_bpermute(int addr, int src) { return emu::wave_read(src, (addr >> 2) & 63); }
 inline int __builtin_amdgcn_readlane(int x, int lane) { return emu::wave_read(x, lane & 63); }
 using std::max;
+using std::min;
 inline int __double2loint(double d) { unsigned long long u; std::memcpy(&u, &d, 8); return (int)(u & 0xffffffffu); }
 inline int __double2hiint(double d) { unsigned long long u; std::memcpy(&u, &d, 8); return (int)(u >> 32); }
 inline double __hiloint2double(int hi, int lo) {
@@ -119,7 +120,7 @@ inline unsigned atomicMax(unsigned* p, unsigned v) {
     if (v > old) *p = v;
     return old;
 }
-using std::fma; using std::fmax; using std::fabs; using std::rint;
+using std::fma; using std::fmax; using std::fmin; using std::fabs; using std::rint;
 inline float fmaxf(float a, float b) { return std::fmax(a, b); }
 
 template <typename F, typename... Args>

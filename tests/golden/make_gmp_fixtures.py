@@ -4,7 +4,8 @@
 Each entry is the low 64 bits of the canonical s_k mod (2^p - 1) after k
 squarings from s_0 = 4, computed by the GMP oracle (oracle/gmp_ll, built by
 ``make -C oracle``).  Runs are independent and spread over a process pool;
-each run's lines are kept in a scratch folder so an interrupted generation
+each run's lines are kept in a scratch folder (a fresh temporary folder
+unless --scratch names one), so a generation interrupted with --scratch
 resumes where it stopped.
 
 Usage:
@@ -18,6 +19,7 @@ import json
 import os
 import subprocess
 import sys
+import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
@@ -52,6 +54,10 @@ def plan():
     # The longest lengths: M48 (N = 3670016) and, past the ordinary tilings
     # (wide tiles, N up to 8064000), a depth-4 exponent and M52.
     runs["longest"] = [(57885161, [100]), (100000007, [100]), (136279841, [100])]
+    # The serving workload at its stated size: the first 48 Poisson jobs of
+    # seed 5 (p log-uniform in [1e5, 4e7], K uniform in [5000, 50000]), each
+    # at its own full K, plus an early checkpoint.
+    runs["c5_serving"] = [(p, sorted({1000, k})) for _, p, k in wl.serving_jobs(seed=5, count=48)]
     return runs
 
 
@@ -79,13 +85,15 @@ def run_one(p, iters, scratch):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=6)
-    ap.add_argument("--scratch", default=os.path.join(ROOT, "_work", "gmp"))
+    ap.add_argument("--scratch", default=None, help="resumable scratch folder (default: a temporary one)")
     ap.add_argument("--only", default=None)
     ap.add_argument("--out", default=OUT)
     args = ap.parse_args()
     out_path = args.out
     if not os.path.exists(GMP_LL):
         sys.exit("build the oracle first: make -C oracle")
+    if args.scratch is None:
+        args.scratch = tempfile.mkdtemp(prefix="gmp_fixtures_")
     os.makedirs(args.scratch, exist_ok=True)
     runs = plan()
     if args.only:

@@ -20,7 +20,7 @@ def test_batch64_residues_vs_gmp(gmp_fixtures):
             for p, r, z, e in zip(ps, res, zero, err):
                 want, wz = fixture_res64(gmp_fixtures, "c4_batch64", p, k)
                 assert (r, z) == (want, wz), (p, k)
-                assert e < 0.4, (p, e)
+                assert e < lucas.DEFAULT_ERROR_LIMIT, (p, e)
             print(f"k={k} max roundoff {max(err):.4f}", flush=True)
 
 

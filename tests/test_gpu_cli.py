@@ -43,3 +43,16 @@ def test_cli_interrupt_then_resume(tmp_path):
     assert r.returncode == 0, r.stderr
     assert "Resuming M859433" in r.stdout
     assert "M( 859433 )P, n = 49152" in r.stdout
+
+
+def test_cli_refuses_a_length_past_the_roundoff_gate(tmp_path):
+    """-f with a length far too short for the exponent (20 bits per word at
+    N = 2^20): the error trial trips the gate, nothing is written to
+    results.txt and the exit status is non-zero."""
+    from gpulucas_amd import workloads as wl
+    N = 1 << 20
+    p = wl.next_prime(20 * N)
+    r = subprocess.run([CLI, "-q", "-f", str(N), str(p)], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode != 0
+    assert "roundoff" in r.stderr
+    assert not (tmp_path / "results.txt").exists()

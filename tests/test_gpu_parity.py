@@ -4,7 +4,8 @@
   restatement of the reference (same carry-save restoration, same ripple);
 * Res64 equals GMP at the iterations the workloads name; Mersenne-prime
   exponents end at residue 0;
-* max roundoff stays below 0.4 on every case.
+* max roundoff stays below the library's gate (0.30, the reference's
+  ERROR_LIMIT) on every case, and the gate fires where it must.
 """
 import numpy as np
 import pytest
@@ -15,7 +16,7 @@ from gpulucas_amd import lucas
 
 pytestmark = pytest.mark.gpu
 
-ROUNDOFF_LIMIT = 0.4
+ROUNDOFF_LIMIT = lucas.DEFAULT_ERROR_LIMIT
 
 
 def random_balanced(p, N, seed):
@@ -275,11 +276,84 @@ def test_config3_near3m_primes(gmp_fixtures):
 def test_error_test_contract():
     ms = lucas.errorTest(86243, 4608, 200)
     assert ms > 0
-    # q = 20 bits per word at a short length: roundoff must trip the 0.30 gate
-    bad = lucas.error_test(1310719, 65536, 200)
-    assert bad["status"] == lucas.ERR_ROUNDOFF or bad["max_roundoff"] < 0.30
-    if bad["status"] == lucas.ERR_ROUNDOFF:
-        assert lucas.errorTest(1310719, 65536, 200) == -1.0
+
+
+def test_roundoff_gate_fires_on_an_explicit_limit():
+    """errorTrial's contract (gpuLucas.cu:904-907, 1239-1242): a limit below
+    the measured roundoff returns ERR_ROUNDOFF and errorTest returns -1.0;
+    mersenneTest stops at the first chunk and reports the breach."""
+    ok = lucas.error_test(86243, 4608, 200)
+    assert ok["status"] == lucas.OK and 0.0 < ok["max_roundoff"] < ROUNDOFF_LIMIT
+    bad = lucas.error_test(86243, 4608, 200, error_limit=1e-9)
+    assert bad["status"] == lucas.ERR_ROUNDOFF and bad["max_roundoff"] == ok["max_roundoff"]
+    assert lucas.errorTest(86243, 4608, 200, error_limit=1e-9) == -1.0
+    r = lucas.mersenne_test(86243, 4608, error_limit=1e-9)
+    assert r["status"] == lucas.ERR_ROUNDOFF and r["is_prime"] == 0
+    assert 0 < r["iterations"] < 86241  # stopped at the first gate check
+
+
+def test_roundoff_gate_fires_past_the_calibrated_line():
+    """A length far too short for its exponent: p = 20.0 bits per word at
+    N = 2^20, 1.8 bits above the calibrated line.  The roundoff must reach
+    the 0.30 gate with no help from the caller."""
+    from gpulucas_amd import workloads as wl
+    N = 1 << 20
+    p = wl.next_prime(20 * N)
+    info = lucas.plan(p, N)
+    assert info["q"] == 20 and p / N - 18.2 > 1.5
+    r = lucas.error_test(p, N, 200)
+    print(f"p={p} N={N} roundoff={r['max_roundoff']:.4f}", flush=True)
+    assert r["status"] == lucas.ERR_ROUNDOFF and r["max_roundoff"] >= ROUNDOFF_LIMIT
+    assert lucas.errorTest(p, N, 200) == -1.0
+
+
+def test_product_bits_match_exact_convolution():
+    """computeMaxBitVector's diagnostic (gpuLucas.cu:826-836): after one
+    iteration from random digits, the device maximum of bit lengths and of
+    64-bit leading-zero counts of the rounded products equals the exact
+    weighted convolution's."""
+    for p, N in [(4423, 240), (9941, 560), (19937, 1080)]:
+        x = random_balanced(p, N, 31)
+        want = oracle.product_bits(oracle.ibdwt_conv_exact(p, N, x))
+        with lucas.Context(p, N) as ctx:
+            ctx.set_digits(x, 0)
+            ctx.product_bits(reset=True)
+            ctx.step(1)
+            assert ctx.product_bits() == want, (p, N)
+
+
+@pytest.mark.parametrize("p,N", [(86243, 4608), (1257787, 65536), (2976221, 163840)])
+def test_product_bits_at_config_shapes(p, N):
+    """The same at the configuration lengths against the oracle's
+    long-double convolution (exact integers at these roundoffs)."""
+    x = random_balanced(p, N, 32)
+    c, err = oracle.ibdwt_conv_fft(p, N, x)
+    assert err < 0.25
+    want = oracle.product_bits(c)
+    with lucas.Context(p, N) as ctx:
+        ctx.set_digits(x, 0)
+        ctx.product_bits(reset=True)
+        ctx.step(1)
+        got = ctx.product_bits()
+    print(f"p={p} N={N} product bits {got}", flush=True)
+    assert got == want
+    assert got[0] <= 53
+
+
+@pytest.mark.parametrize("p,group,k", [(2976221, "c3_p2976221", 20000), (37156667, "large", 2000)])
+def test_autotune_n_stays_under_the_calibrated_line(p, group, k, gmp_fixtures):
+    """findSignalSize (gpuLucas.cu:1064-1187): the measured choice lies in
+    the candidate window, at or under the calibrated bits-per-word line, and
+    its residue equals GMP's."""
+    info = lucas.autotune_N(p)
+    N = info["N"]
+    print(f"p={p} chose N={N} ({p / N:.3f} bits/word)", flush=True)
+    assert N in lucas.autotune_candidates(p)
+    assert p / N <= lucas.max_bits_per_word(N)
+    with lucas.Context(p, N) as ctx:
+        ctx.step(k)
+        assert ctx.res64() == fixture_res64(gmp_fixtures, group, p, k)
+        assert ctx.max_roundoff() < ROUNDOFF_LIMIT
 
 
 def test_reference_checkpoint_resume_vs_gmp(tmp_path, gmp_fixtures):

@@ -245,3 +245,40 @@ def test_tuned_table_entries_are_the_plans_chosen():
         pl = lucas.plan(p, N)
         assert (pl["A"], pl["C"], pl["threads"], pl["radix_pref"]) == (A, C, T, pref), N
         assert (pl["wave_A"], pl["wave_B"]) == (wA, wB), N
+
+
+@pytest.mark.parametrize("p", [86243, 1257787, 2976221, 37156667])
+def test_autotune_candidates_follow_the_reference_window(p):
+    """findSignalSize's window (gpuLucas.cu:1072-1077): N from p/20 to the
+    smallest power of two >= p/16, but not past p/14.  Every candidate is an
+    admissible length at or under the calibrated bits-per-word line, every
+    admissible length of the window under the line is a candidate, and the
+    default length is always among them."""
+    from gpulucas_amd import lucas
+    lo = -(-p // 20)
+    pow2 = 1
+    while pow2 < p // 16:
+        pow2 *= 2
+    hi = min(pow2, p // 14)
+    cands = lucas.autotune_candidates(p)
+    assert cands == sorted(set(cands))
+    dn = lucas.default_N(p)
+    assert dn in cands
+    for N in cands:
+        assert N == dn or lo <= N <= hi
+        assert p / N <= lucas.max_bits_per_word(N)
+        lucas.plan(p, N)  # has a tiling
+    smooth = [N for N in range(lo + (lo & 1), hi + 1, 2) if _smooth(N) and p / N <= lucas.max_bits_per_word(N)]
+    for N in smooth:
+        try:
+            lucas.plan(p, N)
+        except lucas.LucasError:
+            continue
+        assert N in cands, N
+
+
+def _smooth(n):
+    for f in (2, 3, 5, 7):
+        while n % f == 0:
+            n //= f
+    return n == 1

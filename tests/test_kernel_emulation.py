@@ -108,3 +108,27 @@ def test_emulated_wide_tiles_match_oracle(p, N, A, C):
     want, _ = oracle.ibdwt_iterate(p, N, x, 2)
     assert np.array_equal(got, want)
     assert err < 0.4
+
+
+@pytest.mark.parametrize("p,N,A,C,iters", [
+    (4423, 240, 15, 8, 1),   # odd A, one tile
+    (9941, 560, 7, 4, 3),    # radix 7, several iterations: the maximum over all of them
+    (1279, 64, 4, 1, 2),     # C < halo: halo products counted as well
+])
+def test_emulated_product_bits_match_exact_convolution(p, N, A, C, iters):
+    """The kernels' product-size diagnostic (computeMaxBitVector) equals
+    the bit lengths of the exact weighted convolution's outputs, maximised
+    over every iteration of the run."""
+    w, _, _, _ = oracle.tables(p, N)
+    rng = np.random.default_rng(21)
+    half = (1 << (w - 1)).astype(np.int64)
+    x = (rng.integers(0, 2 * half) - half).astype(np.int32)
+    want_b, want_z = 0, 0
+    y = x
+    for _ in range(iters):
+        b, z = oracle.product_bits(oracle.ibdwt_conv_exact(p, N, y))
+        want_b, want_z = max(want_b, b), max(want_z, z)
+        y, _ = oracle.ibdwt_iterate(p, N, y, 1)
+    got, _ = emu.run(p, N, x, iters, A, C, 0)
+    assert np.array_equal(got, y)
+    assert emu.product_bits() == (want_b, want_z)

@@ -174,3 +174,22 @@ def test_carry_depth_rule():
     # setSliceAndDice, gpuLucas.cu:323-343
     assert [oracle.carry_depth(q) for q in (21, 20, 18, 17, 16, 15, 14, 13, 12, 11, 8)] == \
         [-1, 2, 2, 3, 3, 4, 4, 5, 5, 6, 6]
+
+
+def test_product_bits_restatement_pinned_by_bit_length():
+    """computeMaxBitVector's reduction (oracle) against Python's exact
+    int.bit_length: largest bit length, and largest 64-bit leading-zero
+    count (64 for a zero product)."""
+    rng = np.random.default_rng(3)
+    for trial in range(50):
+        n = int(rng.integers(1, 40))
+        bits = rng.integers(0, 53, size=n)
+        c = [int(rng.integers(0, 1 << int(b))) * (1 if rng.random() < 0.5 else -1) for b in bits]
+        if trial % 5 == 0:
+            c[0] = 0
+        want_b = max(abs(v).bit_length() for v in c)
+        want_z = max(64 - abs(v).bit_length() for v in c)
+        assert oracle.product_bits(np.array(c, dtype=np.int64)) == (want_b, want_z)
+    # a single power of two on either side of a word boundary
+    for k in (0, 1, 31, 32, 52):
+        assert oracle.product_bits(np.array([-(1 << k)], dtype=np.int64)) == (k + 1, 63 - k)
