@@ -38,8 +38,10 @@ hipError_t read_phase_stamps(unsigned long long* col, int ncol, unsigned long lo
 // kLongStep or more (measured 0.8 % faster per iteration than graphs of 32 at
 // N = 163840), otherwise the binary digits of the remainder, largest first.
 // So every iteration of every step is graph-launched; a step of k < 128 pairs
-// launches at most 2 + log2(k) graphs.  Graphs are built on first use, or
-// ahead of a timed region by prepare_graphs().
+// launches at most 2 + log2(k) graphs.  A single-job engine runs a step of
+// at most kWholeMax iterations as one whole-step graph instead (each graph
+// launch costs host time the GPU waits for at the start of a step).  Graphs
+// are built on first use, or ahead of a timed region by prepare_graphs().
 static const int kLevels = 8;  // 1, 2, 4, .., 128 pairs
 static const long long kLongStep = 512;
 
@@ -326,6 +328,51 @@ void Engine::destroy_graphs() {
         gexec_[l] = nullptr;
         graph_[l] = nullptr;
     }
+    for (Whole& w : whole_) {
+        if (w.exec) hipGraphExecDestroy(w.exec);
+        if (w.graph) hipGraphDestroy(w.graph);
+    }
+    whole_.clear();
+}
+
+// The whole step of k iterations as one graph (head, k - 1 pairs, tail): a
+// short step then costs one graph launch.  Cached per k, most recent first.
+int Engine::whole_graph(long long k, hipGraphExec_t* out) {
+    for (size_t i = 0; i < whole_.size(); ++i)
+        if (whole_[i].k == k) {
+            std::rotate(whole_.begin(), whole_.begin() + i, whole_.begin() + i + 1);
+            *out = whole_[0].exec;
+            return LUCAS_OK;
+        }
+    Whole w;
+    w.k = k;
+    hipError_t e = hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return LUCAS_ERR_DEVICE;
+    int rc = launch_col(LUCAS_IN_DIGITS);
+    for (long long i = 1; i < k && rc == LUCAS_OK; ++i) {
+        rc = launch_row();
+        if (rc == LUCAS_OK) rc = launch_col(0);
+    }
+    if (rc == LUCAS_OK) rc = launch_row();
+    if (rc == LUCAS_OK) rc = launch_col(LUCAS_OUT_DIGITS);
+    e = hipStreamEndCapture(stream_, &w.graph);
+    if (rc != LUCAS_OK || e != hipSuccess) {
+        if (w.graph) hipGraphDestroy(w.graph);
+        (void)hipGetLastError();
+        return LUCAS_ERR_DEVICE;
+    }
+    if (hipGraphInstantiate(&w.exec, w.graph, nullptr, nullptr, 0) != hipSuccess) {
+        hipGraphDestroy(w.graph);
+        return LUCAS_ERR_DEVICE;
+    }
+    if ((int)whole_.size() == kWholeCache) {
+        hipGraphExecDestroy(whole_.back().exec);
+        hipGraphDestroy(whole_.back().graph);
+        whole_.pop_back();
+    }
+    whole_.insert(whole_.begin(), w);
+    *out = w.exec;
+    return LUCAS_OK;
 }
 
 // Slot l < kLevels: 2^l steady-state [row ; col] pairs; kHead: the column
@@ -381,6 +428,10 @@ int Engine::prepare_graphs(long long iters) {
     if (jobs_.empty()) return LUCAS_OK;
     hipSetDevice(device_);
     int rc = prepare();
+    if (rc == LUCAS_OK && single_ && iters <= kWholeMax) {
+        hipGraphExec_t x = nullptr;
+        return whole_graph(iters, &x);
+    }
     std::vector<int> slots;
     step_plan(iters, &slots);
     for (size_t i = 0; i < slots.size() && rc == LUCAS_OK; ++i) rc = build_graph(slots[i]);
@@ -392,6 +443,15 @@ int Engine::step(long long iters) {
     if (jobs_.empty()) return LUCAS_OK;
     hipSetDevice(device_);
     int rc = prepare();
+    if (rc == LUCAS_OK && single_ && iters <= kWholeMax) {
+        // one job, short step: the whole step is one graph launch
+        hipGraphExec_t x = nullptr;
+        rc = whole_graph(iters, &x);
+        if (rc == LUCAS_OK && hipGraphLaunch(x, stream_) != hipSuccess) rc = LUCAS_ERR_DEVICE;
+        if (rc == LUCAS_OK)
+            for (Job* j : jobs_) j->iter += iters;
+        return rc;
+    }
     std::vector<int> slots;
     step_plan(iters, &slots);
     for (size_t i = 0; i < slots.size() && rc == LUCAS_OK; ++i) {

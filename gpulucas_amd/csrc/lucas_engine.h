@@ -93,6 +93,17 @@ private:
     // replay graphs by slot (see kHead, kTail), built on first use
     hipGraph_t graph_[kGraphSlots] = {};
     hipGraphExec_t gexec_[kGraphSlots] = {};
+    // whole-step graphs of a single-job engine: one graph per short step
+    // length k <= kWholeMax (head, k - 1 pairs, tail), the most recent
+    // kWholeCache lengths kept
+    static const int kWholeMax = 64, kWholeCache = 8;
+    struct Whole {
+        long long k = 0;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<Whole> whole_;
+    int whole_graph(long long k, hipGraphExec_t* out);
     void destroy_graphs();
 };
 
