@@ -599,18 +599,30 @@ __device__ __forceinline__ int restore_digit(const unsigned long long* mag, cons
     return sum - h * (1 << W);
 }
 
-// A rounded convolution output c_j as the carry restoration reads it: sign in
-// bit 63, bit 62 set iff digit j is wide (q+1 bits), |c_j| below (|c| < 2^53
-// whenever the roundoff is meaningful).
-__device__ __forceinline__ double pack_product(double c, bool wide) {
-    const unsigned long long m = (unsigned long long)fabs(c);
-    const unsigned long long bits = m | ((unsigned long long)wide << 62) | (c < 0.0 ? (1ull << 63) : 0ull);
-    return __longlong_as_double((long long)bits);
+// Rounding of an unweighted convolution output v to c = rint(v) (ties to
+// even, as the reference's __double2ll_rn, gpuLucas.cu:791-796), packed as
+// the carry restoration reads it: sign in bit 63, bit 62 set iff the digit
+// is wide (q+1 bits), |c| in bits 0..51.  For |v| < 2^52, t = |v| + 2^52
+// lies in [2^52, 2^53) where doubles are the integers, so the addition rounds
+// |v| to |c| (rint is odd-symmetric) and t's mantissa field holds |c|; the
+// packing only rewrites t's sign and exponent bits.  *err receives |v - c|,
+// and *mag |c|.  |v| >= 2^52 (or NaN) has no meaningful rounding at all: it
+// reports roundoff 1 and packs 0, so the gate fires.
+__device__ __forceinline__ double round_pack(double v, bool wide, float* err, double* mag) {
+    const double av = fabs(v);
+    const bool fits = av < 0x1p52;
+    const double t = (fits ? av : 0.0) + 0x1p52;
+    const double c = t - 0x1p52;
+    *err = fits ? (float)fabs(av - c) : 1.0f;
+    *mag = c;
+    const unsigned int hi = ((unsigned int)__double2hiint(t) & 0xFFFFFu) | ((unsigned int)wide << 30) |
+                            (v < 0.0 ? 0x80000000u : 0u);
+    return __hiloint2double((int)hi, __double2loint(t));
 }
 
 // Carry-save restoration for element m = (row r, local column lc) of the
 // tile: digits 2m+1 and 2m, and with `first` also digit 2m-1 (carry only).
-// The products of digits 2m+1-k, k = 0 .. NC+2 (pack_product words), are read
+// The products of digits 2m+1-k, k = 0 .. NC+2 (round_pack words), are read
 // top-down; a local column left of global column 0 is the previous row's halo
 // (the cyclic wrap of IrrBaseBalanced.cu:64-70).  Returns (d_2m, d_2m+1 + h_2m)
 // and the carries h_2m+1 and h_2m-1.
@@ -627,7 +639,7 @@ __device__ __forceinline__ int2 restore_element(const double2* lds, const LucasJ
         const double2 v = lds[pidx(rr * ncl + (lc - ie))];
         const unsigned long long bits = (unsigned long long)__double_as_longlong((k & 1) ? v.x : v.y);
         neg[k] = (bits >> 63) != 0;
-        mag[k] = bits & ((1ull << 62) - 1);
+        mag[k] = bits & ((1ull << 52) - 1);
         w[k] = J.q + (int)((bits >> 62) & 1);  // width of digit 2m+1-k
     }
     int h0;
@@ -740,7 +752,7 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc& J, int tile,
         const SeqGeom g{ncl, mncl, 0, 0u, 1, 0, 0, ncl};
         const SeqGeom gA{ncl * wA, J.m_sqA0, 0, 0u, 1, 0, 0, wA * ncl};
         // unweight, -2 on digit 0, round half-even, roundoff of element
-        // (r, lc); the rounded products go back as pack_product words
+        // (r, lc); the rounded products go back as round_pack words
         auto unweight = [&](int lc, int r, double2 v) -> double2 {
             int gc = c0 - H + lc;
             if (gc < 0) gc += B;
@@ -755,12 +767,14 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc& J, int tile,
             double ve = v.x * ie;
             const double vo = v.y * io;
             if (gc == 0 && r == 0) ve -= 2.0;
-            const double re = rint(ve), ro = rint(vo);
-            err = fmaxf(err, (float)fmax(fabs(ve - re), fabs(vo - ro)));
-            pmax = fmax(pmax, fmax(fabs(re), fabs(ro)));
-            pmin = fmin(pmin, fmin(fabs(re), fabs(ro)));
-            return make_double2(pack_product(re, (rje >= N ? rje - N : rje) < J.b),
-                                pack_product(ro, (rjo >= N ? rjo - N : rjo) < J.b));
+            float ee, eo;
+            double me, mo;
+            const double pe = round_pack(ve, (rje >= N ? rje - N : rje) < J.b, &ee, &me);
+            const double po = round_pack(vo, (rjo >= N ? rjo - N : rjo) < J.b, &eo, &mo);
+            err = fmaxf(err, fmaxf(ee, eo));
+            pmax = fmax(pmax, fmax(me, mo));
+            pmin = fmin(pmin, fmin(me, mo));
+            return make_double2(pe, po);
         };
         if constexpr (PA > 0) {
             // applied to the wave factor's outputs as they are written
