@@ -20,7 +20,7 @@ LIB = os.path.join(PKG, "libgpulucas.so")
 PHASES_LIB = os.path.join(PKG, "libgpulucas_phases.so")
 CLI = os.path.join(PKG, "gpulucas")
 
-SOURCES = ["lucas_kernels.hip", "lucas_host.cpp", "lucas_engine.cpp", "lucas_capi.cpp"]
+SOURCES = ["lucas_kernels.hip", "lucas_host.cpp", "lucas_engine.cpp", "lucas_capi.cpp", "lucas_noalloc.hip"]
 HEADERS = ["lucas_desc.h", "lucas_host.h", "lucas_engine.h"]
 
 ARCH = os.environ.get("GPULUCAS_ARCH", "gfx950")

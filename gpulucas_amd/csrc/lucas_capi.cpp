@@ -400,6 +400,12 @@ int lucas_ctx_get_digits(lucas_ctx* ctx, int32_t* digits) {
     return job_get_digits(ctx->job, ctx->engine->stream(), digits);
 }
 
+int lucas_ctx_device_digits(lucas_ctx* ctx, void** ptr) {
+    if (!ctx || !ptr) return LUCAS_ERR_INVALID_ARG;
+    *ptr = ctx->job->desc.digits;
+    return LUCAS_OK;
+}
+
 int lucas_ctx_set_digits(lucas_ctx* ctx, const int32_t* digits, long long iter) {
     if (!ctx || !digits || iter < 0) return LUCAS_ERR_INVALID_ARG;
     std::vector<int> w(ctx->job->plan.N);

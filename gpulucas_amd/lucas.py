@@ -113,6 +113,7 @@ _SIGS = {
     "lucas_ctx_iteration": (C.c_longlong, [C.c_void_p]),
     "lucas_ctx_get_digits": (C.c_int, [C.c_void_p, C.c_void_p]),
     "lucas_ctx_set_digits": (C.c_int, [C.c_void_p, C.c_void_p, C.c_longlong]),
+    "lucas_ctx_device_digits": (C.c_int, [C.c_void_p, _P(C.c_void_p)]),
     "lucas_ctx_res64": (C.c_int, [C.c_void_p, _P(C.c_uint64), _P(C.c_int)]),
     "lucas_ctx_max_roundoff": (C.c_int, [C.c_void_p, C.c_int, _P(C.c_double)]),
     "lucas_ctx_product_bits": (C.c_int, [C.c_void_p, C.c_int, _P(C.c_int), _P(C.c_int)]),
@@ -127,6 +128,8 @@ _SIGS = {
     "lucas_error_test": (C.c_int, [C.c_int, C.c_int, C.c_int, _P(Options), _P(Result)]),
     "lucas_autotune_N": (C.c_int, [C.c_int, C.c_int, _P(Options), _P(PlanInfo)]),
     "lucas_autotune_candidates": (C.c_int, [C.c_int, C.c_void_p, C.c_int, _P(C.c_int)]),
+    "lucas_step_digits_noalloc": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_longlong, C.c_int, C.c_void_p,
+                                            _P(C.c_double)]),
     "lucas_batch_create": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, _P(C.c_void_p)]),
     "lucas_batch_destroy": (C.c_int, [C.c_void_p]),
     "lucas_batch_step": (C.c_int, [C.c_void_p, C.c_longlong]),
@@ -294,6 +297,12 @@ class Context:
         _check(_lib().lucas_ctx_get_digits(self._h, _ptr(d)), "get_digits")
         return d
 
+    def device_digits(self) -> int:
+        """Device address of the context's digit vector (N int32)."""
+        ptr = C.c_void_p()
+        _check(_lib().lucas_ctx_device_digits(self._h, C.byref(ptr)), "device_digits")
+        return ptr.value
+
     def set_digits(self, digits, iteration: int = 0):
         d = np.ascontiguousarray(digits, dtype=np.int32)
         assert d.shape == (self.N,)
@@ -406,6 +415,24 @@ def autotune_N(p: int, iters: int = 300, device: int = 0, error_limit: float = 0
 
 def findSignalSize(p: int, **kw) -> int:  # noqa: N802
     return autotune_N(p, **kw)["N"]
+
+
+# ---- no device allocation -----------------------------------------------------
+
+NOALLOC_MAX_N = 524288
+
+
+def step_digits_noalloc(p: int, N: int, digits_dev_ptr: int, iters: int, device: int = 0, stream: int = 0,
+                        with_roundoff: bool = True):
+    """Advance N int32 digits held in caller-owned device memory (address
+    `digits_dev_ptr`) by `iters` iterations with no device allocation (the
+    library's static per-GPU scratch array).  Returns the max roundoff (after
+    synchronising the stream) or None."""
+    err = C.c_double(0)
+    _check(_lib().lucas_step_digits_noalloc(p, N, C.c_void_p(digits_dev_ptr), iters, device, C.c_void_p(stream),
+                                            C.byref(err) if with_roundoff else None),
+           f"step_digits_noalloc(p={p}, N={N})")
+    return err.value if with_roundoff else None
 
 
 # ---- batches ----------------------------------------------------------------

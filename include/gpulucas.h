@@ -139,6 +139,11 @@ int lucas_ctx_step(lucas_ctx* ctx, long long iters);
 long long lucas_ctx_iteration(const lucas_ctx* ctx);
 /* Current pseudo-balanced digits (N int32), D2H. */
 int lucas_ctx_get_digits(lucas_ctx* ctx, int32_t* digits);
+/* Device address of the context's digit vector (N int32, pseudo-balanced,
+ * valid between steps), for device-side interop such as
+ * lucas_step_digits_noalloc.  No reference counterpart (its state was a
+ * global, gpuLucas.cu:179-257). */
+int lucas_ctx_device_digits(lucas_ctx* ctx, void** ptr);
 /* Replace the state by a digit vector (|d_j| < 2^(w_j+1)) at iteration `iter`. */
 int lucas_ctx_set_digits(lucas_ctx* ctx, const int32_t* digits, long long iter);
 int lucas_ctx_res64(lucas_ctx* ctx, uint64_t* res64, int* is_zero);
@@ -183,6 +188,26 @@ int lucas_ctx_time_iterations(lucas_ctx* ctx, long long iters, double* ms_per_it
  * files return LUCAS_ERR_IO before any device work. */
 int lucas_ctx_save(lucas_ctx* ctx, const char* path);
 int lucas_ctx_load(const char* path, int device, lucas_ctx** out);
+
+/* ---- no device allocation --------------------------------------------- */
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the computation of lucas_ctx_set_digits +
+ * lucas_ctx_step(iters) + lucas_ctx_get_digits on a context of (p, N), with
+ * the pseudo-balanced digits (N int32) read and written in place in
+ * caller-owned device memory `digits`, launched on `stream` (a hipStream_t;
+ * NULL: the null stream).  Every intermediate (spectra, weight and twiddle
+ * tables, roundoff accumulator) lives in a static device array of
+ * LUCAS_NOALLOC_SCRATCH_BYTES that the library holds on each GPU it runs on
+ * until the process ends; the call makes no hipMalloc / hipFree.  N above
+ * LUCAS_NOALLOC_MAX_N, or a plan whose tables do not fit the array, returns
+ * LUCAS_ERR_INVALID_ARG.  Calls on one GPU share its array and overwrite it:
+ * calls that overlap on one GPU race.  max_roundoff (optional; non-NULL
+ * synchronises the stream) receives the call's max roundoff. */
+#define LUCAS_NOALLOC_MAX_N 524288
+#define LUCAS_NOALLOC_SCRATCH_BYTES (9 * 1024 * 1024)
+int lucas_step_digits_noalloc(int p, int N, int32_t* digits, long long iters, int device, void* stream,
+                              double* max_roundoff);
 
 /* ---- reference entry points ------------------------------------------- */
 /* Full test: p-2 iterations from s_0 = 4 (or from a checkpoint), roundoff
