@@ -92,7 +92,8 @@ struct LucasJobDesc {
 //     {2^(rhi/N), 2^(-rhi/N)/(4M)}; per local digit {2^(rlo/N), 2^(-rlo/N)};
 //     per-row rhi; per local digit rlo; stage table;
 //     omega_A^t, t < A, and the wave-factor table (with a wave factor only);
-//     int8 carries of the own elements, A x C
+//     int8 carries of the own elements, A x C; int8 carry into each row's
+//     first own element, A
 //   row kernel: two rows; stage twiddles; omega_B^t, t < B; stage table;
 //     the wave-factor table (with a wave factor only)
 // A tile of n words occupies whole 16-word blocks (its word index is permuted
@@ -104,7 +105,7 @@ __host__ __device__ constexpr int lucas_wave_table_words(int P) { return P ? 32 
 __host__ __device__ inline int lucas_col_lds_words(int A, int C, int H, int ntws, int nst, int pA) {
     const int ncl = C + H;
     return lucas_pad_words(A * ncl) + ntws + A + 2 * ncl + (A + 3) / 4 + (2 * ncl + 3) / 4 + 2 * nst +
-           (pA ? A : 0) + lucas_wave_table_words(pA) + (A * C + 15) / 16;
+           (pA ? A : 0) + lucas_wave_table_words(pA) + (A * C + 15) / 16 + (A + 15) / 16;
 }
 __host__ __device__ inline int lucas_row_lds_words(int B, int ntws, int nst, int pB) {
     return lucas_pad_words(2 * B) + ntws + B + 2 * nst + lucas_wave_table_words(pB);

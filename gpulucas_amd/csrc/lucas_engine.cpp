@@ -361,7 +361,9 @@ int Engine::whole_graph(long long k, hipGraphExec_t* out) {
         (void)hipGetLastError();
         return LUCAS_ERR_DEVICE;
     }
-    if (hipGraphInstantiate(&w.exec, w.graph, nullptr, nullptr, 0) != hipSuccess) {
+    // uploaded at once: the first launch then skips the upload
+    if (hipGraphInstantiate(&w.exec, w.graph, nullptr, nullptr, 0) != hipSuccess ||
+        hipGraphUpload(w.exec, stream_) != hipSuccess) {
         hipGraphDestroy(w.graph);
         return LUCAS_ERR_DEVICE;
     }
@@ -401,6 +403,7 @@ int Engine::build_graph(int slot) {
         return LUCAS_ERR_DEVICE;
     }
     e = hipGraphInstantiate(&gexec_[slot], graph_[slot], nullptr, nullptr, 0);
+    if (e == hipSuccess) e = hipGraphUpload(gexec_[slot], stream_);
     if (e != hipSuccess) {
         hipGraphDestroy(graph_[slot]);
         graph_[slot] = nullptr;

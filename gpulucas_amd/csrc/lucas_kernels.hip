@@ -589,12 +589,15 @@ __device__ __forceinline__ int restore_digit(const unsigned long long* mag, cons
         const int part = (NC <= 3 || shift < 64) ? (int)((unsigned int)(mag[i] >> shift) & mask) : 0;
         sum += neg[i] ? -part : part;
     }
+    // The reference's carry (IrrBaseBalanced.cu:103-109): h = (sum + half) >> W
+    // for sum >= half, -((-sum + half) >> W) for sum < -half, else 0; here
+    // without branches, as the magnitude's carry with sum's sign.  |sum| =
+    // half with sum < 0 is not past the threshold: no carry.
     const int half = 1 << (W - 1);
-    int h = 0;
-    if (sum < -half)
-        h = -((-sum + half) >> W);
-    else if (sum >= half)
-        h = (sum + half) >> W;
+    const int sg = sum >> 31;         // 0 or -1
+    const int a = (sum ^ sg) - sg;    // |sum|
+    const int hm = (a + half - (sum == -half ? 1 : 0)) >> W;
+    const int h = (hm ^ sg) - sg;
     *carry = h;
     return sum - h * (1 << W);
 }
@@ -620,34 +623,55 @@ __device__ __forceinline__ double round_pack(double v, bool wide, float* err, do
     return __hiloint2double((int)hi, __double2loint(t));
 }
 
-// Carry-save restoration for element m = (row r, local column lc) of the
-// tile: digits 2m+1 and 2m, and with `first` also digit 2m-1 (carry only).
-// The products of digits 2m+1-k, k = 0 .. NC+2 (round_pack words), are read
-// top-down; a local column left of global column 0 is the previous row's halo
-// (the cyclic wrap of IrrBaseBalanced.cu:64-70).  Returns (d_2m, d_2m+1 + h_2m)
-// and the carries h_2m+1 and h_2m-1.
-template <int NC>
-__device__ __forceinline__ int2 restore_element(const double2* lds, const LucasJobDesc& J, int r, int lc, int c0mH,
-                                                int ncl, bool first, int* h1, int* hm1) {
-    unsigned long long mag[NC + 3];
-    bool neg[NC + 3];
-    int w[NC + 3];
+// Window of CNT rounded products (round_pack words) of digits 2m+1-k,
+// k = k0 .. k0+CNT-1, for element m = (row r, local column lc) of the tile,
+// top-down: |c| in mag, sign in neg, the digit's width in w.  A local column
+// left of global column 0 is the previous row's halo (the cyclic wrap of
+// IrrBaseBalanced.cu:64-70).
+template <int CNT>
+__device__ __forceinline__ void load_window(const double2* lds, const LucasJobDesc& J, int r, int lc, int c0mH,
+                                            int ncl, int k0, unsigned long long* mag, bool* neg, int* w) {
 #pragma unroll
-    for (int k = 0; k < NC + 3; ++k) {
+    for (int i = 0; i < CNT; ++i) {
+        const int k = k0 + i;
         const int ie = k >> 1;  // element offset back
         const int rr = (c0mH + lc - ie < 0) ? (r == 0 ? J.A - 1 : r - 1) : r;
         const double2 v = lds[pidx(rr * ncl + (lc - ie))];
         const unsigned long long bits = (unsigned long long)__double_as_longlong((k & 1) ? v.x : v.y);
-        neg[k] = (bits >> 63) != 0;
-        mag[k] = bits & ((1ull << 52) - 1);
-        w[k] = J.q + (int)((bits >> 62) & 1);  // width of digit 2m+1-k
+        neg[i] = (bits >> 63) != 0;
+        mag[i] = bits & ((1ull << 52) - 1);
+        w[i] = J.q + (int)((bits >> 62) & 1);  // width of digit 2m+1-k
     }
+}
+
+// Carry-save restoration for element m = (row r, local column lc) of the
+// tile: digits 2m+1 and 2m from the products of digits 2m+1 .. 2m-NC.
+// Returns (d_2m, d_2m+1 + h_2m) and the carry h_2m+1.
+template <int NC>
+__device__ __forceinline__ int2 restore_element(const double2* lds, const LucasJobDesc& J, int r, int lc, int c0mH,
+                                                int ncl, int* h1) {
+    unsigned long long mag[NC + 2];
+    bool neg[NC + 2];
+    int w[NC + 2];
+    load_window<NC + 2>(lds, J, r, lc, c0mH, ncl, 0, mag, neg, w);
     int h0;
     const int d1 = restore_digit<NC>(mag, neg, w, h1);               // digit 2m+1
     const int d0 = restore_digit<NC>(mag + 1, neg + 1, w + 1, &h0);  // digit 2m
-    *hm1 = 0;
-    if (first) (void)restore_digit<NC>(mag + 2, neg + 2, w + 2, hm1);  // digit 2m-1: carry only
     return make_int2(d0, d1 + h0);
+}
+
+// Carry h_2m-1 out of digit 2m-1, the digit before element m = (row r,
+// local column lc): the ripple into the row's first own element.
+template <int NC>
+__device__ __forceinline__ int restore_carry_before(const double2* lds, const LucasJobDesc& J, int r, int lc,
+                                                    int c0mH, int ncl) {
+    unsigned long long mag[NC + 1];
+    bool neg[NC + 1];
+    int w[NC + 1];
+    load_window<NC + 1>(lds, J, r, lc, c0mH, ncl, 2, mag, neg, w);
+    int h;
+    (void)restore_digit<NC>(mag, neg, w, &h);
+    return h;
 }
 
 // ---------------------------------------------------------------------------
@@ -686,6 +710,8 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc& J, int tile,
     double2* const twW = p;  // wave-factor twiddles (fill_wave_table)
     p += lucas_wave_table_words(PA);
     signed char* const hb = reinterpret_cast<signed char*>(p);  // carries h_2m+1 of own elements, [A][C]
+    p += (A * C + 15) / 16;
+    signed char* const hf = reinterpret_cast<signed char*>(p);  // carry into each row's first own element, [A]
     // the job's maximum so far: a relaxed device-coherent load, first needed
     // by the wave reductions at the end, so no wait follows it here
     const unsigned int seen = __hip_atomic_load(J.maxerr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -792,33 +818,47 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc& J, int tile,
         // element, rows in order.  T is a multiple of C, so a pass holds whole
         // rows: its items read their own row's products and the halo of the
         // row before, which no pass writes.  Stage 1 restores digits 2m, 2m+1
-        // (the first element of a row also digit 2m-1 for its carry) and
-        // publishes h_2m+1 in hb; after the barrier stage 2 adds h_2m-1 from
-        // the element to the left and overwrites its own element with the
-        // weighted new digits (or stores them).
+        // and publishes h_2m+1 in hb; after the barrier stage 2 adds h_2m-1
+        // from the element to the left (for a row's first element, from hf)
+        // and overwrites its own element with the weighted new digits (or
+        // stores them).
         LUCAS_STAMP_COL(2);  // inverse FFT and unweight done
         const int c0mH = c0 - H;
+        // the carry into each row's first own element comes from its left
+        // neighbour in the halo columns, which no pass overwrites: one
+        // work-item per row, before the first barrier below
+        for (int r = tid; r < A; r += T) {
+            int h;
+            switch (n) {
+                case 2: h = restore_carry_before<2>(lds, J, r, H, c0mH, ncl); break;
+                case 3: h = restore_carry_before<3>(lds, J, r, H, c0mH, ncl); break;
+                case 4: h = restore_carry_before<4>(lds, J, r, H, c0mH, ncl); break;
+                case 5: h = restore_carry_before<5>(lds, J, r, H, c0mH, ncl); break;
+                default: h = restore_carry_before<6>(lds, J, r, H, c0mH, ncl); break;
+            }
+            hf[r] = (signed char)h;
+        }
         const int nper = (A * C + T - 1) / T;
         for (int i = 0; i < nper; ++i) {
             const int t = tid + i * T;
             const bool own = t < A * C;
             const int r = udiv(own ? t : 0, C, mC), lo = (own ? t : 0) - r * C;
             int2 x = make_int2(0, 0);
-            int h1 = 0, hm1 = 0;
+            int h1 = 0;
             if (own) {
                 const int lc = lo + H;
                 switch (n) {
-                    case 2: x = restore_element<2>(lds, J, r, lc, c0mH, ncl, lo == 0, &h1, &hm1); break;
-                    case 3: x = restore_element<3>(lds, J, r, lc, c0mH, ncl, lo == 0, &h1, &hm1); break;
-                    case 4: x = restore_element<4>(lds, J, r, lc, c0mH, ncl, lo == 0, &h1, &hm1); break;
-                    case 5: x = restore_element<5>(lds, J, r, lc, c0mH, ncl, lo == 0, &h1, &hm1); break;
-                    default: x = restore_element<6>(lds, J, r, lc, c0mH, ncl, lo == 0, &h1, &hm1); break;
+                    case 2: x = restore_element<2>(lds, J, r, lc, c0mH, ncl, &h1); break;
+                    case 3: x = restore_element<3>(lds, J, r, lc, c0mH, ncl, &h1); break;
+                    case 4: x = restore_element<4>(lds, J, r, lc, c0mH, ncl, &h1); break;
+                    case 5: x = restore_element<5>(lds, J, r, lc, c0mH, ncl, &h1); break;
+                    default: x = restore_element<6>(lds, J, r, lc, c0mH, ncl, &h1); break;
                 }
                 hb[t] = (signed char)h1;
             }
             __syncthreads();
             if (!own) continue;
-            x.x += lo == 0 ? hm1 : (int)hb[t - 1];
+            x.x += (int)(lo == 0 ? hf[r] : hb[t - 1]);
             if (mode & LUCAS_OUT_DIGITS) {
                 gstore2i(J.digits, (size_t)(c0 + lo) + (size_t)B * r, x);
                 continue;
