@@ -56,7 +56,9 @@ def headline(lucas, steps, warmup):
     info = ctx.info
     log(f"plan {info}")
     ctx.step(max(warmup, 1))  # untimed warm-up iterations: s fills all p bits
-    ctx.prepare(steps)  # set-up: build the replay graphs of a `steps` step (runs nothing)
+    # set-up: build the replay graph of a `steps`-iteration step and launch it
+    # once, restoring the state (a graph's first launch costs ~30 us extra)
+    ctx.prepare(steps)
     ctx.max_roundoff(reset=True)
     t0 = time.perf_counter()
     ctx.step(steps)  # ends with a device synchronisation

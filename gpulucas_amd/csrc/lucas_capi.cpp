@@ -442,6 +442,32 @@ int lucas_ctx_prepare(lucas_ctx* ctx, long long iters) {
     if (!ctx || iters < 0) return LUCAS_ERR_INVALID_ARG;
     int rc = ctx->engine->prepare_graphs(iters);
     if (rc == LUCAS_OK) rc = ctx->engine->sync();
+    if (rc != LUCAS_OK || iters == 0) return rc;
+    // Rehearsal: the first launch of a new graph costs tens of microseconds
+    // more than the next ones, so the step is run once here and the state
+    // (digits, roundoff and product-size accumulators, iteration count) put
+    // back as it was.
+    Job* j = ctx->job;
+    hipStream_t st = ctx->engine->stream();
+    const size_t dbytes = sizeof(int32_t) * (size_t)j->plan.N;
+    void* save = nullptr;
+    unsigned int acc[3] = {0, 0, 0};
+    if (hipMalloc(&save, dbytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return LUCAS_ERR_OOM;
+    }
+    hipError_t e = hipMemcpyAsync(save, j->desc.digits, dbytes, hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(acc, j->desc.maxerr, sizeof(acc), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    rc = e == hipSuccess ? ctx->engine->step(iters) : LUCAS_ERR_DEVICE;
+    if (rc == LUCAS_OK) {
+        j->iter -= iters;
+        e = hipMemcpyAsync(j->desc.digits, save, dbytes, hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(j->desc.maxerr, acc, sizeof(acc), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) rc = LUCAS_ERR_DEVICE;
+    }
+    hipFree(save);
     return rc;
 }
 
@@ -774,12 +800,22 @@ int lucas_batch_time_iterations(lucas_batch* b, long long iters, double* ms_per_
 
 // ---- serving ----------------------------------------------------------------
 //
-// Continuous batching: a worker thread owns the device.  Each tick it admits
-// queued requests (FIFO) while job slots and the HBM budget allow, advances
-// every resident job by one chunk in a single batched launch sequence, and
-// retires the jobs that reached their iteration count.  A job's arithmetic is
-// independent of its neighbours in the batch, so its residue is the same
-// alone or in any mix.
+// Continuous batching over three threads that share the device:
+//   * the admitter takes queued requests in order while job slots and the
+//     HBM budget allow, builds each job's tables and uploads them on its own
+//     stream (device memory from a pool of freed job blocks), and hands the
+//     resident job to the worker;
+//   * the worker owns the launch engine: each tick it adds the jobs that
+//     became resident, advances every resident job by one chunk in a single
+//     batched launch sequence, copies out the digits of the jobs that reached
+//     their iteration count and of those due a checkpoint, and drops the
+//     finished ones from the batch;
+//   * the retirer normalises finished residues (host O(N)), applies the
+//     roundoff gate, writes checkpoint files, and returns device memory and
+//     budget to the admitter.
+// So admission, retirement and checkpoint I/O overlap the device work.  A
+// job's arithmetic is independent of its neighbours in the batch, so its
+// residue is the same alone or in any mix.
 
 struct lucas_server {
     struct Pending {
@@ -792,17 +828,36 @@ struct lucas_server {
         Job* job;
         long long target;
     };
+    // work for the retirer: a finished job (its digits and roundoff already
+    // read), or a checkpoint of a running one
+    struct Retire {
+        long long id = 0;
+        Job* job = nullptr;       // finished job to release, or nullptr
+        int status = LUCAS_OK;
+        int p = 0, N = 0;
+        long long iter = 0;
+        double max_roundoff = 0;
+        std::vector<int32_t> digits;
+        std::string checkpoint;   // non-empty: write digits there
+    };
     lucas_server_options opt{};
     std::string ckdir;
+    double limit = LUCAS_DEFAULT_ERROR_LIMIT;
     Engine* engine = nullptr;
-    std::thread worker;
+    DevicePool* pool = nullptr;
+    hipStream_t admit_stream = nullptr;
+    std::thread worker, admitter, retirer;
     std::mutex mu;
-    std::condition_variable cv;       // work arrived / stop
-    std::condition_variable done_cv;  // a job finished
+    std::condition_variable cv;         // worker: jobs ready / stop
+    std::condition_variable admit_cv;   // admitter: queue or budget changed / stop
+    std::condition_variable retire_cv;  // retirer: work queued / stop
+    std::condition_variable done_cv;    // a job finished
     std::deque<Pending> queue;
+    std::deque<Active> ready;           // resident, not yet in the batch
+    std::deque<Retire> retiring;
     std::map<long long, lucas_job_result> records;
-    std::vector<Active> active;
     size_t budget = 0, used = 0;
+    int resident = 0;                   // jobs admitted and not yet retired
     long long next_id = 1;
     long long outstanding = 0;
     bool stopping = false;
@@ -813,50 +868,111 @@ struct lucas_server {
     int worker_rc = LUCAS_OK;
 
     void run();
+    void admit();
+    void retire();
     // <dir>/<p>_<id>.chk: two resident jobs of one exponent keep separate files
     std::string checkpoint_path(int p, long long id) const {
         return ckdir + "/" + std::to_string(p) + "_" + std::to_string(id) + ".chk";
     }
+    int max_jobs() const { return opt.max_jobs > 0 ? opt.max_jobs : 256; }
 };
+
+void lucas_server::admit() {
+    hipSetDevice(opt.device);
+    for (;;) {
+        Pending pnd;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            admit_cv.wait(lk, [&] {
+                if (stopping) return true;
+                if (queue.empty() || resident >= max_jobs()) return false;
+                // a job larger than what is free waits for memory, unless the
+                // device holds nothing (the submit check bounds it by the budget)
+                return used + job_bytes(queue.front().plan) <= budget || resident == 0;
+            });
+            if (stopping) return;
+            pnd = queue.front();
+            queue.pop_front();
+            used += job_bytes(pnd.plan);
+            ++resident;
+        }
+        Job* j = nullptr;
+        const int rc = job_create(pnd.plan, opt.device, admit_stream, &j, pool);
+        std::lock_guard<std::mutex> lk(mu);
+        lucas_job_result& rec = records[pnd.id];
+        rec.start_s = now_s() - t0;
+        if (rc != LUCAS_OK) {
+            used -= job_bytes(pnd.plan);
+            --resident;
+            rec.done = 1;
+            rec.status = rc;
+            rec.finish_s = rec.start_s;
+            --outstanding;
+            done_cv.notify_all();
+            admit_cv.notify_all();
+            continue;
+        }
+        ready.push_back({pnd.id, j, pnd.iters});
+        cv.notify_all();
+    }
+}
+
+void lucas_server::retire() {
+    for (;;) {
+        Retire w;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            retire_cv.wait(lk, [&] { return stopping || !retiring.empty(); });
+            if (retiring.empty()) return;  // stopping with nothing left
+            w = std::move(retiring.front());
+            retiring.pop_front();
+        }
+        if (!w.checkpoint.empty()) {
+            write_checkpoint(w.checkpoint.c_str(), w.p, w.N, w.iter, w.digits.data());
+            continue;
+        }
+        lucas_job_result res;
+        memset(&res, 0, sizeof(res));
+        res.status = w.status;
+        if (res.status == LUCAS_OK) {
+            normalize_digits(w.p, w.N, w.digits.data(), nullptr, &res.res64, &res.is_zero);
+            // the gate of errorTrial / mersenneTest (gpuLucas.cu:904-907, 1239-1242)
+            if (w.max_roundoff >= limit) res.status = LUCAS_ERR_ROUNDOFF;
+        }
+        const size_t bytes = w.job ? job_bytes(w.job->plan) : 0;
+        job_destroy(w.job, pool);
+        std::lock_guard<std::mutex> lk(mu);
+        lucas_job_result& rec = records[w.id];
+        rec.done = 1;
+        rec.status = res.status;
+        rec.res64 = res.res64;
+        rec.is_zero = res.is_zero;
+        rec.max_roundoff = w.max_roundoff;
+        rec.finish_s = now_s() - t0;
+        used -= bytes;
+        --resident;
+        --outstanding;
+        done_cv.notify_all();
+        admit_cv.notify_all();
+    }
+}
 
 void lucas_server::run() {
     hipSetDevice(opt.device);
-    const int max_jobs = opt.max_jobs > 0 ? opt.max_jobs : 256;
     const long long chunk_max = opt.chunk_iters > 0 ? opt.chunk_iters : 256;
-    const double limit = opt.error_limit > 0 ? opt.error_limit : LUCAS_DEFAULT_ERROR_LIMIT;
+    std::vector<Active> active;
     for (;;) {
-        std::vector<Pending> admit;
+        bool changed = false;
         {
             std::unique_lock<std::mutex> lk(mu);
-            cv.wait(lk, [&] { return stopping || !queue.empty() || !active.empty(); });
+            cv.wait(lk, [&] { return stopping || !ready.empty() || !active.empty(); });
             if (stopping) break;
-            while (!queue.empty() && (int)(active.size() + admit.size()) < max_jobs) {
-                const size_t need = job_bytes(queue.front().plan);
-                if (used + need > budget && !(active.empty() && admit.empty())) break;
-                used += need;
-                admit.push_back(queue.front());
-                queue.pop_front();
+            while (!ready.empty()) {
+                active.push_back(ready.front());
+                ready.pop_front();
+                changed = true;
             }
         }
-        bool changed = !admit.empty();
-        for (const Pending& pnd : admit) {
-            Job* j = nullptr;
-            int rc = job_create(pnd.plan, opt.device, engine->stream(), &j);
-            std::lock_guard<std::mutex> lk(mu);
-            lucas_job_result& rec = records[pnd.id];
-            rec.start_s = now_s() - t0;
-            if (rc != LUCAS_OK) {
-                used -= job_bytes(pnd.plan);
-                rec.done = 1;
-                rec.status = rc;
-                rec.finish_s = rec.start_s;
-                --outstanding;
-                done_cv.notify_all();
-                continue;
-            }
-            active.push_back({pnd.id, j, pnd.iters});
-        }
-        if (active.empty()) continue;
         if (changed) {
             std::vector<Job*> js;
             for (const Active& a : active) js.push_back(a.job);
@@ -874,58 +990,49 @@ void lucas_server::run() {
             iters_done += chunk * (long long)active.size();
             peak_jobs = std::max(peak_jobs, (int)active.size());
         }
-        // retire finished jobs (and checkpoint the others when due)
+        // finished jobs and due checkpoints: digits out, the rest to the retirer
         std::vector<Active> keep;
-        bool retired = false;
+        std::vector<Retire> out;
         for (Active& a : active) {
             const bool finished = a.job->iter >= a.target || rc != LUCAS_OK;
-            if (!finished) {
-                if (!ckdir.empty() && opt.checkpoint_every > 0 && a.job->iter % opt.checkpoint_every < chunk) {
-                    std::vector<int32_t> d(a.job->plan.N);
-                    if (job_get_digits(a.job, engine->stream(), d.data()) == LUCAS_OK) {
-                        std::string path = checkpoint_path(a.job->plan.p, a.id);
-                        write_checkpoint(path.c_str(), a.job->plan.p, a.job->plan.N, a.job->iter, d.data());
-                    }
-                }
+            const bool ckpt = !finished && !ckdir.empty() && opt.checkpoint_every > 0 &&
+                              a.job->iter % opt.checkpoint_every < chunk;
+            if (!finished && !ckpt) {
                 keep.push_back(a);
                 continue;
             }
-            lucas_job_result res;
-            memset(&res, 0, sizeof(res));
-            res.status = rc;
-            if (rc == LUCAS_OK) {
-                std::vector<int32_t> d(a.job->plan.N);
-                res.status = job_get_digits(a.job, engine->stream(), d.data());
-                if (res.status == LUCAS_OK) normalize_digits(a.job->plan.p, a.job->plan.N, d.data(), nullptr, &res.res64, &res.is_zero);
-                if (res.status == LUCAS_OK) res.status = job_read_maxerr(a.job, engine->stream(), 0, &res.max_roundoff);
-                // the gate of errorTrial / mersenneTest (gpuLucas.cu:904-907, 1239-1242)
-                if (res.status == LUCAS_OK && res.max_roundoff >= limit) res.status = LUCAS_ERR_ROUNDOFF;
+            Retire w;
+            w.id = a.id;
+            w.p = a.job->plan.p;
+            w.N = a.job->plan.N;
+            w.iter = a.job->iter;
+            w.status = rc;
+            w.digits.resize(a.job->plan.N);
+            if (w.status == LUCAS_OK) w.status = job_get_digits(a.job, engine->stream(), w.digits.data());
+            if (finished) {
+                if (w.status == LUCAS_OK) w.status = job_read_maxerr(a.job, engine->stream(), 0, &w.max_roundoff);
+                w.job = a.job;
+            } else {
+                keep.push_back(a);
+                if (w.status != LUCAS_OK) continue;
+                w.checkpoint = checkpoint_path(w.p, a.id);
             }
-            const size_t bytes = job_bytes(a.job->plan);
-            job_destroy(a.job);
-            retired = true;
-            std::lock_guard<std::mutex> lk(mu);
-            lucas_job_result& rec = records[a.id];
-            rec.done = 1;
-            rec.status = res.status;
-            rec.res64 = res.res64;
-            rec.is_zero = res.is_zero;
-            rec.max_roundoff = res.max_roundoff;
-            rec.finish_s = now_s() - t0;
-            used -= bytes;
-            --outstanding;
-            done_cv.notify_all();
+            out.push_back(std::move(w));
         }
-        if (retired) {
-            std::lock_guard<std::mutex> lk(mu);
+        if (keep.size() != active.size()) {
             active.swap(keep);
             std::vector<Job*> js;
             for (const Active& a : active) js.push_back(a.job);
             engine->set_jobs(js);
         }
+        if (!out.empty()) {
+            std::lock_guard<std::mutex> lk(mu);
+            for (Retire& w : out) retiring.push_back(std::move(w));
+            retire_cv.notify_all();
+        }
     }
     // shut down: release resident jobs
-    for (Active& a : active) job_destroy(a.job);
+    for (Active& a : active) job_destroy(a.job, pool);
     active.clear();
 }
 
@@ -937,8 +1044,11 @@ int lucas_server_start(const lucas_server_options* opt, lucas_server** out) {
     if (opt) s->opt = *opt;
     if (opt && opt->checkpoint_dir) s->ckdir = opt->checkpoint_dir;
     s->opt.checkpoint_dir = nullptr;
+    s->limit = s->opt.error_limit > 0 ? s->opt.error_limit : LUCAS_DEFAULT_ERROR_LIMIT;
     s->engine = new Engine(s->opt.device);
     int rc = s->engine->init();
+    if (rc == LUCAS_OK && hipStreamCreateWithFlags(&s->admit_stream, hipStreamNonBlocking) != hipSuccess)
+        rc = LUCAS_ERR_DEVICE;
     if (rc != LUCAS_OK) {
         delete s->engine;
         delete s;
@@ -947,8 +1057,12 @@ int lucas_server_start(const lucas_server_options* opt, lucas_server** out) {
     size_t freeb = 0, total = 0;
     if (hipMemGetInfo(&freeb, &total) != hipSuccess) freeb = 0;
     s->budget = s->opt.hbm_budget_bytes ? s->opt.hbm_budget_bytes : (size_t)(0.8 * (double)freeb);
+    // freed job blocks kept for reuse: up to a quarter of the budget
+    s->pool = new DevicePool(s->opt.device, s->budget / 4);
     s->t0 = now_s();
     s->worker = std::thread([s] { s->run(); });
+    s->admitter = std::thread([s] { s->admit(); });
+    s->retirer = std::thread([s] { s->retire(); });
     *out = s;
     return LUCAS_OK;
 }
@@ -982,7 +1096,7 @@ int lucas_server_submit(lucas_server* s, int p, int N, long long iterations, lon
     rec.submit_s = now_s() - s->t0;
     s->queue.push_back({jid, pl, iterations});
     ++s->outstanding;
-    s->cv.notify_all();
+    s->admit_cv.notify_all();
     if (id) *id = jid;
     return LUCAS_OK;
 }
@@ -1023,9 +1137,18 @@ int lucas_server_stop(lucas_server* s) {
         std::lock_guard<std::mutex> lk(s->mu);
         s->stopping = true;
         s->cv.notify_all();
+        s->admit_cv.notify_all();
+        s->retire_cv.notify_all();
     }
+    if (s->admitter.joinable()) s->admitter.join();
     if (s->worker.joinable()) s->worker.join();
+    if (s->retirer.joinable()) s->retirer.join();
+    // jobs admitted but never picked up, and retire work left over
+    for (lucas_server::Active& a : s->ready) job_destroy(a.job, s->pool);
+    for (lucas_server::Retire& w : s->retiring) job_destroy(w.job, s->pool);
     delete s->engine;
+    if (s->admit_stream) hipStreamDestroy(s->admit_stream);
+    delete s->pool;
     delete s;
     return LUCAS_OK;
 }

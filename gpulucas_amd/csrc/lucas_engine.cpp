@@ -36,7 +36,8 @@ hipError_t read_phase_stamps(unsigned long long* col, int ncol, unsigned long lo
 // column kernel with digits out).  The pairs run as graphs of 2^l pairs,
 // l = 0 .. kLevels - 1: graphs of 128 while at least 128 remain in a step of
 // kLongStep or more (measured 0.8 % faster per iteration than graphs of 32 at
-// N = 163840), otherwise the binary digits of the remainder, largest first.
+// N = 163840), graphs of 32 in a shorter step, then the binary digits of the
+// remainder, largest first.
 // So every iteration of every step is graph-launched; a step of k < 128 pairs
 // launches at most 2 + log2(k) graphs.  A single-job engine runs a step of
 // at most kWholeMax iterations as one whole-step graph instead (each graph
@@ -47,18 +48,70 @@ static const long long kLongStep = 512;
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-int job_create(const Plan& plan, int device, hipStream_t stream, Job** out) {
+DevicePool::~DevicePool() {
+    hipSetDevice(device_);
+    for (auto& kv : free_) hipFree(kv.second);
+}
+
+hipError_t DevicePool::acquire(size_t bytes, char** ptr) {
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        auto it = free_.find(bytes);
+        if (it != free_.end()) {
+            *ptr = it->second;
+            free_.erase(it);
+            cached_ -= bytes;
+            return hipSuccess;
+        }
+    }
+    hipSetDevice(device_);
+    hipError_t e = hipMalloc((void**)ptr, bytes);
+    if (e == hipErrorOutOfMemory) {
+        // give the cache back to the device and try once more
+        std::multimap<size_t, char*> drop;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            drop.swap(free_);
+            cached_ = 0;
+        }
+        (void)hipGetLastError();
+        for (auto& kv : drop) hipFree(kv.second);
+        e = hipMalloc((void**)ptr, bytes);
+    }
+    return e;
+}
+
+void DevicePool::release(char* ptr, size_t bytes) {
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (cached_ + bytes <= cap_) {
+            free_.emplace(bytes, ptr);
+            cached_ += bytes;
+            return;
+        }
+    }
+    hipSetDevice(device_);
+    hipFree(ptr);
+}
+
+int job_create(const Plan& plan, int device, hipStream_t stream, Job** out, DevicePool* pool) {
     if (hipSetDevice(device) != hipSuccess) return LUCAS_ERR_NO_DEVICE;
     Job* job = new Job();
     job->plan = plan;
     job->device = device;
     job->bytes = job_bytes(plan);
-    hipError_t e = hipMalloc((void**)&job->mem, job->bytes);
+    hipError_t e = pool ? pool->acquire(job->bytes, &job->mem) : hipMalloc((void**)&job->mem, job->bytes);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         delete job;
         return e == hipErrorOutOfMemory ? LUCAS_ERR_OOM : LUCAS_ERR_DEVICE;
     }
+    auto give_back = [&]() {
+        if (pool)
+            pool->release(job->mem, job->bytes);
+        else
+            hipFree(job->mem);
+    };
     Tables t;
     make_tables(plan, &t);
     // carve the allocation (layout mirrors job_bytes)
@@ -86,7 +139,7 @@ int job_create(const Plan& plan, int device, hipStream_t stream, Job** out) {
     const size_t o_twsB = take(sizeof(double) * t.twsB.size());
     const size_t o_err = take(sizeof(unsigned int));
     if (off > job->bytes) {
-        hipFree(job->mem);
+        give_back();
         delete job;
         return LUCAS_ERR_INVALID_ARG;
     }
@@ -105,10 +158,11 @@ int job_create(const Plan& plan, int device, hipStream_t stream, Job** out) {
     memcpy(host.data() + o_iwhi, t.iwhi.data(), sizeof(double) * t.iwhi.size());
     memcpy(host.data() + o_twsA, t.twsA.data(), sizeof(double) * t.twsA.size());
     memcpy(host.data() + o_twsB, t.twsB.data(), sizeof(double) * t.twsB.size());
+    // the accumulators, and a reused block's old state, are overwritten here
     e = hipMemcpyAsync(job->mem + o_dig, host.data() + o_dig, job->bytes - o_dig, hipMemcpyHostToDevice, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e != hipSuccess) {
-        hipFree(job->mem);
+        give_back();
         delete job;
         return LUCAS_ERR_DEVICE;
     }
@@ -137,11 +191,15 @@ int job_create(const Plan& plan, int device, hipStream_t stream, Job** out) {
     return LUCAS_OK;
 }
 
-void job_destroy(Job* job) {
+void job_destroy(Job* job, DevicePool* pool) {
     if (!job) return;
     if (job->mem) {
-        hipSetDevice(job->device);
-        hipFree(job->mem);
+        if (pool) {
+            pool->release(job->mem, job->bytes);
+        } else {
+            hipSetDevice(job->device);
+            hipFree(job->mem);
+        }
     }
     delete job;
 }
@@ -418,11 +476,16 @@ static void step_plan(long long iters, std::vector<int>* slots) {
     slots->clear();
     slots->push_back(Engine::kHead);
     long long rem = iters - 1;
-    const int top = kLevels - 1;
-    if (iters >= kLongStep)
-        for (; rem >= (1LL << top); rem -= 1LL << top) slots->push_back(top);
-    for (int l = top; l >= 0; --l)
-        for (; rem >= (1LL << l); rem -= 1LL << l) slots->push_back(l);
+    // the largest graph a step replays: 128 pairs in a long step, 32 in a
+    // shorter one (a server's chunks, whose job mix changes often, so that a
+    // mix change rebuilds graphs of at most 63 iterations in all)
+    const int top = iters >= kLongStep ? kLevels - 1 : 5;
+    for (; rem >= (1LL << top); rem -= 1LL << top) slots->push_back(top);
+    for (int l = top - 1; l >= 0; --l)
+        if (rem >= (1LL << l)) {
+            slots->push_back(l);
+            rem -= 1LL << l;
+        }
     slots->push_back(Engine::kTail);
 }
 

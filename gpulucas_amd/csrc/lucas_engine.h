@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "lucas_desc.h"
@@ -22,9 +24,32 @@ struct Job {
     long long iter = 0;
 };
 
-// Creates a job with s_0 = 4 (digits = [4, 0, ...]).  Returns a LUCAS_* code.
-int job_create(const Plan& plan, int device, hipStream_t stream, Job** out);
-void job_destroy(Job* job);
+// Device allocations of jobs that come and go (a server's): freed blocks
+// are kept by exact size, up to `cap` bytes in all, and handed to the next
+// job of that size instead of going through hipFree / hipMalloc.
+class DevicePool {
+public:
+    DevicePool(int device, size_t cap) : device_(device), cap_(cap) {}
+    ~DevicePool();
+    // A block of exactly `bytes` (cached or new); hipErrorOutOfMemory when
+    // neither is possible.
+    hipError_t acquire(size_t bytes, char** ptr);
+    // Returns a block; kept while the cache stays within cap, else freed.
+    void release(char* ptr, size_t bytes);
+    size_t cached() const { return cached_; }
+
+private:
+    int device_;
+    size_t cap_, cached_ = 0;
+    std::mutex mu_;
+    std::multimap<size_t, char*> free_;
+};
+
+// Creates a job with s_0 = 4 (digits = [4, 0, ...]) and uploads its tables on
+// `stream`; device memory from `pool` when given.  Returns a LUCAS_* code.
+int job_create(const Plan& plan, int device, hipStream_t stream, Job** out, DevicePool* pool = nullptr);
+// Releases a job's device memory (to `pool` when given) and the job.
+void job_destroy(Job* job, DevicePool* pool = nullptr);
 int job_get_digits(Job* job, hipStream_t stream, int32_t* digits);
 int job_set_digits(Job* job, hipStream_t stream, const int32_t* digits);
 int job_read_maxerr(Job* job, hipStream_t stream, int reset, double* out);

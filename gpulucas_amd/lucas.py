@@ -328,7 +328,8 @@ class Context:
         return b.value, z.value
 
     def prepare(self, iters: int):
-        """Build the replay graphs a step of `iters` iterations launches."""
+        """Make a step of `iters` iterations launch-ready: build its replay
+        graphs and run it once, restoring the state afterwards."""
         _check(_lib().lucas_ctx_prepare(self._h, iters), f"prepare({iters})")
 
     def time_kernels(self, iters: int = 100) -> tuple[float, float]:

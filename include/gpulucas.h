@@ -156,10 +156,12 @@ int lucas_ctx_max_roundoff(lucas_ctx* ctx, int reset, double* out);
  * leading-zero count of |c_j|, the value the reference prints as "max Bit
  * Vector".  Either pointer may be NULL. */
 int lucas_ctx_product_bits(lucas_ctx* ctx, int reset, int* max_bits, int* max_clz);
-/* Builds the replay graphs a step of `iters` iterations launches without
- * running them (no reference counterpart: the reference launched every
- * kernel from the host, gpuLucas.cu:845-942), so that a later timed step of
- * that length instantiates nothing. */
+/* Makes a step of `iters` iterations launch-ready (no reference
+ * counterpart: the reference launched every kernel from the host,
+ * gpuLucas.cu:845-942): builds and uploads its replay graphs and runs the
+ * step once, then restores the state (digits, roundoff and product-size
+ * accumulators, iteration count), so that the next lucas_ctx_step of that
+ * length pays no graph instantiation or first-launch cost. */
 int lucas_ctx_prepare(lucas_ctx* ctx, long long iters);
 /* Mean time per kernel launch over `iters` iterations launched as in
  * lucas_ctx_step (row kernel, column kernel, alternating), from HIP events

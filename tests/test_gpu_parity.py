@@ -395,3 +395,22 @@ def test_checkpoint_resume_identical(tmp_path):
     # full test with periodic checkpoints then resume from the file
     r1 = lucas.mersenne_test(p, N, checkpoint_every=20000, checkpoint_path=path)
     assert r1["is_prime"] == 1
+
+
+def test_prepare_rehearses_and_restores_the_state(gmp_fixtures):
+    """lucas_ctx_prepare builds and runs a step once, then puts the state
+    back: digits, iteration count and accumulators are unchanged, and the
+    next step reaches GMP's residue."""
+    p, N = 86243, 4608
+    with lucas.Context(p, N) as ctx:
+        ctx.step(600)
+        x = ctx.digits()
+        err = ctx.max_roundoff()
+        bits = ctx.product_bits()
+        for k in (20, 400):
+            ctx.prepare(k)
+            assert ctx.iteration == 600
+            assert np.array_equal(ctx.digits(), x)
+            assert ctx.max_roundoff() == err and ctx.product_bits() == bits
+        ctx.step(400)
+        assert ctx.res64() == fixture_res64(gmp_fixtures, "c1_p86243", p, 1000)
