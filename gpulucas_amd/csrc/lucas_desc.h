@@ -56,17 +56,23 @@ struct LucasJobDesc {
     // L1 in LDS and a W = 64P point wave-register factor (P = 0: no wave
     // factor, L1 = L).
     int pA, pB;             // P of the column (length A) and row (length B) transforms
+    int ohA, ohB;           // ceil(A/64), ceil(B/64): row lengths of ct2 and rt2
     LucasStage stA[LUCAS_MAX_RADICES];  // column transform Stockham stages (length A1)
     LucasStage stB[LUCAS_MAX_RADICES];  // row transform Stockham stages (length B1)
     unsigned int m_ncl, m_C;             // reciprocals of C+halo and C
     unsigned int m_A1, m_B1;             // reciprocals of A1 and B1
     unsigned int m_sqA0, m_sqA1;         // reciprocals of (C+halo)*64pA and C*64pA
-    double invB;                         // 1/B
     // Twiddle tables (forward sign, exp(-2*pi*i*t/L)), rounded from long double.
     const double2* twA;     // omega_A^t, t < A
     const double2* twB;     // omega_B^t, t < B
     const double2* twMA;    // omega_M^t, t < A
-    const double2* twMB;    // omega_M^t, t < B
+    const double2* twMB;    // omega_M^t, t < B (column kernels below the dense size)
+    // Outer twiddles omega_M^(c*r) = ct1[c][r mod 64] * ct2[c][r div 64]
+    // (column kernel) = rt1[r][c mod 64] * rt2[r][c div 64] (row kernel).
+    const double2* ct1;     // [B][64]: omega_M^(c*rl)
+    const double2* ct2;     // [B][ceil(A/64)]: omega_M^(64*c*rh)
+    const double2* rt1;     // [A][64]: omega_M^(r*cl)
+    const double2* rt2;     // [A][ceil(B/64)]: omega_M^(64*r*ch)
     const double2* twsA;    // stage twiddles of the length-A transform (see LucasStage)
     const double2* twsB;    // stage twiddles of the length-B transform
     // Weight tables: j = Jlo + 2B*r;  r_j = (rlo[Jlo] + rhi[r]) mod N.

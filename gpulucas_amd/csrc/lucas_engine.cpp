@@ -114,52 +114,23 @@ int job_create(const Plan& plan, int device, hipStream_t stream, Job** out, Devi
     };
     Tables t;
     make_tables(plan, &t);
-    // carve the allocation (layout mirrors job_bytes)
-    std::vector<char> host(job->bytes, 0);
-    size_t off = 0;
-    auto take = [&](size_t n) {
-        size_t o = off;
-        off += align256(n);
-        return o;
-    };
-    const size_t o_spec = take(sizeof(double2) * plan.M);
-    const size_t o_spec2 = take(sizeof(double2) * plan.M);
-    const size_t o_dig = take(sizeof(int) * plan.N);
-    const size_t o_twA = take(sizeof(double2) * plan.A);
-    const size_t o_twMA = take(sizeof(double2) * plan.A);
-    const size_t o_twB = take(sizeof(double2) * plan.B);
-    const size_t o_twMB = take(sizeof(double2) * plan.B);
-    const size_t o_rlo = take(sizeof(int) * 2 * plan.B);
-    const size_t o_rhi = take(sizeof(int) * plan.A);
-    const size_t o_wlo = take(sizeof(double) * 2 * plan.B);
-    const size_t o_iwlo = take(sizeof(double) * 2 * plan.B);
-    const size_t o_whi = take(sizeof(double) * plan.A);
-    const size_t o_iwhi = take(sizeof(double) * plan.A);
-    const size_t o_twsA = take(sizeof(double) * t.twsA.size());
-    const size_t o_twsB = take(sizeof(double) * t.twsB.size());
-    const size_t o_err = take(sizeof(unsigned int));
-    if (off > job->bytes) {
+    // spectra, digits, then the table block (layout mirrors job_bytes)
+    const size_t o_spec = 0;
+    const size_t o_spec2 = align256(sizeof(double2) * plan.M);
+    const size_t o_dig = 2 * o_spec2;
+    const size_t o_tab = o_dig + align256(sizeof(int) * plan.N);
+    const TableLayout lay = table_layout(plan);
+    if (o_tab + lay.bytes > job->bytes) {
         give_back();
         delete job;
         return LUCAS_ERR_INVALID_ARG;
     }
     // Host image of everything from the digits on (spectra need no init).
-    int* dig = (int*)(host.data() + o_dig);
-    dig[0] = 4;  // s_0 = 4 with a_0 = 1 (loadValue4ToFFTarray, gpuLucas.cu:718-726)
-    memcpy(host.data() + o_twA, t.twA.data(), sizeof(double) * t.twA.size());
-    memcpy(host.data() + o_twMA, t.twMA.data(), sizeof(double) * t.twMA.size());
-    memcpy(host.data() + o_twB, t.twB.data(), sizeof(double) * t.twB.size());
-    memcpy(host.data() + o_twMB, t.twMB.data(), sizeof(double) * t.twMB.size());
-    memcpy(host.data() + o_rlo, t.rlo.data(), sizeof(int) * t.rlo.size());
-    memcpy(host.data() + o_rhi, t.rhi.data(), sizeof(int) * t.rhi.size());
-    memcpy(host.data() + o_wlo, t.wlo.data(), sizeof(double) * t.wlo.size());
-    memcpy(host.data() + o_iwlo, t.iwlo.data(), sizeof(double) * t.iwlo.size());
-    memcpy(host.data() + o_whi, t.whi.data(), sizeof(double) * t.whi.size());
-    memcpy(host.data() + o_iwhi, t.iwhi.data(), sizeof(double) * t.iwhi.size());
-    memcpy(host.data() + o_twsA, t.twsA.data(), sizeof(double) * t.twsA.size());
-    memcpy(host.data() + o_twsB, t.twsB.data(), sizeof(double) * t.twsB.size());
+    std::vector<char> host(job->bytes - o_dig, 0);
+    ((int*)host.data())[0] = 4;  // s_0 = 4 with a_0 = 1 (loadValue4ToFFTarray, gpuLucas.cu:718-726)
+    table_image(plan, t, lay, host.data() + (o_tab - o_dig));
     // the accumulators, and a reused block's old state, are overwritten here
-    e = hipMemcpyAsync(job->mem + o_dig, host.data() + o_dig, job->bytes - o_dig, hipMemcpyHostToDevice, stream);
+    e = hipMemcpyAsync(job->mem + o_dig, host.data(), host.size(), hipMemcpyHostToDevice, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e != hipSuccess) {
         give_back();
@@ -172,20 +143,7 @@ int job_create(const Plan& plan, int device, hipStream_t stream, Job** out, Devi
     d.spec = (double2*)(m + o_spec);
     d.spec2 = (double2*)(m + o_spec2);
     d.digits = (int*)(m + o_dig);
-    d.twA = (const double2*)(m + o_twA);
-    d.twMA = (const double2*)(m + o_twMA);
-    d.twB = (const double2*)(m + o_twB);
-    d.twMB = (const double2*)(m + o_twMB);
-    d.rlo = (const int*)(m + o_rlo);
-    d.rhi = (const int*)(m + o_rhi);
-    d.wlo = (const double*)(m + o_wlo);
-    d.iwlo = (const double*)(m + o_iwlo);
-    d.whi = (const double*)(m + o_whi);
-    d.iwhi = (const double*)(m + o_iwhi);
-    d.twsA = (const double2*)(m + o_twsA);
-    d.twsB = (const double2*)(m + o_twsB);
-    d.maxerr = (unsigned int*)(m + o_err);
-    d.pbits = d.maxerr + 1;  // same 256-byte block, zeroed with it
+    desc_tables(lay, m + o_tab, &d);
     job->iter = 0;
     *out = job;
     return LUCAS_OK;

@@ -375,6 +375,8 @@ void fill_desc_shape(const Plan& plan, LucasJobDesc* d) {
     d->ntwsB = stage_twiddle_count(B1, plan.radB);
     d->pA = plan.wA / 64;
     d->pB = plan.wB / 64;
+    d->ohA = outer_hi_count(plan.A);
+    d->ohB = outer_hi_count(plan.B);
     stage_table(A1, plan.radA, d->stA);
     stage_table(B1, plan.radB, d->stB);
     d->m_A1 = magic_u32((unsigned)A1);
@@ -383,24 +385,93 @@ void fill_desc_shape(const Plan& plan, LucasJobDesc* d) {
     d->m_sqA1 = magic_u32((unsigned)(plan.C * std::max(1, plan.wA)));
     d->m_ncl = magic_u32((unsigned)(plan.C + plan.halo));
     d->m_C = magic_u32((unsigned)plan.C);
-    d->invB = 1.0 / (double)plan.B;
 }
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+TableLayout table_layout(const Plan& pl) {
+    TableLayout l;
+    size_t off = 0;
+    auto take = [&](size_t n) {
+        const size_t o = off;
+        off += align256(n);
+        return o;
+    };
+    const size_t c2 = sizeof(double) * 2;  // one complex entry
+    const int A1 = pl.wA ? pl.A / pl.wA : pl.A, B1 = pl.wB ? pl.B / pl.wB : pl.B;
+    l.twA = take(c2 * pl.A);
+    l.twMA = take(c2 * pl.A);
+    l.twB = take(c2 * pl.B);
+    // the column kernel's outer twiddles: two-factor rows for the dense
+    // kernels, omega_M^t (t < B) below that
+    l.twMB = take(pl.col_dense ? 0 : c2 * pl.B);
+    l.ct1 = take(pl.col_dense ? c2 * pl.B * kOuterLo : 0);
+    l.ct2 = take(pl.col_dense ? c2 * pl.B * outer_hi_count(pl.A) : 0);
+    l.rt1 = take(c2 * pl.A * kOuterLo);
+    l.rt2 = take(c2 * pl.A * outer_hi_count(pl.B));
+    l.rlo = take(sizeof(int) * 2 * pl.B);
+    l.rhi = take(sizeof(int) * pl.A);
+    l.wlo = take(sizeof(double) * 2 * pl.B);
+    l.iwlo = take(sizeof(double) * 2 * pl.B);
+    l.whi = take(sizeof(double) * pl.A);
+    l.iwhi = take(sizeof(double) * pl.A);
+    l.twsA = take(c2 * std::max(1, stage_twiddle_count(A1, pl.radA)));
+    l.twsB = take(c2 * std::max(1, stage_twiddle_count(B1, pl.radB)));
+    l.acc = take(256);  // roundoff maximum and product-size diagnostic
+    l.bytes = off;
+    return l;
+}
+
+void table_image(const Plan& pl, const Tables& t, const TableLayout& l, char* img) {
+    (void)pl;
+    memset(img, 0, l.bytes);
+    auto put = [&](size_t o, const void* src, size_t n) {
+        if (n) memcpy(img + o, src, n);
+    };
+    put(l.twA, t.twA.data(), sizeof(double) * t.twA.size());
+    put(l.twMA, t.twMA.data(), sizeof(double) * t.twMA.size());
+    put(l.twB, t.twB.data(), sizeof(double) * t.twB.size());
+    put(l.twMB, t.twMB.data(), sizeof(double) * t.twMB.size());
+    put(l.ct1, t.ct1.data(), sizeof(double) * t.ct1.size());
+    put(l.ct2, t.ct2.data(), sizeof(double) * t.ct2.size());
+    put(l.rt1, t.rt1.data(), sizeof(double) * t.rt1.size());
+    put(l.rt2, t.rt2.data(), sizeof(double) * t.rt2.size());
+    put(l.rlo, t.rlo.data(), sizeof(int) * t.rlo.size());
+    put(l.rhi, t.rhi.data(), sizeof(int) * t.rhi.size());
+    put(l.wlo, t.wlo.data(), sizeof(double) * t.wlo.size());
+    put(l.iwlo, t.iwlo.data(), sizeof(double) * t.iwlo.size());
+    put(l.whi, t.whi.data(), sizeof(double) * t.whi.size());
+    put(l.iwhi, t.iwhi.data(), sizeof(double) * t.iwhi.size());
+    put(l.twsA, t.twsA.data(), sizeof(double) * t.twsA.size());
+    put(l.twsB, t.twsB.data(), sizeof(double) * t.twsB.size());
+}
+
+void desc_tables(const TableLayout& l, char* m, LucasJobDesc* d) {
+    d->twA = (const double2*)(m + l.twA);
+    d->twMA = (const double2*)(m + l.twMA);
+    d->twB = (const double2*)(m + l.twB);
+    d->twMB = (const double2*)(m + l.twMB);
+    d->ct1 = (const double2*)(m + l.ct1);
+    d->ct2 = (const double2*)(m + l.ct2);
+    d->rt1 = (const double2*)(m + l.rt1);
+    d->rt2 = (const double2*)(m + l.rt2);
+    d->rlo = (const int*)(m + l.rlo);
+    d->rhi = (const int*)(m + l.rhi);
+    d->wlo = (const double*)(m + l.wlo);
+    d->iwlo = (const double*)(m + l.iwlo);
+    d->whi = (const double*)(m + l.whi);
+    d->iwhi = (const double*)(m + l.iwhi);
+    d->twsA = (const double2*)(m + l.twsA);
+    d->twsB = (const double2*)(m + l.twsB);
+    d->maxerr = (unsigned int*)(m + l.acc);
+    d->pbits = d->maxerr + 1;
+}
 
 size_t job_bytes(const Plan& pl) {
     size_t s = 0;
     s += align256(sizeof(double) * 2 * (size_t)pl.M) * 2;  // spec, spec2
     s += align256(sizeof(int) * (size_t)pl.N);             // digits
-    s += align256(sizeof(double) * 2 * (size_t)pl.A) * 2;  // twA, twMA
-    s += align256(sizeof(double) * 2 * (size_t)pl.B) * 2;  // twB, twMB
-    s += align256(sizeof(int) * 2 * (size_t)pl.B) + align256(sizeof(int) * (size_t)pl.A);
-    s += align256(sizeof(double) * 2 * (size_t)pl.B) * 2;  // wlo, iwlo
-    s += align256(sizeof(double) * (size_t)pl.A) * 2;      // whi, iwhi
-    s += align256(sizeof(double) * 2 * (size_t)std::max(1, stage_twiddle_count(pl.A, pl.radA)));
-    s += align256(sizeof(double) * 2 * (size_t)std::max(1, stage_twiddle_count(pl.B, pl.radB)));
-    s += 256;                                               // maxerr
-    return s;
+    return s + table_layout(pl).bytes;
 }
 
 static void root_table(long long count, long long denom, std::vector<double>* out) {
@@ -438,7 +509,25 @@ void make_tables(const Plan& pl, Tables* t) {
     root_table(pl.A, pl.A, &t->twA);
     root_table(pl.B, pl.B, &t->twB);
     root_table(pl.A, pl.M, &t->twMA);
-    root_table(pl.B, pl.M, &t->twMB);
+    // outer twiddles omega_M^e, e = x*y mod M, in two factors per index
+    auto outer = [&](std::vector<double>* out, int n_outer, int n_inner, long long step) {
+        out->resize(2 * (size_t)n_outer * n_inner);
+        for (long long x = 0; x < n_outer; ++x)
+            for (long long y = 0; y < n_inner; ++y) {
+                const long long e = (x * y * step) % (long long)pl.M;
+                const long double ang = 2.0L * kPi * (long double)e / (long double)pl.M;
+                (*out)[2 * (x * n_inner + y)] = (double)cosl(ang);
+                (*out)[2 * (x * n_inner + y) + 1] = (double)(-sinl(ang));
+            }
+    };
+    if (pl.col_dense) {
+        outer(&t->ct1, pl.B, kOuterLo, 1);
+        outer(&t->ct2, pl.B, outer_hi_count(pl.A), kOuterLo);
+    } else {
+        root_table(pl.B, pl.M, &t->twMB);
+    }
+    outer(&t->rt1, pl.A, kOuterLo, 1);
+    outer(&t->rt2, pl.A, outer_hi_count(pl.B), kOuterLo);
     const long long N = pl.N, p = pl.p;
     t->rlo.resize(2 * (size_t)pl.B);
     t->wlo.resize(2 * (size_t)pl.B);

@@ -73,14 +73,34 @@ void fill_desc_shape(const Plan& plan, LucasJobDesc* d);
 // Device footprint of one job under `plan` (state + tables), bytes.
 size_t job_bytes(const Plan& plan);
 
+// Outer twiddles omega_M^(c*r) (c < B, r < A) as the product of two table
+// entries: split a row index r = 64*rh + rl (column kernel) or a column
+// index c = 64*ch + cl (row kernel).
+static const int kOuterLo = 64;
+inline int outer_hi_count(int L) { return (L + kOuterLo - 1) / kOuterLo; }
+
 // Tables (host copies).
 struct Tables {
     std::vector<double> twA, twB, twMA, twMB;  // interleaved re,im
+    std::vector<double> ct1, ct2;              // [c][rl]: omega_M^(c*rl); [c][rh]: omega_M^(c*64*rh)
+    std::vector<double> rt1, rt2;              // [r][cl]: omega_M^(r*cl); [r][ch]: omega_M^(r*64*ch)
     std::vector<double> twsA, twsB;            // stage twiddles, interleaved
     std::vector<int> rlo, rhi;
     std::vector<double> wlo, whi, iwlo, iwhi;
 };
 void make_tables(const Plan& plan, Tables* t);
+
+// Device layout of a job's tables and accumulators: byte offsets (from the
+// start of the table block, each 256-aligned) and the block's size.
+struct TableLayout {
+    size_t twA, twMA, twB, twMB, ct1, ct2, rt1, rt2, rlo, rhi, wlo, iwlo, whi, iwhi, twsA, twsB, acc;
+    size_t bytes;
+};
+TableLayout table_layout(const Plan& plan);
+// Host image of the table block (layout.bytes, accumulators zero).
+void table_image(const Plan& plan, const Tables& t, const TableLayout& lay, char* img);
+// Points a descriptor's table and accumulator fields into the block at `base`.
+void desc_tables(const TableLayout& lay, char* base, LucasJobDesc* d);
 
 // r_j = (-j p) mod N.
 inline long long r_of(long long j, long long p, long long N) {

@@ -405,22 +405,19 @@ __device__ __forceinline__ void xlevel_all(double2 (&v)[P], const double2* tab, 
 // LDS twiddle table of a W = 64P point wave factor (lucas_wave_table_words(P)
 // entries): omega_W^u for u < W/2 (in-register levels), then for each
 // cross-lane half-size H = 1, 2, .., 32 the H entries omega_2H^t at offset
-// W/2 + H - 1.  Filled from the global table gtw of omega_L (L = L1*W).
-template <int T, int P>
-__device__ __forceinline__ void fill_wave_table(double2* twW, const double2* gtw, int L1, int tid) {
-    for (int t = tid; t < 32 * P + 63; t += T) {
-        int u = t;
-        if (t >= 32 * P) {
-            const int i = t - 32 * P;                // (H - 1) + tt
-            const int H = 1 << (31 - __clz(i + 1));  // largest power of two <= i + 1
-            u = (i - (H - 1)) * (32 * P / H);        // omega_2H^tt = omega_W^(tt*W/(2H))
-        }
-        twW[t] = gload2(gtw, (size_t)u * L1);
-    }
+// W/2 + H - 1.  The kernels fill it from their global table of omega_L
+// (L = L1*W): entry t is omega_L^(u*L1), u = wave_table_exponent<P>(t).
+// Exponent u (of omega_W) of entry t of that table.
+template <int P>
+__device__ __forceinline__ int wave_table_exponent(int t) {
+    if (t < 32 * P) return t;
+    const int i = t - 32 * P;                // (H - 1) + tt
+    const int H = 1 << (31 - __clz(i + 1));  // largest power of two <= i + 1
+    return (i - (H - 1)) * (32 * P / H);     // omega_2H^tt = omega_W^(tt*W/(2H))
 }
 
 // W = 64P point DIF of v (element 64m + lane in v[m]), twiddles from the
-// wave table twW (fill_wave_table).  Afterwards v[m] holds
+// wave table twW (see wave_table_exponent).  Afterwards v[m] holds
 // X[brev6(lane)*P + brev_slot(m)].
 template <int P>
 __device__ __forceinline__ void wave_dif(double2 (&v)[P], const double2* twW, int lane) {
@@ -507,21 +504,6 @@ __device__ __forceinline__ void fft_seq(double2* lds, const LucasStage* stages, 
     }
 }
 
-// omega_M^e for 0 <= e < M, as omega_A^(e / B) * omega_M^(e mod B); the
-// quotient comes from a double reciprocal of B plus one correction.
-__device__ __forceinline__ double2 outer_twiddle(const LucasJobDesc& J, const double2* twA, const double2* twMB,
-                                                 int e) {
-    int hi = (int)((double)e * J.invB);
-    int lo = e - hi * J.B;
-    if (lo < 0) {
-        --hi;
-        lo += J.B;
-    } else if (lo >= J.B) {
-        ++hi;
-        lo -= J.B;
-    }
-    return cmul(gload2(twA, hi), gload2(twMB, lo));
-}
 
 // Maximum over a wave of a non-negative value: DPP quad permutes and row
 // rotates take the max within each 16-lane row (no LDS crossbar), four lane
@@ -681,7 +663,9 @@ __device__ __forceinline__ int restore_carry_before(const double2* lds, const Lu
 // data (stage twiddles, weight and r tables of the tile's rows and digits) is
 // copied into LDS while the tile's own loads are in flight.
 // ---------------------------------------------------------------------------
-template <int T, int PA>
+// DENSE: built for two 1024-item workgroups per CU (64 VGPRs), which loads
+// its tile in smaller batches.
+template <int T, int PA, bool DENSE = false>
 __device__ __forceinline__ void col_kernel_body(const LucasJobDesc& J, int tile, int mode) {
     LUCAS_STAMP_COL(0);
     double2* const lds = lucas_dynamic_lds();
@@ -707,7 +691,7 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc& J, int tile,
     p += 2 * J.nradA;
     double2* const twL = p;  // omega_A^t, t < A (wave factor only)
     p += PA ? A : 0;
-    double2* const twW = p;  // wave-factor twiddles (fill_wave_table)
+    double2* const twW = p;  // wave-factor twiddles (wave_table_exponent)
     p += lucas_wave_table_words(PA);
     signed char* const hb = reinterpret_cast<signed char*>(p);  // carries h_2m+1 of own elements, [A][C]
     p += (A * C + 15) / 16;
@@ -718,42 +702,125 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc& J, int tile,
     const unsigned int seen_b = __hip_atomic_load(J.pbits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned int seen_z = __hip_atomic_load(J.pbits + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const double2* const twA = J.twA;
-    const double2* const twMB = J.twMB;
     const bool in_digits = (mode & LUCAS_IN_DIGITS) != 0;
     float err = 0.0f;
     double pmax = 0.0, pmin = 0x1p60;  // largest / smallest |c_j| of this work-item
 
-    if (!in_digits) {
-        // load the tile with the inverse outer twiddle omega_M^(-c*r), conjugated
-        for (int t = tid; t < A * ncl; t += T) {
+    // The prologue issues every global load before it uses any result: a
+    // loop that loads, waits and stores one element at a time keeps a single
+    // request per work-item in flight, and the kernel then moves its tile at
+    // a fraction of the HBM bandwidth.  The tile goes in batches of TB
+    // elements per work-item (values and outer-twiddle factors in registers),
+    // the first batch in flight while the tables are copied.
+    constexpr int TB = DENSE ? 2 : 4;
+    constexpr int wA = 64 * PA;
+    const int A1 = PA ? A / wA : A;
+    const int ntile = in_digits ? 0 : A * ncl;
+    double2 tv[TB], ta[TB], tm[TB];
+    // tile element t = (row r, local column lc) with the inverse outer
+    // twiddle omega_M^(-c*r), conjugated.  omega_M^(c*r) is a product of two
+    // table entries: for the dense kernels (M of a million and more) one
+    // entry of each of the column's two twiddle rows (ct1, ct2), a few KB
+    // per workgroup; below that omega_A^(e div B) * omega_M^(e mod B) with
+    // e = c*r, from two tables of A and B entries that stay in L1 / L2.
+    const double inv_B = 1.0 / (double)B;
+    auto tile_issue = [&](int t0) {
+#pragma unroll
+        for (int k = 0; k < TB; ++k) {
+            const int t = t0 + tid + k * T;
+            if (t >= ntile) continue;
             const int r = udiv(t, ncl, mncl), lc = t - r * ncl;
             int gc = c0 - H + lc;
             if (gc < 0) gc += B;
-            const double2 v = gload2(J.spec2, (size_t)r * B + gc);
-            lds[pidx(t)] = cconj(cmulc(v, outer_twiddle(J, twA, twMB, gc * r)));
+            tv[k] = gload2(J.spec2, (size_t)r * B + gc);
+            if constexpr (DENSE) {
+                ta[k] = gload2(J.ct1, (size_t)gc * 64 + (r & 63));
+                tm[k] = gload2(J.ct2, (size_t)gc * J.ohA + (r >> 6));
+            } else {
+                // e < M < 2^31; the quotient from the double reciprocal is
+                // within one of e div B
+                const int e = gc * r;
+                int hi = (int)((double)e * inv_B);
+                int lo = e - hi * B;
+                if (lo < 0) {
+                    --hi;
+                    lo += B;
+                } else if (lo >= B) {
+                    ++hi;
+                    lo -= B;
+                }
+                ta[k] = gload2(twA, hi);
+                tm[k] = gload2(J.twMB, lo);
+            }
+        }
+    };
+    auto tile_store = [&](int t0) {
+#pragma unroll
+        for (int k = 0; k < TB; ++k) {
+            const int t = t0 + tid + k * T;
+            if (t < ntile) lds[pidx(t)] = cconj(cmulc(tv[k], cmul(ta[k], tm[k])));
+        }
+    };
+    // The load counter drains in issue order, so the tables' loads go first:
+    // waiting for them never waits for the tile's.  Tables, one pass: all
+    // loads of an index before its stores; the first index of each
+    // work-item is loaded before the tile and stored after it is issued.
+    const int n_tws = J.ntwsA;                     // stage twiddles
+    const int n_twL = (PA && A1 > 1) ? A : 0;      // omega_A^t (wave factor's outer twiddles)
+    const int n_wav = PA ? 32 * PA + 63 : 0;       // wave-factor table
+    const int n_col = 2 * ncl;                     // per local digit
+    const int nmax = max(max(n_tws, max(n_twL, n_wav)), max(A, n_col));
+    double2 v_tws, v_twL = make_double2(0.0, 0.0), v_wav = make_double2(0.0, 0.0);
+    double v_wh, v_iwh, v_wl, v_iwl;
+    int v_rh, v_rl;
+    auto table_issue = [&](int t) {
+        if (t < n_tws) v_tws = gload2(J.twsA, t);
+        if (t < A) {
+            if constexpr (PA > 0) v_twL = gload2(twA, t);
+            v_wh = gload(J.whi, t);
+            v_iwh = gload(J.iwhi, t);
+            v_rh = gload(J.rhi, t);
+        }
+        if constexpr (PA > 0)
+            if (t < n_wav) v_wav = gload2(twA, (size_t)wave_table_exponent<PA>(t) * A1);
+        if (t < n_col) {
+            int gc = c0 - H + (t >> 1);
+            if (gc < 0) gc += B;
+            const int jj = 2 * gc + (t & 1);
+            v_wl = gload(J.wlo, jj);
+            v_iwl = gload(J.iwlo, jj);
+            v_rl = gload(J.rlo, jj);
+        }
+    };
+    auto table_store = [&](int t) {
+        if (t < n_tws) tws[t] = v_tws;
+        if (t < n_twL) twL[t] = v_twL;
+        if (t < n_wav) twW[t] = v_wav;
+        if (t < A) {
+            roww[t] = make_double2(v_wh, v_iwh);
+            rowr[t] = v_rh;
+        }
+        if (t < n_col) {
+            colw[t] = make_double2(v_wl, v_iwl);
+            colr[t] = v_rl;
+        }
+    };
+    table_issue(tid);
+    if (ntile) tile_issue(0);
+    table_store(tid);
+    for (int t = tid + T; t < nmax; t += T) {
+        table_issue(t);
+        table_store(t);
+    }
+    if (ntile) {
+        tile_store(0);
+        for (int t0 = TB * T; t0 < ntile; t0 += TB * T) {
+            tile_issue(t0);
+            tile_store(t0);
         }
     }
-    for (int t = tid; t < J.ntwsA; t += T) tws[t] = gload2(J.twsA, t);
+    // the stage table last: its copy waits for every load issued before it
     copy_stages<T>(lst, J.stA, J.nradA, tid);
-    constexpr int wA = 64 * PA;
-    const int A1 = PA ? A / wA : A;
-    if (PA) {
-        // omega_A is read only by the wave factor's outer twiddles (A1 > 1)
-        if (A1 > 1)
-            for (int t = tid; t < A; t += T) twL[t] = gload2(twA, t);
-        fill_wave_table<T, PA>(twW, twA, A1, tid);
-    }
-    for (int t = tid; t < A; t += T) {
-        roww[t] = make_double2(gload(J.whi, t), gload(J.iwhi, t));
-        rowr[t] = gload(J.rhi, t);
-    }
-    for (int t = tid; t < 2 * ncl; t += T) {
-        int gc = c0 - H + (t >> 1);
-        if (gc < 0) gc += B;
-        const int jj = 2 * gc + (t & 1);
-        colw[t] = make_double2(gload(J.wlo, jj), gload(J.iwlo, jj));
-        colr[t] = gload(J.rlo, jj);
-    }
     __syncthreads();
     LUCAS_STAMP_COL(1);  // tile and tables loaded
 
@@ -768,10 +835,23 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc& J, int tile,
     };
 
     if (in_digits) {
-        for (int t = tid; t < A * C; t += T) {
-            const int r = udiv(t, C, mC), lo = t - r * C;
-            const int2 x = gload2i(J.digits, (size_t)(c0 + lo) + (size_t)B * r);
-            lds[pidx(r * ncl + H + lo)] = weighted(r, lo, x);
+        // the tile's digits, TB elements per work-item per batch, loads first
+        for (int t0 = 0; t0 < A * C; t0 += TB * T) {
+            int2 x[TB];
+#pragma unroll
+            for (int k = 0; k < TB; ++k) {
+                const int t = t0 + tid + k * T;
+                if (t >= A * C) continue;
+                const int r = udiv(t, C, mC), lo = t - r * C;
+                x[k] = gload2i(J.digits, (size_t)(c0 + lo) + (size_t)B * r);
+            }
+#pragma unroll
+            for (int k = 0; k < TB; ++k) {
+                const int t = t0 + tid + k * T;
+                if (t >= A * C) continue;
+                const int r = udiv(t, C, mC), lo = t - r * C;
+                lds[pidx(r * ncl + H + lo)] = weighted(r, lo, x[k]);
+            }
         }
     } else {
         // inverse column FFT of the tile and halo
@@ -917,22 +997,72 @@ __device__ __forceinline__ void row_kernel_body(const LucasJobDesc& J, int r0) {
     double2* const tws = lds + lucas_pad_words(2 * B);  // stage twiddles of the length-B transform
     double2* const twu = tws + J.ntwsB;          // omega_B^t, t < B (untangle)
     LucasStage* const lst = reinterpret_cast<LucasStage*>(twu + B);  // row transform stages
-    double2* const twW = twu + B + 2 * J.nradB;  // wave-factor twiddles (fill_wave_table)
+    double2* const twW = twu + B + 2 * J.nradB;  // wave-factor twiddles (wave_table_exponent)
 
-    // load both rows with the column kernel's outer twiddle omega_M^(c*k);
-    // the tables go to LDS while those loads are in flight
-    for (int t = tid; t < nrows * B; t += T) {
-        const int rl = t >= B ? 1 : 0, kc = t - rl * B;
-        const int row = rl ? r1 : r0;
-        lds[pidx(t)] = cmul(gload2(J.spec, (size_t)row * B + kc), outer_twiddle(J, J.twA, J.twMB, kc * row));
-    }
-    for (int t = tid; t < J.ntwsB; t += T) tws[t] = gload2(J.twsB, t);
-    for (int t = tid; t < B; t += T) twu[t] = gload2(J.twB, t);
-    copy_stages<T>(lst, J.stB, J.nradB, tid);
-    if (PB) fill_wave_table<T, PB>(twW, J.twB, J.B / (64 * PB), tid);
-    const double2 wr0 = gload2(J.twMA, r0), wr1 = gload2(J.twMA, r1);
+    // Both rows with the column kernel's outer twiddle omega_M^(c*k) (one
+    // entry of each of the row's two twiddle rows, rt1 and rt2), and the
+    // tables.  Every global load is issued before its result is used (see
+    // the column kernel): the rows go in batches of 4 elements per work-item
+    // (values and twiddle factors in registers), the first batch in flight
+    // while the tables are copied.
+    constexpr int TB = 4;
     constexpr int wB = 64 * PB;
     const int B1 = PB ? B / wB : B;
+    const int ntile = nrows * B;
+    double2 tv[TB], ta[TB], tm[TB];
+    auto tile_issue = [&](int t0) {
+#pragma unroll
+        for (int k = 0; k < TB; ++k) {
+            const int t = t0 + tid + k * T;
+            if (t >= ntile) continue;
+            const int rl = t >= B ? 1 : 0, kc = t - rl * B;
+            const int row = rl ? r1 : r0;
+            tv[k] = gload2(J.spec, (size_t)row * B + kc);
+            ta[k] = gload2(J.rt1, (size_t)row * 64 + (kc & 63));
+            tm[k] = gload2(J.rt2, (size_t)row * J.ohB + (kc >> 6));
+        }
+    };
+    auto tile_store = [&](int t0) {
+#pragma unroll
+        for (int k = 0; k < TB; ++k) {
+            const int t = t0 + tid + k * T;
+            if (t < ntile) lds[pidx(t)] = cmul(tv[k], cmul(ta[k], tm[k]));
+        }
+    };
+    // The load counter drains in issue order, so the tables' loads go first:
+    // waiting for them never waits for the tile's.  Tables, one pass: all
+    // loads of an index before its stores; the first index of each
+    // work-item is loaded before the tile and stored after it is issued.
+    const int n_tws = J.ntwsB;
+    const int n_wav = PB ? 32 * PB + 63 : 0;
+    const int nmax = max(max(n_tws, B), n_wav);
+    double2 v_tws, v_twu, v_wav = make_double2(0.0, 0.0);
+    auto table_issue = [&](int t) {
+        if (t < n_tws) v_tws = gload2(J.twsB, t);
+        if (t < B) v_twu = gload2(J.twB, t);
+        if constexpr (PB > 0)
+            if (t < n_wav) v_wav = gload2(J.twB, (size_t)wave_table_exponent<PB>(t) * B1);
+    };
+    auto table_store = [&](int t) {
+        if (t < n_tws) tws[t] = v_tws;
+        if (t < B) twu[t] = v_twu;
+        if (t < n_wav) twW[t] = v_wav;
+    };
+    table_issue(tid);
+    const double2 wr0 = gload2(J.twMA, r0), wr1 = gload2(J.twMA, r1);
+    tile_issue(0);
+    table_store(tid);
+    for (int t = tid + T; t < nmax; t += T) {
+        table_issue(t);
+        table_store(t);
+    }
+    tile_store(0);
+    for (int t0 = TB * T; t0 < ntile; t0 += TB * T) {
+        tile_issue(t0);
+        tile_store(t0);
+    }
+    // the stage table last: its copy waits for every load issued before it
+    copy_stages<T>(lst, J.stB, J.nradB, tid);
     const SeqGeom g{nrows, pow2_recip(nrows), 0, 0u, B, 0, 0, 1};
     const SeqGeom gA{nrows * wB, pow2_recip(nrows * wB), wB, pow2_recip(wB), 1, B, 0, wB};
     // forward row FFT; square; inverse row FFT (conjugated input).  Two call
@@ -1037,11 +1167,11 @@ LUCAS_KERNEL_PAIR(1024, 4)
             lucas_col_kernel_1024d_p##P(const LucasJobDesc* __restrict__ jobs,                              \
                                         const int2* __restrict__ wgmap, int mode) {                         \
         const int2 jm = wgmap[blockIdx.x];                                                                 \
-        col_kernel_body<1024, P>(jobs[jm.x], jm.y, mode);                                                  \
+        col_kernel_body<1024, P, true>(jobs[jm.x], jm.y, mode);                                            \
     }                                                                                                      \
     extern "C" __global__ void __launch_bounds__(1024, 8)                                                  \
             lucas_col1_kernel_1024d_p##P(const LucasJobDesc J, int mode) {                                   \
-        col_kernel_body<1024, P>(J, single_col_tile(blockIdx.x, J.col_wgs), mode);                          \
+        col_kernel_body<1024, P, true>(J, single_col_tile(blockIdx.x, J.col_wgs), mode);                    \
     }
 LUCAS_DENSE_COL(0)
 LUCAS_DENSE_COL(1)

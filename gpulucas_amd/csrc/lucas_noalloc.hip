@@ -34,9 +34,8 @@ hipError_t configure_kernels(size_t max_lds);
 
 using namespace lucas;
 
-// Sized for N = LUCAS_NOALLOC_MAX_N: two spectra of 8N bytes each, the
-// tables (at most 4096-entry twiddle and weight tables per dimension) and
-// the accumulators.
+// Sized for N = LUCAS_NOALLOC_MAX_N: two spectra of 8N bytes each, then
+// the tables and the accumulators of the length's plan.
 static __device__ __align__(256) unsigned char scratch[LUCAS_NOALLOC_SCRATCH_BYTES];
 
 // Zeroes the roundoff and product-size accumulators of a call.
@@ -80,43 +79,13 @@ int load_tables(DeviceScratch* ds, int p, int N, hipStream_t stream) {
     if (!make_plan(p, N, 0, 0, &pl).empty()) return LUCAS_ERR_INVALID_ARG;
     Tables t;
     make_tables(pl, &t);
-    size_t off = 0;
-    auto take = [&](size_t n) {
-        size_t o = off;
-        off += align256(n);
-        return o;
-    };
-    const size_t o_spec = take(sizeof(double2) * pl.M);
-    const size_t o_spec2 = take(sizeof(double2) * pl.M);
-    const size_t o_acc = take(4 * sizeof(unsigned int));
-    const size_t o_tab = off;
-    const size_t o_twA = take(sizeof(double) * t.twA.size());
-    const size_t o_twMA = take(sizeof(double) * t.twMA.size());
-    const size_t o_twB = take(sizeof(double) * t.twB.size());
-    const size_t o_twMB = take(sizeof(double) * t.twMB.size());
-    const size_t o_rlo = take(sizeof(int) * t.rlo.size());
-    const size_t o_rhi = take(sizeof(int) * t.rhi.size());
-    const size_t o_wlo = take(sizeof(double) * t.wlo.size());
-    const size_t o_iwlo = take(sizeof(double) * t.iwlo.size());
-    const size_t o_whi = take(sizeof(double) * t.whi.size());
-    const size_t o_iwhi = take(sizeof(double) * t.iwhi.size());
-    const size_t o_twsA = take(sizeof(double) * t.twsA.size());
-    const size_t o_twsB = take(sizeof(double) * t.twsB.size());
-    if (off > (size_t)LUCAS_NOALLOC_SCRATCH_BYTES) return LUCAS_ERR_INVALID_ARG;
-    std::vector<char> host(off - o_tab, 0);
-    auto put = [&](size_t o, const void* src, size_t n) { memcpy(host.data() + (o - o_tab), src, n); };
-    put(o_twA, t.twA.data(), sizeof(double) * t.twA.size());
-    put(o_twMA, t.twMA.data(), sizeof(double) * t.twMA.size());
-    put(o_twB, t.twB.data(), sizeof(double) * t.twB.size());
-    put(o_twMB, t.twMB.data(), sizeof(double) * t.twMB.size());
-    put(o_rlo, t.rlo.data(), sizeof(int) * t.rlo.size());
-    put(o_rhi, t.rhi.data(), sizeof(int) * t.rhi.size());
-    put(o_wlo, t.wlo.data(), sizeof(double) * t.wlo.size());
-    put(o_iwlo, t.iwlo.data(), sizeof(double) * t.iwlo.size());
-    put(o_whi, t.whi.data(), sizeof(double) * t.whi.size());
-    put(o_iwhi, t.iwhi.data(), sizeof(double) * t.iwhi.size());
-    put(o_twsA, t.twsA.data(), sizeof(double) * t.twsA.size());
-    put(o_twsB, t.twsB.data(), sizeof(double) * t.twsB.size());
+    const size_t o_spec = 0;
+    const size_t o_spec2 = align256(sizeof(double2) * pl.M);
+    const size_t o_tab = 2 * o_spec2;
+    const TableLayout lay = table_layout(pl);
+    if (o_tab + lay.bytes > (size_t)LUCAS_NOALLOC_SCRATCH_BYTES) return LUCAS_ERR_INVALID_ARG;
+    std::vector<char> host(lay.bytes);
+    table_image(pl, t, lay, host.data());
     char* m = ds->ptr;
     hipError_t e = hipMemcpyAsync(m + o_tab, host.data(), host.size(), hipMemcpyHostToDevice, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
@@ -129,20 +98,7 @@ int load_tables(DeviceScratch* ds, int p, int N, hipStream_t stream) {
     d.spec = (double2*)(m + o_spec);
     d.spec2 = (double2*)(m + o_spec2);
     d.digits = nullptr;  // the caller's, per call
-    d.twA = (const double2*)(m + o_twA);
-    d.twMA = (const double2*)(m + o_twMA);
-    d.twB = (const double2*)(m + o_twB);
-    d.twMB = (const double2*)(m + o_twMB);
-    d.rlo = (const int*)(m + o_rlo);
-    d.rhi = (const int*)(m + o_rhi);
-    d.wlo = (const double*)(m + o_wlo);
-    d.iwlo = (const double*)(m + o_iwlo);
-    d.whi = (const double*)(m + o_whi);
-    d.iwhi = (const double*)(m + o_iwhi);
-    d.twsA = (const double2*)(m + o_twsA);
-    d.twsB = (const double2*)(m + o_twsB);
-    d.maxerr = (unsigned int*)(m + o_acc);
-    d.pbits = d.maxerr + 1;
+    desc_tables(lay, m + o_tab, &d);
     ds->desc = d;
     ds->plan = pl;
     ds->p = p;

@@ -207,7 +207,7 @@ int lucas_ctx_load(const char* path, int device, lucas_ctx** out);
  * calls that overlap on one GPU race.  max_roundoff (optional; non-NULL
  * synchronises the stream) receives the call's max roundoff. */
 #define LUCAS_NOALLOC_MAX_N 524288
-#define LUCAS_NOALLOC_SCRATCH_BYTES (9 * 1024 * 1024)
+#define LUCAS_NOALLOC_SCRATCH_BYTES (10 * 1024 * 1024)
 int lucas_step_digits_noalloc(int p, int N, int32_t* digits, long long iters, int device, void* stream,
                               double* max_roundoff);
 

@@ -83,6 +83,8 @@ extern "C" int emu_run(int p, int N, int A, int C, int T, const int32_t* digits_
     fill_desc_shape(pl, &d);
     d.twA = (const double2*)t.twA.data(); d.twB = (const double2*)t.twB.data();
     d.twMA = (const double2*)t.twMA.data(); d.twMB = (const double2*)t.twMB.data();
+    d.ct1 = (const double2*)t.ct1.data(); d.ct2 = (const double2*)t.ct2.data();
+    d.rt1 = (const double2*)t.rt1.data(); d.rt2 = (const double2*)t.rt2.data();
     d.twsA = (const double2*)t.twsA.data(); d.twsB = (const double2*)t.twsB.data();
     d.rlo = t.rlo.data(); d.rhi = t.rhi.data();
     d.wlo = t.wlo.data(); d.whi = t.whi.data(); d.iwlo = t.iwlo.data(); d.iwhi = t.iwhi.data();
