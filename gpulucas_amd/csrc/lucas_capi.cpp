@@ -1067,15 +1067,16 @@ int lucas_server_start(const lucas_server_options* opt, lucas_server** out) {
     return LUCAS_OK;
 }
 
-static const long long kServerBusyJobs = 16;
+static const long long kServerBusyJobs = 4;
 
 int lucas_server_submit(lucas_server* s, int p, int N, long long iterations, long long* id) {
     if (!s || iterations <= 0) return LUCAS_ERR_INVALID_ARG;
     Plan pl;
     // Lightly loaded, a served job's latency is what counts: the single-job
     // plan.  With kServerBusyJobs or more jobs already waiting or running the
-    // device is shared anyway, and the batch plan's wider tiles (less halo
-    // recomputation) raise the aggregate rate.
+    // device is shared anyway, and the batch plan's small workgroups and wide
+    // tiles raise the aggregate rate (MI355X: four exponents near 3e6 as one
+    // batch already match their single-job plans run back to back).
     long long outstanding = 0;
     {
         std::lock_guard<std::mutex> lk(s->mu);

@@ -241,10 +241,18 @@ std::string make_plan(int p, int N, int forceA, int forceC, Plan* out, bool shar
         for (const TunedShape* t = kTuned; t->N; ++t) {
             if (t->N != N) continue;
             if (shared) {
-                // throughput over latency: the widest tile (least halo
-                // recomputation) at the tuned split, above 2^16 words with
-                // 1024-thread workgroups first (fewer launch groups in a
-                // batch: +4% on a 1M..5M batch, -17% below 2^16)
+                // Throughput over latency: a batch fills the device with
+                // many jobs' workgroups, so the smallest workgroups come
+                // first -- four 256-item workgroups per CU hide one
+                // another's load and barrier latency, where a 1024-item one
+                // runs alone on its CU -- each with the widest tile (least
+                // halo recomputation) at the tuned split.  MI355X, 64
+                // exponents: 1.65x the rate of 1024-item plans at p in
+                // [1e6, 5e6], 2.1x at p in [1e5, 1.2e6]; 8 exponents near
+                // 9e6: 1.09x; 4 near 3.4e7 (no 256-item tiling): the same.
+                for (int T : {LUCAS_THREADS_SMALL, LUCAS_THREADS_LARGE})
+                    for (int C : {8, 4, 2, 1})
+                        if (C >= t->C && make_plan_exact(p, N, t->A, C, T, t->pref, out).empty()) return "";
                 if (N > 65536) {
                     for (int C : {8, 4})
                         if (C > t->C && make_plan_exact(p, N, t->A, C, 1024, t->pref, out).empty()) return "";

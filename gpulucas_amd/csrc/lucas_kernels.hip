@@ -321,14 +321,6 @@ __device__ __forceinline__ LucasStage uniform_stage(const LucasStage* lst, int s
     return st;
 }
 
-// Copy a stage table (nst entries of 8 ints) from the job descriptor to LDS.
-template <int T>
-__device__ __forceinline__ void copy_stages(LucasStage* lst, const LucasStage* src, int nst, int tid) {
-    int* const d = reinterpret_cast<int*>(lst);
-    const int* const s = reinterpret_cast<const int*>(src);
-    for (int t = tid; t < 8 * nst; t += T) d[t] = s[t];
-}
-
 // Forward transforms only: an inverse transform is conj(DFT(conj(x))), so the
 // kernels conjugate on the way in and out.
 template <int T, int SIGN, bool SEQ_FAST>
@@ -769,11 +761,15 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc& J, int tile,
     const int n_twL = (PA && A1 > 1) ? A : 0;      // omega_A^t (wave factor's outer twiddles)
     const int n_wav = PA ? 32 * PA + 63 : 0;       // wave-factor table
     const int n_col = 2 * ncl;                     // per local digit
-    const int nmax = max(max(n_tws, max(n_twL, n_wav)), max(A, n_col));
+    const int n_st = 8 * J.nradA;                  // stage table words
+    const int nmax = max(max(max(n_tws, n_st), max(n_twL, n_wav)), max(A, n_col));
+    const int* const st_src = reinterpret_cast<const int*>(J.stA);
+    int* const st_dst = reinterpret_cast<int*>(lst);
     double2 v_tws, v_twL = make_double2(0.0, 0.0), v_wav = make_double2(0.0, 0.0);
     double v_wh, v_iwh, v_wl, v_iwl;
-    int v_rh, v_rl;
+    int v_rh, v_rl, v_st;
     auto table_issue = [&](int t) {
+        if (t < n_st) v_st = st_src[t];
         if (t < n_tws) v_tws = gload2(J.twsA, t);
         if (t < A) {
             if constexpr (PA > 0) v_twL = gload2(twA, t);
@@ -793,6 +789,7 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc& J, int tile,
         }
     };
     auto table_store = [&](int t) {
+        if (t < n_st) st_dst[t] = v_st;
         if (t < n_tws) tws[t] = v_tws;
         if (t < n_twL) twL[t] = v_twL;
         if (t < n_wav) twW[t] = v_wav;
@@ -819,8 +816,6 @@ __device__ __forceinline__ void col_kernel_body(const LucasJobDesc& J, int tile,
             tile_store(t0);
         }
     }
-    // the stage table last: its copy waits for every load issued before it
-    copy_stages<T>(lst, J.stA, J.nradA, tid);
     __syncthreads();
     LUCAS_STAMP_COL(1);  // tile and tables loaded
 
@@ -1035,15 +1030,21 @@ __device__ __forceinline__ void row_kernel_body(const LucasJobDesc& J, int r0) {
     // work-item is loaded before the tile and stored after it is issued.
     const int n_tws = J.ntwsB;
     const int n_wav = PB ? 32 * PB + 63 : 0;
-    const int nmax = max(max(n_tws, B), n_wav);
+    const int n_st = 8 * J.nradB;  // stage table words
+    const int nmax = max(max(n_tws, n_st), max(B, n_wav));
+    const int* const st_src = reinterpret_cast<const int*>(J.stB);
+    int* const st_dst = reinterpret_cast<int*>(lst);
     double2 v_tws, v_twu, v_wav = make_double2(0.0, 0.0);
+    int v_st;
     auto table_issue = [&](int t) {
+        if (t < n_st) v_st = st_src[t];
         if (t < n_tws) v_tws = gload2(J.twsB, t);
         if (t < B) v_twu = gload2(J.twB, t);
         if constexpr (PB > 0)
             if (t < n_wav) v_wav = gload2(J.twB, (size_t)wave_table_exponent<PB>(t) * B1);
     };
     auto table_store = [&](int t) {
+        if (t < n_st) st_dst[t] = v_st;
         if (t < n_tws) tws[t] = v_tws;
         if (t < B) twu[t] = v_twu;
         if (t < n_wav) twW[t] = v_wav;
@@ -1061,8 +1062,6 @@ __device__ __forceinline__ void row_kernel_body(const LucasJobDesc& J, int r0) {
         tile_issue(t0);
         tile_store(t0);
     }
-    // the stage table last: its copy waits for every load issued before it
-    copy_stages<T>(lst, J.stB, J.nradB, tid);
     const SeqGeom g{nrows, pow2_recip(nrows), 0, 0u, B, 0, 0, 1};
     const SeqGeom gA{nrows * wB, pow2_recip(nrows * wB), wB, pow2_recip(wB), 1, B, 0, wB};
     // forward row FFT; square; inverse row FFT (conjugated input).  Two call
