@@ -164,14 +164,15 @@ def test_config1_p86243_n4608(gmp_fixtures):
 
 
 def test_step_sizes_across_replay_graph_boundaries(gmp_fixtures):
-    """One step of 1000 iterations (graphs of 128 and 32, then single
-    launches) and the same 1000 split at every graph-size boundary reach the
-    same digits and GMP's residue."""
+    """One step of 1000 iterations (graphs of 128 pairs, then the binary
+    digits of the rest) and the same 1000 split at every graph-size boundary
+    (whole-step graphs up to 64 iterations, 32-pair graphs below 512) reach
+    the same digits and GMP's residue."""
     p, N = 86243, 4608
     want, zero = fixture_res64(gmp_fixtures, "c1_p86243", p, 1000)
     with lucas.Context(p, N) as a, lucas.Context(p, N) as b:
         a.step(1000)
-        for k in (1, 31, 32, 33, 127, 128, 129, 519):
+        for k in (1, 31, 32, 33, 63, 64, 65, 127, 128, 129, 327):
             b.step(k)
         assert a.iteration == b.iteration == 1000
         assert np.array_equal(a.digits(), b.digits())
