@@ -20,8 +20,11 @@ LIB = os.path.join(PKG, "libgpulucas.so")
 PHASES_LIB = os.path.join(PKG, "libgpulucas_phases.so")
 CLI = os.path.join(PKG, "gpulucas")
 
-SOURCES = ["lucas_kernels.hip", "lucas_host.cpp", "lucas_engine.cpp", "lucas_capi.cpp", "lucas_noalloc.hip"]
-HEADERS = ["lucas_desc.h", "lucas_host.h", "lucas_engine.h"]
+SOURCES = ["lucas_kernels.hip", "lucas_host.cpp", "lucas_engine.cpp", "lucas_capi.cpp", "lucas_noalloc.hip",
+           "lucas_line.hip"]
+# kernel files built a second time, with phase stamps, for the profiling library
+PHASED = ["lucas_kernels.hip", "lucas_line.hip"]
+HEADERS = ["lucas_desc.h", "lucas_host.h", "lucas_engine.h", "lucas_device.h", "lucas_line.h"]
 
 ARCH = os.environ.get("GPULUCAS_ARCH", "gfx950")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
@@ -58,11 +61,15 @@ def build_library(verbose: bool = False, jobs: int = 4) -> str:
     objdir = os.path.join(PKG, "_build")
     os.makedirs(objdir, exist_ok=True)
     objs, procs = [], []
-    phases_obj = os.path.join(objdir, "lucas_kernels_phases.o")
-    cmd = [HIPCC] + HIP_FLAGS + ["-DLUCAS_PHASE_STAMPS", "-c", os.path.join(CSRC, "lucas_kernels.hip"), "-o", phases_obj]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+    # the kernel files once more with the phase stamps compiled in
+    phases_objs = []
+    for src in PHASED:
+        obj = os.path.join(objdir, src.rsplit(".", 1)[0] + "_phases.o")
+        phases_objs.append(obj)
+        cmd = [HIPCC] + HIP_FLAGS + ["-DLUCAS_PHASE_STAMPS", "-c", os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
     for src in SOURCES:
         obj = os.path.join(objdir, src.rsplit(".", 1)[0] + ".o")
         objs.append(obj)
@@ -89,7 +96,8 @@ def build_library(verbose: bool = False, jobs: int = 4) -> str:
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"], verbose)
     os.replace(tmp, LIB)
     tmp = PHASES_LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, phases_obj] + objs[1:] + ["-lpthread"],
+    plain = [o for o, src in zip(objs, SOURCES) if src not in PHASED]
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + phases_objs + plain + ["-lpthread"],
          verbose)
     os.replace(tmp, PHASES_LIB)
     build_cli(verbose)

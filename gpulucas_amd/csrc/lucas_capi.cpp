@@ -58,6 +58,7 @@ void fill_info(const Plan& pl, lucas_plan_info* out) {
     out->wave_A = pl.wA;
     out->wave_B = pl.wB;
     out->col_dense = pl.col_dense;
+    out->line = pl.line;
     out->device_bytes = job_bytes(pl);
 }
 
@@ -330,6 +331,14 @@ int lucas_plan_list(int p, int N, lucas_plan_info* out, int cap, int* count) {
     return LUCAS_OK;
 }
 
+int lucas_line_plan_list(int p, int N, lucas_plan_info* out, int cap, int* count) {
+    if (!count || cap < 0 || (cap > 0 && !out)) return LUCAS_ERR_INVALID_ARG;
+    std::vector<Plan> c = line_candidates(p, N);
+    *count = (int)c.size();
+    for (int i = 0; i < cap && i < (int)c.size(); ++i) fill_info(c[i], &out[i]);
+    return LUCAS_OK;
+}
+
 int lucas_digit_tables(int p, int N, int* widths, double* weights, double* inv_weights) {
     if (p < 7 || !valid_length(N) || N > p) return LUCAS_ERR_INVALID_ARG;
     if (widths) digit_widths(p, N, widths);
@@ -363,6 +372,10 @@ int lucas_ctx_create_shape(int p, int N, int A, int C, int device, lucas_ctx** o
 int lucas_ctx_create_plan(const lucas_plan_info* plan, int device, lucas_ctx** out) {
     if (!plan || !out) return LUCAS_ERR_INVALID_ARG;
     Plan pl;
+    if (plan->line) {
+        if (!make_line_plan(plan->p, plan->N, plan->A, plan->C, &pl).empty()) return LUCAS_ERR_INVALID_ARG;
+        return ctx_new(pl, device, out);
+    }
     const bool ordinary = make_plan_exact(plan->p, plan->N, plan->A, plan->C, plan->threads, plan->radix_pref, &pl,
                                           plan->wave_A, plan->wave_B).empty();
     if (!ordinary && !make_plan_exact(plan->p, plan->N, plan->A, plan->C, plan->threads, plan->radix_pref, &pl,
@@ -684,8 +697,12 @@ int lucas_autotune_shape(int p, int N, int iters, int max_candidates, const luca
                          lucas_plan_info* out, double* best_ms) {
     if (!out || p < 7) return LUCAS_ERR_INVALID_ARG;
     if (N == 0) N = default_length(p);
-    std::vector<Plan> cands = plan_candidates(p, N);
-    if (cands.empty()) cands = plan_candidates(p, N, true);
+    // the line-kernel shapes first (few, and usually the fastest), then the
+    // tilings of the general kernels in modelled order
+    std::vector<Plan> cands = line_candidates(p, N);
+    std::vector<Plan> tiled = plan_candidates(p, N);
+    if (tiled.empty()) tiled = plan_candidates(p, N, true);
+    cands.insert(cands.end(), tiled.begin(), tiled.end());
     if (cands.empty()) return LUCAS_ERR_INVALID_ARG;
     if (max_candidates > 0 && (int)cands.size() > max_candidates) cands.resize(max_candidates);
     if (iters < 10) iters = 10;

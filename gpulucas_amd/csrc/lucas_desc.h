@@ -73,6 +73,10 @@ struct LucasJobDesc {
     const double2* ct2;     // [B][ceil(A/64)]: omega_M^(64*c*rh)
     const double2* rt1;     // [A][64]: omega_M^(r*cl)
     const double2* rt2;     // [A][ceil(B/64)]: omega_M^(64*r*ch)
+    // Line-kernel twiddles (line plans only): omega_L^(n*k1) at [k1*64 + n]
+    // (L entries, L = A or B), then omega_64^(a*c) at [L + c*8 + a].
+    const double2* lnA;
+    const double2* lnB;
     const double2* twsA;    // stage twiddles of the length-A transform (see LucasStage)
     const double2* twsB;    // stage twiddles of the length-B transform
     // Weight tables: j = Jlo + 2B*r;  r_j = (rlo[Jlo] + rhi[r]) mod N.

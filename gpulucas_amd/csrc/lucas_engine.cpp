@@ -28,8 +28,14 @@ hipError_t launch_col1_kernel(int threads, int p, int dense, const LucasJobDesc&
                               hipStream_t stream);
 hipError_t launch_row1_kernel(int threads, int p, const LucasJobDesc& job, int nwg, size_t lds, hipStream_t stream);
 hipError_t configure_kernels(size_t max_lds);
+hipError_t configure_line_kernels(size_t max_lds);
+hipError_t launch_line_col(int idx, const LucasJobDesc* jobs, const int2* map, const LucasJobDesc* one, int nwg,
+                           int mode, hipStream_t stream);
+hipError_t launch_line_row(int idx, const LucasJobDesc* jobs, const int2* map, const LucasJobDesc* one, int nwg,
+                           hipStream_t stream);
 bool phase_stamps_built();
 hipError_t read_phase_stamps(unsigned long long* col, int ncol, unsigned long long* row, int nrow);
+hipError_t read_line_phase_stamps(unsigned long long* col, int ncol, unsigned long long* row, int nrow);
 
 // Replay graphs.  A step of k iterations is the head graph (column kernel,
 // digits in), k - 1 steady-state [row ; col] pairs and the tail graph (row,
@@ -212,9 +218,9 @@ Engine::~Engine() {
     hipSetDevice(device_);
     destroy_graphs();
     if (d_desc_) hipFree(d_desc_);
-    for (Group* gs : {colg_, rowg_})
-        for (int i = 0; i < kGroups; ++i)
-            if (gs[i].d_map) hipFree(gs[i].d_map);
+    for (auto* gs : {&colg_, &rowg_})
+        for (auto& kv : *gs)
+            if (kv.second.d_map) hipFree(kv.second.d_map);
     if (ev0_) hipEventDestroy(ev0_);
     if (ev1_) hipEventDestroy(ev1_);
     if (stream_) hipStreamDestroy(stream_);
@@ -228,7 +234,8 @@ int Engine::init() {
     }
     if (device_ < 0 || device_ >= count) return LUCAS_ERR_INVALID_ARG;
     if (hipSetDevice(device_) != hipSuccess) return LUCAS_ERR_NO_DEVICE;
-    if (configure_kernels(kMaxLdsBytes) != hipSuccess) return LUCAS_ERR_DEVICE;
+    if (configure_kernels(kMaxLdsBytes) != hipSuccess || configure_line_kernels(kMaxLdsBytes) != hipSuccess)
+        return LUCAS_ERR_DEVICE;
     if (hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking) != hipSuccess) return LUCAS_ERR_DEVICE;
     if (hipEventCreate(&ev0_) != hipSuccess || hipEventCreate(&ev1_) != hipSuccess) return LUCAS_ERR_DEVICE;
     return LUCAS_OK;
@@ -244,16 +251,25 @@ int Engine::prepare() {
     hipSetDevice(device_);
     destroy_graphs();
     std::vector<LucasJobDesc> desc;
-    std::vector<int2> cmap[kGroups], rmap[kGroups];
+    std::map<int, std::vector<int2>> cmap, rmap;
     static const int kThreads[4] = {LUCAS_THREADS_SMALL, LUCAS_THREADS_LARGE, LUCAS_THREADS_MAX, LUCAS_THREADS_MAX};
     static const int kP[4] = {0, 1, 2, 4};
-    for (int i = 0; i < kGroups; ++i) {
-        colg_[i].threads = rowg_[i].threads = kThreads[i / 4];
-        colg_[i].p = rowg_[i].p = kP[i % 4];
-        colg_[i].dense = i / 4 == 3;
-        rowg_[i].dense = 0;
-        colg_[i].lds = rowg_[i].lds = 0;
-    }
+    // groups keep their device maps across composition changes; a group no
+    // job uses any more launches nothing
+    for (auto* gs : {&colg_, &rowg_})
+        for (auto& kv : *gs) kv.second.n = 0;
+    auto setup = [&](Group& g, int key, bool col) {
+        if (key >= kLineBase) {
+            g.line = key - kLineBase;
+            g.threads = g.p = g.dense = 0;
+            g.lds = 0;
+        } else {
+            g.line = -1;
+            g.threads = kThreads[key / 4];
+            g.p = kP[key % 4];
+            g.dense = col && key / 4 == 3;
+        }
+    };
     auto group_of = [](int threads, int p, int dense) {
         const int ti = threads == LUCAS_THREADS_MAX ? (dense ? 3 : 2) : (threads == LUCAS_THREADS_LARGE ? 1 : 0);
         return 4 * ti + (p == 4 ? 3 : p);
@@ -261,18 +277,25 @@ int Engine::prepare() {
     for (size_t i = 0; i < jobs_.size(); ++i) {
         const Job* j = jobs_[i];
         desc.push_back(j->desc);
-        const int gc = group_of(j->plan.threads, j->plan.wA / 64, j->plan.col_dense);
-        const int gr = group_of(j->plan.threads, j->plan.wB / 64, 0);
+        const int gc = j->plan.line ? kLineBase + j->plan.line_col : group_of(j->plan.threads, j->plan.wA / 64, j->plan.col_dense);
+        const int gr = j->plan.line ? kLineBase + j->plan.line_row : group_of(j->plan.threads, j->plan.wB / 64, 0);
         for (int w = 0; w < j->plan.col_wgs; ++w) cmap[gc].push_back(make_int2((int)i, w));
         for (int w = 0; w < j->plan.row_wgs; ++w) rmap[gr].push_back(make_int2((int)i, w));
-        colg_[gc].lds = std::max(colg_[gc].lds, j->plan.lds_col);
-        rowg_[gr].lds = std::max(rowg_[gr].lds, j->plan.lds_row);
+        if (!colg_.count(gc)) colg_[gc].lds = 0;
+        if (!rowg_.count(gr)) rowg_[gr].lds = 0;
+        setup(colg_[gc], gc, true);
+        setup(rowg_[gr], gr, false);
+        if (!j->plan.line) {
+            colg_[gc].lds = std::max(colg_[gc].lds, j->plan.lds_col);
+            rowg_[gr].lds = std::max(rowg_[gr].lds, j->plan.lds_row);
+        }
     }
     // Workgroups are dealt to the 8 XCDs round-robin by workgroup id, and each
     // XCD has its own L2.  Neighbouring column tiles share halo columns and
     // cache lines, so each XCD gets a contiguous range of tiles: workgroup
     // 8k + x runs tile k of XCD x's range.
-    for (std::vector<int2>& m : cmap) {
+    for (auto& kv : cmap) {
+        std::vector<int2>& m = kv.second;
         const size_t n = m.size();
         if (n < 16) continue;
         std::vector<int2> out;
@@ -298,13 +321,14 @@ int Engine::prepare() {
     hipError_t e = hipSuccess;
     if (!desc.empty())
         e = hipMemcpyAsync(d_desc_, desc.data(), sizeof(LucasJobDesc) * desc.size(), hipMemcpyHostToDevice, stream_);
-    for (int gi = 0; gi < kGroups && e == hipSuccess; ++gi) {
-        for (int k = 0; k < 2 && e == hipSuccess; ++k) {
-            Group& g = k == 0 ? colg_[gi] : rowg_[gi];
-            const std::vector<int2>& m = k == 0 ? cmap[gi] : rmap[gi];
+    for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+        for (auto& kv : k == 0 ? cmap : rmap) {
+            Group& g = k == 0 ? colg_[kv.first] : rowg_[kv.first];
+            const std::vector<int2>& m = kv.second;
             if (!grow((void**)&g.d_map, &g.cap, sizeof(int2) * (m.size() + 1))) return LUCAS_ERR_OOM;
             g.n = (int)m.size();
             if (g.n) e = hipMemcpyAsync(g.d_map, m.data(), sizeof(int2) * g.n, hipMemcpyHostToDevice, stream_);
+            if (e != hipSuccess) break;
         }
     }
     if (e == hipSuccess) e = hipStreamSynchronize(stream_);
@@ -317,10 +341,14 @@ int Engine::prepare() {
 }
 
 int Engine::launch_col(int mode) {
-    for (const Group& g : colg_) {
+    for (const auto& kv : colg_) {
+        const Group& g = kv.second;
         if (!g.n) continue;
-        const hipError_t e =
-                single_ ? launch_col1_kernel(g.threads, g.p, g.dense, one_, g.n, g.lds, mode, stream_)
+        hipError_t e;
+        if (g.line >= 0)
+            e = launch_line_col(g.line, d_desc_, g.d_map, single_ ? &one_ : nullptr, g.n, mode, stream_);
+        else
+            e = single_ ? launch_col1_kernel(g.threads, g.p, g.dense, one_, g.n, g.lds, mode, stream_)
                         : launch_col_kernel(g.threads, g.p, g.dense, d_desc_, g.d_map, g.n, g.lds, mode, stream_);
         if (e != hipSuccess) return LUCAS_ERR_DEVICE;
     }
@@ -328,10 +356,15 @@ int Engine::launch_col(int mode) {
 }
 
 int Engine::launch_row() {
-    for (const Group& g : rowg_) {
+    for (const auto& kv : rowg_) {
+        const Group& g = kv.second;
         if (!g.n) continue;
-        const hipError_t e = single_ ? launch_row1_kernel(g.threads, g.p, one_, g.n, g.lds, stream_)
-                                     : launch_row_kernel(g.threads, g.p, d_desc_, g.d_map, g.n, g.lds, stream_);
+        hipError_t e;
+        if (g.line >= 0)
+            e = launch_line_row(g.line, d_desc_, g.d_map, single_ ? &one_ : nullptr, g.n, stream_);
+        else
+            e = single_ ? launch_row1_kernel(g.threads, g.p, one_, g.n, g.lds, stream_)
+                        : launch_row_kernel(g.threads, g.p, d_desc_, g.d_map, g.n, g.lds, stream_);
         if (e != hipSuccess) return LUCAS_ERR_DEVICE;
     }
     return LUCAS_OK;
@@ -543,8 +576,8 @@ int Engine::phase_stamps(int iters, unsigned long long* col, unsigned long long*
     int rc = prepare();
     if (rc != LUCAS_OK) return rc;
     int nc = 0, nr = 0;
-    for (const Group& g : colg_) nc += g.n;
-    for (const Group& g : rowg_) nr += g.n;
+    for (const auto& kv : colg_) nc += kv.second.n;
+    for (const auto& kv : rowg_) nr += kv.second.n;
     if (nc > max_wgs || nr > max_wgs) return LUCAS_ERR_INVALID_ARG;
     rc = launch_col(LUCAS_IN_DIGITS);  // valid spectra; digits untouched
     for (int i = 0; i < iters && rc == LUCAS_OK; ++i) {
@@ -552,7 +585,9 @@ int Engine::phase_stamps(int iters, unsigned long long* col, unsigned long long*
         if (rc == LUCAS_OK) rc = launch_col(0);
     }
     if (rc == LUCAS_OK && hipStreamSynchronize(stream_) != hipSuccess) rc = LUCAS_ERR_DEVICE;
-    if (rc == LUCAS_OK && read_phase_stamps(col, nc, row, nr) != hipSuccess) rc = LUCAS_ERR_DEVICE;
+    if (rc == LUCAS_OK && (jobs_[0]->plan.line ? read_line_phase_stamps(col, nc, row, nr)
+                                               : read_phase_stamps(col, nc, row, nr)) != hipSuccess)
+        rc = LUCAS_ERR_DEVICE;
     if (rc != LUCAS_OK) return rc;
     *ncol = nc;
     *nrow = nr;

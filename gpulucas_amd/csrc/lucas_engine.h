@@ -94,11 +94,14 @@ private:
     int build_graph(int slot);
 
     // Jobs whose kernel has the same workgroup size and wave factor share
-    // one launch; column and row launches are grouped separately.
-    // 3 workgroup sizes (and the dense 1024 column variant) x 4 wave factors (P = 0, 1, 2, 4)
-    static const int kGroups = 16;
+    // one launch; column and row launches are grouped separately.  Groups
+    // 0..15 are the tiled kernels (3 workgroup sizes and the dense 1024
+    // column variant x 4 wave factors P = 0, 1, 2, 4); group kLineBase + i
+    // is line instantiation i.
+    static const int kLineBase = 16;
     struct Group {
         int threads = 0, p = 0, dense = 0;
+        int line = -1;  // line instantiation, -1: tiled kernels
         int2* d_map = nullptr;
         size_t cap = 0;
         int n = 0;
@@ -114,7 +117,7 @@ private:
     size_t cap_desc_ = 0;
     bool single_ = false;  // one job: launch the by-value-descriptor kernels with one_
     LucasJobDesc one_;
-    Group colg_[kGroups], rowg_[kGroups];
+    std::map<int, Group> colg_, rowg_;
     // replay graphs by slot (see kHead, kTail), built on first use
     hipGraph_t graph_[kGraphSlots] = {};
     hipGraphExec_t gexec_[kGraphSlots] = {};

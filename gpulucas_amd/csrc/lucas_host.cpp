@@ -19,6 +19,7 @@
 #include <limits>
 
 #include "lucas_desc.h"
+#include "lucas_line.h"
 
 namespace lucas {
 
@@ -210,13 +211,58 @@ std::vector<Plan> plan_candidates(int p, int N, bool wide) {
     return out;
 }
 
-// Measured fastest tilings per signal length (gpulucas_amd/tune.py).
+std::vector<Plan> line_candidates(int p, int N) {
+    std::vector<Plan> out;
+    Plan pl;
+    if (!base_plan(p, N, &pl).empty()) return out;
+    for (int EA = 1; 64 * EA <= pl.M; ++EA) {
+        const int A = 64 * EA;
+        if (pl.M % A || (pl.M / A) % 64) continue;
+        const int B = pl.M / A, EB = B / 64;
+        const int ir = lucas_line_row_index(EB);
+        if (ir < 0) continue;
+        for (int K = 1; K <= 16; ++K) {
+            const int ic = lucas_line_col_index(EA, K, pl.ncarry);
+            if (ic < 0 || B % K) continue;
+            Plan c = pl;
+            c.A = A;
+            c.B = B;
+            c.C = K;
+            c.threads = 64 * (K + pl.halo);
+            c.col_wgs = B / K;
+            c.row_wgs = A / 2;
+            c.line = 1;
+            c.line_col = ic;
+            c.line_row = ir;
+            c.lds_col = 16 * (size_t)lucas_line_col_words(EA, K, pl.ncarry);
+            c.lds_row = 16 * (size_t)lucas_line_row_words(EB);
+            if (c.lds_col > kMaxLds || c.lds_row > kMaxLds) continue;
+            out.push_back(c);
+        }
+    }
+    return out;
+}
+
+std::string make_line_plan(int p, int N, int A, int K, Plan* out) {
+    Plan pl;
+    std::string why = base_plan(p, N, &pl);
+    if (!why.empty()) return why;
+    for (const Plan& c : line_candidates(p, N)) {
+        if ((A && c.A != A) || (K && c.C != K)) continue;
+        *out = c;
+        return "";
+    }
+    return "no line-kernel shape for this N and carry depth";
+}
+
+// Measured fastest tilings per signal length (gpulucas_amd/tune.py); line: 1
+// for the line kernels (A rows, C own columns per column workgroup).
 struct TunedShape {
-    int N, A, C, threads, pref, wA, wB;
+    int N, A, C, threads, pref, wA, wB, line;
 };
 static const TunedShape kTuned[] = {
 #include "tuned_shapes.inc"
-    {0, 0, 0, 0, 0, 0, 0}};
+    {0, 0, 0, 0, 0, 0, 0, 0}};
 
 std::string make_plan_exact(int p, int N, int A, int C, int threads, int pref, Plan* out, int wA, int wB,
                             bool wide) {
@@ -240,6 +286,11 @@ std::string make_plan(int p, int N, int forceA, int forceC, Plan* out, bool shar
     if (!forceA && !forceC) {
         for (const TunedShape* t = kTuned; t->N; ++t) {
             if (t->N != N) continue;
+            if (t->line) {
+                // a single exponent's latency plan (batches keep the tiled kernels)
+                if (!shared && make_line_plan(p, N, t->A, t->C, out).empty()) return "";
+                continue;
+            }
             if (shared) {
                 // Throughput over latency: a batch fills the device with
                 // many jobs' workgroups, so the smallest workgroups come
@@ -425,6 +476,8 @@ TableLayout table_layout(const Plan& pl) {
     l.iwhi = take(sizeof(double) * pl.A);
     l.twsA = take(c2 * std::max(1, stage_twiddle_count(A1, pl.radA)));
     l.twsB = take(c2 * std::max(1, stage_twiddle_count(B1, pl.radB)));
+    l.lnA = take(pl.line ? c2 * (pl.A + 64) : 0);
+    l.lnB = take(pl.line ? c2 * (pl.B + 64) : 0);
     l.acc = take(256);  // roundoff maximum and product-size diagnostic
     l.bytes = off;
     return l;
@@ -452,6 +505,8 @@ void table_image(const Plan& pl, const Tables& t, const TableLayout& l, char* im
     put(l.iwhi, t.iwhi.data(), sizeof(double) * t.iwhi.size());
     put(l.twsA, t.twsA.data(), sizeof(double) * t.twsA.size());
     put(l.twsB, t.twsB.data(), sizeof(double) * t.twsB.size());
+    put(l.lnA, t.lnA.data(), sizeof(double) * t.lnA.size());
+    put(l.lnB, t.lnB.data(), sizeof(double) * t.lnB.size());
 }
 
 void desc_tables(const TableLayout& l, char* m, LucasJobDesc* d) {
@@ -471,6 +526,8 @@ void desc_tables(const TableLayout& l, char* m, LucasJobDesc* d) {
     d->iwhi = (const double*)(m + l.iwhi);
     d->twsA = (const double2*)(m + l.twsA);
     d->twsB = (const double2*)(m + l.twsB);
+    d->lnA = (const double2*)(m + l.lnA);
+    d->lnB = (const double2*)(m + l.lnB);
     d->maxerr = (unsigned int*)(m + l.acc);
     d->pbits = d->maxerr + 1;
 }
@@ -514,6 +571,22 @@ static void stage_twiddles(int L, const std::vector<int>& rad, std::vector<doubl
 void make_tables(const Plan& pl, Tables* t) {
     stage_twiddles(pl.wA ? pl.A / pl.wA : pl.A, pl.radA, &t->twsA);
     stage_twiddles(pl.wB ? pl.B / pl.wB : pl.B, pl.radB, &t->twsB);
+    if (pl.line) {
+        // line twiddles of a length-L transform: omega_L^(n*k1) at [k1*64 + n],
+        // then omega_64^(a*c) at [L + c*8 + a]
+        auto line_tw = [](int L, std::vector<double>* out) {
+            out->resize(2 * (size_t)(L + 64));
+            auto put = [&](size_t i, long long e, long long den) {
+                const long double ang = 2.0L * kPi * (long double)e / (long double)den;
+                (*out)[2 * i] = (double)cosl(ang);
+                (*out)[2 * i + 1] = (double)(-sinl(ang));
+            };
+            for (int t = 0; t < L; ++t) put(t, (long long)(t & 63) * (t >> 6), L);
+            for (int u = 0; u < 64; ++u) put(L + u, (long long)(u & 7) * (u >> 3), 64);
+        };
+        line_tw(pl.A, &t->lnA);
+        line_tw(pl.B, &t->lnB);
+    }
     root_table(pl.A, pl.A, &t->twA);
     root_table(pl.B, pl.B, &t->twB);
     root_table(pl.A, pl.M, &t->twMA);

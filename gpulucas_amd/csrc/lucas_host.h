@@ -26,6 +26,10 @@ struct Plan {
     int radix_pref = 8;     // preferred power-of-two radix (8 or 4)
     int wA = 0, wB = 0;     // wave-register factor of each transform (64, 128, 256; 0: none)
     int col_dense = 0;      // column kernel at 8 waves per SIMD (two 1024-item workgroups per CU)
+    // Line kernels (lucas_line.h): one wave per row or column, shapes compiled
+    // in; line_col / line_row index their instantiations.  C is then the
+    // column workgroup's own columns and threads its work-items.
+    int line = 0, line_col = -1, line_row = -1;
     std::vector<int> radA, radB;  // Stockham radices of the remaining factors A/wA, B/wB
     size_t lds_col = 0, lds_row = 0;  // bytes of LDS per workgroup
 };
@@ -58,6 +62,11 @@ std::vector<Plan> plan_candidates(int p, int N, bool wide = false);
 // tilings; "" on success.
 std::string make_plan_exact(int p, int N, int A, int C, int threads, int pref, Plan* out, int wA = -1, int wB = -1,
                             bool wide = false);
+// Line-kernel plan for (p, N) with A rows and K own columns per column
+// workgroup (A = 0 or K = 0: the first instantiated choice); "" on success.
+std::string make_line_plan(int p, int N, int A, int K, Plan* out);
+// Every line-kernel plan of (p, N).
+std::vector<Plan> line_candidates(int p, int N);
 // Default N for p: smallest admissible length under the bits-per-word rule.
 int default_length(int p);
 // Bits per word allowed at length N by the default rule.
@@ -85,6 +94,7 @@ struct Tables {
     std::vector<double> ct1, ct2;              // [c][rl]: omega_M^(c*rl); [c][rh]: omega_M^(c*64*rh)
     std::vector<double> rt1, rt2;              // [r][cl]: omega_M^(r*cl); [r][ch]: omega_M^(r*64*ch)
     std::vector<double> twsA, twsB;            // stage twiddles, interleaved
+    std::vector<double> lnA, lnB;              // line-kernel twiddles (line plans), interleaved
     std::vector<int> rlo, rhi;
     std::vector<double> wlo, whi, iwlo, iwhi;
 };
@@ -93,7 +103,7 @@ void make_tables(const Plan& plan, Tables* t);
 // Device layout of a job's tables and accumulators: byte offsets (from the
 // start of the table block, each 256-aligned) and the block's size.
 struct TableLayout {
-    size_t twA, twMA, twB, twMB, ct1, ct2, rt1, rt2, rlo, rhi, wlo, iwlo, whi, iwhi, twsA, twsB, acc;
+    size_t twA, twMA, twB, twMB, ct1, ct2, rt1, rt2, rlo, rhi, wlo, iwlo, whi, iwhi, twsA, twsB, lnA, lnB, acc;
     size_t bytes;
 };
 TableLayout table_layout(const Plan& plan);

@@ -1,0 +1,64 @@
+// lucas_line.h -- the line kernels' instantiations and LDS sizes, shared by
+// the host planner and the kernel file.
+//
+// Line kernels give every wave one whole line of the A x B matrix of complex
+// words: a row (row kernel) or a column (column kernel) of length L = 64*E.
+// Lane l holds elements l + 64*e (e < E) in registers; the transform runs a
+// radix-E step in registers and the remaining 64 points as two radix-8 steps
+// exchanged through the wave's own LDS region (no workgroup barrier).  The
+// shapes are compile-time: the column kernel is instantiated per (EA, K, NC)
+// -- A = 64*EA rows, K own columns per workgroup, carry depth NC, halo
+// H = ceil((NC+1)/2) columns -- and the row kernel per EB (B = 64*EB).
+#pragma once
+
+// X(EA, K, NC)
+#define LUCAS_LINE_COLS(X)                                                                                  \
+    X(1, 2, 2) X(1, 2, 3) X(1, 4, 2) X(1, 4, 3)                                                             \
+    X(2, 2, 2) X(2, 2, 3) X(2, 4, 2) X(2, 4, 3)                                                             \
+    X(3, 2, 2) X(3, 2, 3) X(3, 4, 2) X(3, 4, 3)                                                             \
+    X(4, 1, 2) X(4, 2, 2) X(4, 2, 3) X(4, 4, 2) X(4, 4, 3)                                                  \
+    X(5, 2, 2) X(5, 2, 3) X(5, 4, 2) X(5, 4, 3)                                                             \
+    X(7, 2, 2) X(7, 2, 3) X(7, 4, 2) X(7, 4, 3)                                                             \
+    X(8, 2, 2) X(8, 2, 3) X(8, 4, 2) X(8, 4, 3)                                                             \
+    X(16, 2, 2) X(16, 2, 3) X(16, 4, 2) X(16, 4, 3)
+// X(EB)
+#define LUCAS_LINE_ROWS(X) X(1) X(2) X(3) X(4) X(5) X(7) X(8) X(16)
+
+// Halo columns of carry depth n: the restoration of a column's first digits
+// reads the products of the n+1 digits before them.
+inline constexpr int lucas_line_halo(int nc) { return (nc + 2) / 2; }
+
+// LDS of a column workgroup, in 16-byte words: the NL = K + H column regions
+// (A + 1 words each, the pad keeping neighbouring columns in different bank
+// groups); the column transform's twiddles omega_A^(l*k1) [k1][l] and
+// omega_64^t; per row {2^(rhi/N), 2^(-rhi/N)/(4M)} and rhi; per local digit
+// {2^(rlo/N), 2^(-rlo/N)} and rlo.
+inline constexpr int lucas_line_col_words(int EA, int K, int NC) {
+    const int A = 64 * EA, NL = K + lucas_line_halo(NC);
+    return NL * (A + 1) + A + 64 + A + (A + 3) / 4 + 2 * NL + (2 * NL + 3) / 4;
+}
+// LDS of a row workgroup (one row pair, two waves): two row regions, the row
+// transform's twiddles omega_B^(l*k1) [k1][l] and omega_64^t.
+inline constexpr int lucas_line_row_words(int EB) { return 3 * 64 * EB + 64; }
+
+// Index of an instantiation in the lists above, or -1.
+inline int lucas_line_col_index(int EA, int K, int NC) {
+    int i = 0;
+#define LUCAS_LINE_X(a, k, n)             \
+    if (EA == (a) && K == (k) && NC == (n)) \
+        return i;                         \
+    ++i;
+    LUCAS_LINE_COLS(LUCAS_LINE_X)
+#undef LUCAS_LINE_X
+    return -1;
+}
+inline int lucas_line_row_index(int EB) {
+    int i = 0;
+#define LUCAS_LINE_X(b) \
+    if (EB == (b))      \
+        return i;       \
+    ++i;
+    LUCAS_LINE_ROWS(LUCAS_LINE_X)
+#undef LUCAS_LINE_X
+    return -1;
+}

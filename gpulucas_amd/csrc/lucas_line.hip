@@ -1,0 +1,699 @@
+// lucas_line.hip -- line kernels (gfx950): one Lucas-Lehmer squaring
+// s <- s^2 - 2 mod 2^p - 1 by the balanced-digit IBDWT (the reference's
+// mersenneIter, gpuLucas.cu:845-942) as two launches, each one HBM round trip
+// of the 8N-byte state, with every transform held by single waves:
+//
+//   row kernel    : one row pair (k, A-k) per workgroup, one row per wave:
+//                   outer twiddle, forward row transform, real-signal square
+//                   (ComplexPointwiseSqr, gpuLucas.cu:647-665), inverse row
+//                   transform, inverse outer twiddle.
+//   column kernel : K own columns and H halo columns per workgroup, one column
+//                   per wave: inverse column transform, unweight, -2, round
+//                   (invDWTproductMinus2, gpuLucas.cu:774-805), carry-save
+//                   restoration (llintToIrrBal<n>, IrrBaseBalanced.cu:52-110)
+//                   and ripple + weight (loadIntToDoubleIBDWT,
+//                   gpuLucas.cu:730-766) by one work-item per row, forward
+//                   column transform.
+//
+// A line of length L = 64E lives in its wave's registers as E words per lane
+// (element lane + 64e).  A transform is a radix-E step in registers followed
+// by the 64-point transforms over the lanes as two radix-8 steps, each
+// exchanged through the wave's own LDS region: no workgroup barrier, and
+// every shape constant (lengths, strides, divisions) is known at compile
+// time.  Workgroup barriers remain only where lines meet: loading and
+// storing tiles, the square's row pairs and the carry across columns.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lucas_desc.h"
+#include "lucas_device.h"
+#include "lucas_line.h"
+
+namespace {
+
+// Orders one wave's LDS accesses: the LDS executes a wave's accesses in
+// issue order, so a wave reading what its own lanes wrote needs no wait,
+// only that the compiler keeps the program order.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_wave_barrier();
+    __asm__ volatile("" ::: "memory");
+}
+
+template <int SIGN> struct Dft<1, SIGN> {
+    __device__ __forceinline__ static void run(double2*) {}
+};
+
+// exp(SIGN*2*pi*i*j/16) * a for the twiddles inside the radix-16 DFT.
+template <int J, int SIGN>
+__device__ __forceinline__ double2 mul_w16(double2 a) {
+    constexpr double c[10] = {1.0, 0.9238795325112867, 0.7071067811865476, 0.3826834323650898, 0.0,
+                              -0.3826834323650898, -0.7071067811865476, -0.9238795325112867, -1.0,
+                              -0.9238795325112867};
+    constexpr double s[10] = {0.0, 0.3826834323650898, 0.7071067811865476, 0.9238795325112867, 1.0,
+                              0.9238795325112867, 0.7071067811865476, 0.3826834323650898, 0.0,
+                              -0.3826834323650898};
+    if constexpr (J == 4) return mul_si<SIGN>(a);
+    else return cmul(a, make_double2(c[J], SIGN * s[J]));
+}
+
+// Radix 16 as 4 x 4: x[n1 + 4n2] -> DFT4 over n2 (k2), twiddle
+// w16^(n1*k2), DFT4 over n1 (k1); output k2 + 4k1.
+template <int SIGN> struct Dft<16, SIGN> {
+    __device__ __forceinline__ static void run(double2* v) {
+        double2 t[16];
+#pragma unroll
+        for (int n1 = 0; n1 < 4; ++n1) {
+            double2 a[4] = {v[n1], v[n1 + 4], v[n1 + 8], v[n1 + 12]};
+            Dft<4, SIGN>::run(a);
+#pragma unroll
+            for (int k2 = 0; k2 < 4; ++k2) t[4 * n1 + k2] = a[k2];
+        }
+        t[5] = mul_w16<1, SIGN>(t[5]);
+        t[6] = mul_w16<2, SIGN>(t[6]);
+        t[7] = mul_w16<3, SIGN>(t[7]);
+        t[9] = mul_w16<2, SIGN>(t[9]);
+        t[10] = mul_w16<4, SIGN>(t[10]);
+        t[11] = mul_w16<6, SIGN>(t[11]);
+        t[13] = mul_w16<3, SIGN>(t[13]);
+        t[14] = mul_w16<6, SIGN>(t[14]);
+        t[15] = mul_w16<9, SIGN>(t[15]);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) {
+            double2 b[4] = {t[k2], t[4 + k2], t[8 + k2], t[12 + k2]};
+            Dft<4, SIGN>::run(b);
+#pragma unroll
+            for (int k1 = 0; k1 < 4; ++k1) v[k2 + 4 * k1] = b[k1];
+        }
+    }
+};
+
+// omega_8^d * a with omega_8 = exp(-2*pi*i/8).
+template <int D>
+__device__ __forceinline__ double2 mul_w8(double2 a) {
+    constexpr double R2 = 0.7071067811865476;
+    if constexpr (D == 0) return a;
+    else if constexpr (D == 2) return make_double2(a.y, -a.x);
+    else if constexpr (D == 4) return make_double2(-a.x, -a.y);
+    else if constexpr (D == 6) return make_double2(-a.y, a.x);
+    else if constexpr (D == 1) return make_double2(R2 * (a.x + a.y), R2 * (a.y - a.x));
+    else if constexpr (D == 3) return make_double2(R2 * (a.y - a.x), -R2 * (a.x + a.y));
+    else if constexpr (D == 5) return make_double2(-R2 * (a.x + a.y), R2 * (a.x - a.y));
+    else return make_double2(R2 * (a.x - a.y), R2 * (a.x + a.y));
+}
+
+// Wave-local LDS positions (16-byte words within a line's region) of the
+// 64-point steps.  S1 holds the radix-E outputs y[k1][n] (n < 64), S2 the
+// first radix-8 step's outputs u[k1][a][c].  The XOR terms make every
+// 16-lane group of each access hit 16 different bank groups: the two k1 of a
+// group differ in bit 3, and the a ^ c term spreads S2's column reads.
+__device__ __forceinline__ int s1_pos(int k1, int n) { return k1 * 64 + (n ^ ((k1 & 1) << 3)); }
+__device__ __forceinline__ int s2_pos(int k1, int c, int a) { return k1 * 64 + ((c ^ (k1 & 1)) << 3) + (a ^ c); }
+
+// Transforms of one line of length L = 64E held by one wave.
+//   Input distribution (natural): x[e] = element l + 64e of lane l.
+//   Output distribution: butterfly beta = l + 64i (i < NB, active iff
+//   beta < NA) holds in x[8i + d] the frequency k1 + E*c + 8E*d, with
+//   k1 = beta >> 3, c = beta & 7.
+// fwd maps natural input to that distribution (forward DFT, unnormalised);
+// inv maps it back (inverse DFT, unnormalised: inv(fwd(x)) = L x).
+// Twiddles: T1[k1*64 + n] = omega_L^(n*k1), T2[c*8 + a] = omega_64^(a*c).
+template <int E>
+struct Line {
+    static_assert(E < 8 || E % 8 == 0, "line length: 64E with E < 8 or a multiple of 8");
+    static constexpr int L = 64 * E;
+    static constexpr int NA = 8 * E;                 // radix-8 butterflies per 64-point step
+    static constexpr int NB = E >= 8 ? E / 8 : 1;    // butterflies per lane
+    static constexpr int NX = E >= 8 ? E : 8;        // words per lane
+    static constexpr bool ALL = NA >= 64;            // every lane busy in the radix-8 steps
+
+    __device__ __forceinline__ static int out_index(int beta, int d) {
+        return (beta >> 3) + E * (beta & 7) + 8 * E * d;
+    }
+    // Region position d*NA + beta of frequency k (the output distribution
+    // laid out lane-contiguously).
+    __device__ __forceinline__ static int out_pos(int k) {
+        const int d = k / NA, rem = k - d * NA;
+        const int c = rem / E, k1 = rem - c * E;
+        return d * NA + 8 * k1 + c;
+    }
+
+    __device__ __forceinline__ static void fwd(double2 (&x)[NX], double2* R, const double2* T1, const double2* T2,
+                                               int l) {
+        Dft<E, -1>::run(x);
+#pragma unroll
+        for (int k = 1; k < E; ++k) x[k] = cmul(x[k], T1[k * 64 + l]);
+        wave_sync();
+#pragma unroll
+        for (int k = 0; k < E; ++k) R[s1_pos(k, l)] = x[k];
+        wave_sync();
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            const int beta = l + 64 * i;
+            if (ALL || beta < NA) {
+                const int k1 = beta >> 3, a = beta & 7;
+#pragma unroll
+                for (int b = 0; b < 8; ++b) x[8 * i + b] = R[s1_pos(k1, a + 8 * b)];
+                Dft<8, -1>::run(&x[8 * i]);
+#pragma unroll
+                for (int c = 1; c < 8; ++c) x[8 * i + c] = cmul(x[8 * i + c], T2[c * 8 + a]);
+            }
+        }
+        wave_sync();
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            const int beta = l + 64 * i;
+            if (ALL || beta < NA) {
+                const int k1 = beta >> 3, a = beta & 7;
+#pragma unroll
+                for (int c = 0; c < 8; ++c) R[s2_pos(k1, c, a)] = x[8 * i + c];
+            }
+        }
+        wave_sync();
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            const int beta = l + 64 * i;
+            if (ALL || beta < NA) {
+                const int k1 = beta >> 3, c = beta & 7;
+#pragma unroll
+                for (int a = 0; a < 8; ++a) x[8 * i + a] = R[s2_pos(k1, c, a)];
+                Dft<8, -1>::run(&x[8 * i]);
+            }
+        }
+    }
+
+    __device__ __forceinline__ static void inv(double2 (&x)[NX], double2* R, const double2* T1, const double2* T2,
+                                               int l) {
+#pragma unroll
+        for (int i = 0; i < NB; ++i)
+            if (ALL || l + 64 * i < NA) Dft<8, 1>::run(&x[8 * i]);
+        wave_sync();
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            const int beta = l + 64 * i;
+            if (ALL || beta < NA) {
+                const int k1 = beta >> 3, c = beta & 7;
+#pragma unroll
+                for (int a = 0; a < 8; ++a) R[s2_pos(k1, c, a)] = x[8 * i + a];
+            }
+        }
+        wave_sync();
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            const int beta = l + 64 * i;
+            if (ALL || beta < NA) {
+                const int k1 = beta >> 3, a = beta & 7;
+#pragma unroll
+                for (int c = 0; c < 8; ++c) x[8 * i + c] = R[s2_pos(k1, c, a)];
+#pragma unroll
+                for (int c = 1; c < 8; ++c) x[8 * i + c] = cmulc(x[8 * i + c], T2[c * 8 + a]);
+                Dft<8, 1>::run(&x[8 * i]);
+            }
+        }
+        wave_sync();
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            const int beta = l + 64 * i;
+            if (ALL || beta < NA) {
+                const int k1 = beta >> 3, a = beta & 7;
+#pragma unroll
+                for (int b = 0; b < 8; ++b) R[s1_pos(k1, a + 8 * b)] = x[8 * i + b];
+            }
+        }
+        wave_sync();
+#pragma unroll
+        for (int k = 0; k < E; ++k) x[k] = R[s1_pos(k, l)];
+#pragma unroll
+        for (int k = 1; k < E; ++k) x[k] = cmulc(x[k], T1[k * 64 + l]);
+        Dft<E, 1>::run(x);
+    }
+};
+
+// Phase stamps (profiling library only, LUCAS_PHASE_STAMPS): work-item 0 of
+// each of the first LUCAS_LINE_STAMP_WGS workgroups records the 100 MHz
+// real-time counter at the phase boundaries of the steady-state column
+// launches and of the row launches, six stamps per workgroup.
+#define LUCAS_LINE_STAMP_WGS 8192
+#ifdef LUCAS_PHASE_STAMPS
+__device__ unsigned long long lucas_line_stamps_col[LUCAS_LINE_STAMP_WGS * 8];
+__device__ unsigned long long lucas_line_stamps_row[LUCAS_LINE_STAMP_WGS * 8];
+#define LINE_STAMP(arr, on, k)                                                       \
+    do {                                                                             \
+        if ((on) && threadIdx.x == 0 && blockIdx.x < LUCAS_LINE_STAMP_WGS)           \
+            arr[blockIdx.x * 8 + (k)] = __builtin_amdgcn_s_memrealtime();            \
+    } while (0)
+#else
+#define LINE_STAMP(arr, on, k) \
+    do {                       \
+    } while (0)
+#endif
+
+// ---------------------------------------------------------------------------
+// Row kernel: rows R0, R1 of a pair (wave 0, wave 1).  Workgroup 0 takes rows
+// 0 and A/2, each paired with itself.  Frequency K = R + A*kk of the
+// half-length transform pairs with M - K: row A - R, column B-1-kk (row 0:
+// row 0, column -kk mod B).  With E = Z_K + conj(Z_{M-K}) and
+// O = -i (Z_K - conj(Z_{M-K})) the packed spectrum of the squared real signal
+// is (E^2 + omega_M^K O^2 + 2i E O) / 4; the 1/4 is in the unweight table.
+// ---------------------------------------------------------------------------
+template <int EB>
+__device__ __forceinline__ void line_row_body(const LucasJobDesc& J, int wg) {
+    using F = Line<EB>;
+    constexpr int B = 64 * EB, NA = F::NA, NB = F::NB;
+    double2* const lds = lucas_dynamic_lds();
+    double2* const T1 = lds + 2 * B;
+    double2* const T2 = T1 + B;
+    const int A = J.A;
+    const int tid = threadIdx.x, l = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const bool self = wg == 0;
+    const int R = self ? (w ? A / 2 : 0) : (w ? A - wg : wg);
+    double2* const mine = lds + w * B;
+    double2* const theirs = self ? mine : lds + (w ^ 1) * B;
+
+    // table loads first: the load counter drains in issue order, so waiting
+    // for them never waits for the row
+    LINE_STAMP(lucas_line_stamps_row, true, 0);
+    constexpr int NTAB = B + 64, NTJ = (NTAB + 127) / 128;
+    double2 tv[NTJ];
+#pragma unroll
+    for (int j = 0; j < NTJ; ++j) {
+        const int t = tid + 128 * j;
+        if (t < NTAB) tv[j] = gload2(J.lnB, t);
+    }
+    double2 x[F::NX];
+#pragma unroll
+    for (int e = 0; e < EB; ++e) x[e] = gload2(J.spec, (size_t)R * B + l + 64 * e);
+    const double2 t1 = gload2(J.rt1, (size_t)R * 64 + l);
+#pragma unroll
+    for (int j = 0; j < NTJ; ++j) {
+        const int t = tid + 128 * j;
+        if (t < NTAB) T1[t] = tv[j];
+    }
+    // outer twiddle omega_M^(R*c), c = l + 64e: rt1[R][l] * rt2[R][e]
+#pragma unroll
+    for (int e = 0; e < EB; ++e) x[e] = cmul(x[e], cmul(t1, gload2(J.rt2, (size_t)R * EB + e)));
+    __syncthreads();
+    LINE_STAMP(lucas_line_stamps_row, true, 1);
+    F::fwd(x, mine, T1, T2, l);
+    wave_sync();
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        const int beta = l + 64 * i;
+        if (F::ALL || beta < NA) {
+#pragma unroll
+            for (int d = 0; d < 8; ++d) mine[d * NA + beta] = x[8 * i + d];
+        }
+    }
+    __syncthreads();
+    LINE_STAMP(lucas_line_stamps_row, true, 2);
+    {
+        const double2 wR = gload2(J.twMA, R);  // omega_M^R
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            const int beta = l + 64 * i;
+            if (F::ALL || beta < NA) {
+                // omega_B^kk = omega_B^(k1 + EB*c) * omega_8^d
+                const double2 wb = cmul(wR, gload2(J.twB, (beta >> 3) + EB * (beta & 7)));
+#pragma unroll
+                for (int d = 0; d < 8; ++d) {
+                    const int kk = F::out_index(beta, d);
+                    int pos;
+                    if (self && R == 0) {
+                        pos = F::out_pos(kk == 0 ? 0 : B - kk);
+                    } else {
+                        pos = B - 1 - (d * NA + beta);  // frequency B-1-kk
+                    }
+                    const double2 zk = x[8 * i + d], zm = theirs[pos];
+                    double2 wk;
+                    switch (d) {
+                        case 0: wk = mul_w8<0>(wb); break;
+                        case 1: wk = mul_w8<1>(wb); break;
+                        case 2: wk = mul_w8<2>(wb); break;
+                        case 3: wk = mul_w8<3>(wb); break;
+                        case 4: wk = mul_w8<4>(wb); break;
+                        case 5: wk = mul_w8<5>(wb); break;
+                        case 6: wk = mul_w8<6>(wb); break;
+                        default: wk = mul_w8<7>(wb); break;
+                    }
+                    const double2 Ev = cadd(zk, cconj(zm));
+                    const double2 Ov = mul_si<-1>(csub(zk, cconj(zm)));
+                    const double2 P = cadd(cmul(Ev, Ev), cmul(wk, cmul(Ov, Ov)));
+                    const double2 Q = cmul(Ev, Ov);
+                    x[8 * i + d] = make_double2(P.x - 2.0 * Q.y, P.y + 2.0 * Q.x);
+                }
+            }
+        }
+    }
+    __syncthreads();  // every partner read is done before the regions are reused
+    LINE_STAMP(lucas_line_stamps_row, true, 3);
+    F::inv(x, mine, T1, T2, l);
+    LINE_STAMP(lucas_line_stamps_row, true, 4);
+#pragma unroll
+    for (int e = 0; e < EB; ++e) {
+        const double2 tw = cmul(t1, gload2(J.rt2, (size_t)R * EB + e));
+        gstore2(J.spec2, (size_t)R * B + l + 64 * e, cmulc(x[e], tw));
+    }
+    LINE_STAMP(lucas_line_stamps_row, true, 5);
+}
+
+// ---------------------------------------------------------------------------
+// Column kernel: workgroup `tile` owns columns c0 = K*tile .. c0+K-1; wave w
+// holds local column w, global column c0 - H + w (mod B).  Local columns
+// 0..H-1 are the halo: the restoration of digit j reads the rounded products
+// of digits j-NC .. j, and a tile's first digits need the products of the
+// H columns to its left.  A halo column left of column 0 wraps to column
+// B - H + w of the previous row (the cyclic wrap of IrrBaseBalanced.cu:64-70).
+// ---------------------------------------------------------------------------
+template <int EA, int K, int NC>
+__device__ __forceinline__ void line_col_body(const LucasJobDesc& J, int tile, int mode) {
+    using F = Line<EA>;
+    constexpr int A = 64 * EA, H = lucas_line_halo(NC), NL = K + H, T = 64 * NL, RS = A + 1;
+    constexpr int NA = F::NA, NB = F::NB;
+    double2* const lds = lucas_dynamic_lds();
+    double2* const T1 = lds + NL * RS;
+    double2* const T2 = T1 + A;
+    double2* const roww = T2 + 64;                          // {whi, iwhi} per row
+    int* const rowr = reinterpret_cast<int*>(roww + A);    // rhi per row
+    double2* const locw = roww + A + (A + 3) / 4;          // {wlo, iwlo} per local digit
+    int* const locr = reinterpret_cast<int*>(locw + 2 * NL);  // rlo per local digit
+    const int B = J.B, N = J.N;
+    const int c0 = tile * K;
+    const int tid = threadIdx.x, l = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const bool in_digits = (mode & LUCAS_IN_DIGITS) != 0;
+    const bool out_digits = (mode & LUCAS_OUT_DIGITS) != 0;
+    auto gcol = [&](int lc) {
+        const int g = c0 - H + lc;
+        return g < 0 ? g + B : g;
+    };
+    double2* const region = lds + w * RS;
+    const unsigned int seen = __hip_atomic_load(J.maxerr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned int seen_b = __hip_atomic_load(J.pbits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned int seen_z = __hip_atomic_load(J.pbits + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+
+    const bool stamp = mode == 0;
+    LINE_STAMP(lucas_line_stamps_col, stamp, 0);
+    // Tables (T1, T2; per row; per local digit), loads first, then the tile.
+    constexpr int NTAB = 2 * A + 64 + 2 * NL, NTJ = (NTAB + T - 1) / T;
+    double2 tv[NTJ];
+    int ti[NTJ];
+#pragma unroll
+    for (int j = 0; j < NTJ; ++j) {
+        const int t = tid + T * j;
+        if (t < A + 64) {
+            tv[j] = gload2(J.lnA, t);
+        } else if (t < 2 * A + 64) {
+            const int r = t - A - 64;
+            tv[j] = make_double2(gload(J.whi, r), gload(J.iwhi, r));
+            ti[j] = gload(J.rhi, r);
+        } else if (t < NTAB) {
+            const int u = t - 2 * A - 64;
+            const int jj = 2 * gcol(u >> 1) + (u & 1);
+            tv[j] = make_double2(gload(J.wlo, jj), gload(J.iwlo, jj));
+            ti[j] = gload(J.rlo, jj);
+        }
+    }
+    double2 tile_v[EA];
+    if (!in_digits) {
+#pragma unroll
+        for (int i = 0; i < EA; ++i) {
+            const int t = tid + T * i;
+            const int r = t / NL, lc = t - r * NL;
+            tile_v[i] = gload2(J.spec2, (size_t)r * B + gcol(lc));
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NTJ; ++j) {
+        const int t = tid + T * j;
+        if (t < A + 64) {
+            T1[t] = tv[j];
+        } else if (t < 2 * A + 64) {
+            roww[t - A - 64] = tv[j];
+            rowr[t - A - 64] = ti[j];
+        } else if (t < NTAB) {
+            locw[t - 2 * A - 64] = tv[j];
+            locr[t - 2 * A - 64] = ti[j];
+        }
+    }
+    if (!in_digits) {
+#pragma unroll
+        for (int i = 0; i < EA; ++i) {
+            const int t = tid + T * i;
+            const int r = t / NL, lc = t - r * NL;
+            lds[lc * RS + F::out_pos(r)] = tile_v[i];
+        }
+    }
+    __syncthreads();
+    LINE_STAMP(lucas_line_stamps_col, stamp, 1);
+
+    float err = 0.0f;
+    double pmax = 0.0, pmin = 0x1p60;  // largest / smallest |c_j| of this work-item
+    double2 x[F::NX];
+    // weight 2^(r_j/N) of the digits (2m, 2m+1) of element (r, local column lc)
+    auto weighted = [&](int lc, int r, int2 d) {
+        const int rh = rowr[r];
+        const double wh = roww[r].x;
+        const double ae = locw[2 * lc].x * wh * (locr[2 * lc] + rh >= N ? 0.5 : 1.0);
+        const double ao = locw[2 * lc + 1].x * wh * (locr[2 * lc + 1] + rh >= N ? 0.5 : 1.0);
+        return make_double2(ae * (double)d.x, ao * (double)d.y);
+    };
+
+    if (in_digits) {
+        // the own columns' digits, weighted, straight into the forward transform
+        if (w >= H) {
+            const int gc = c0 + w - H;
+#pragma unroll
+            for (int e = 0; e < EA; ++e) x[e] = weighted(w, l + 64 * e, gload2i(J.digits, (size_t)gc + (size_t)B * (l + 64 * e)));
+        }
+    } else {
+        // inverse column transform of every local column
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            const int beta = l + 64 * i;
+            if (F::ALL || beta < NA) {
+#pragma unroll
+                for (int d = 0; d < 8; ++d) x[8 * i + d] = region[d * NA + beta];
+            }
+        }
+        F::inv(x, region, T1, T2, l);
+        // unweight, -2 on digit 0, round half-even; the rounded products go
+        // back as round_pack words (natural row order)
+        const int gc = gcol(w);
+        const int rle = locr[2 * w], rlo = locr[2 * w + 1];
+        const double ile = locw[2 * w].y, ilo = locw[2 * w + 1].y;
+        wave_sync();
+#pragma unroll
+        for (int e = 0; e < EA; ++e) {
+            const int r = l + 64 * e;
+            const int rh = rowr[r];
+            const double iwh = roww[r].y;
+            const int rje = rle + rh, rjo = rlo + rh;
+            double ve = x[e].x * (ile * iwh * (rje >= N ? 2.0 : 1.0));
+            const double vo = x[e].y * (ilo * iwh * (rjo >= N ? 2.0 : 1.0));
+            if (gc == 0 && r == 0) ve -= 2.0;
+            float ee, eo;
+            double me, mo;
+            const double pe = round_pack(ve, (rje >= N ? rje - N : rje) < J.b, &ee, &me);
+            const double po = round_pack(vo, (rjo >= N ? rjo - N : rjo) < J.b, &eo, &mo);
+            err = fmaxf(err, fmaxf(ee, eo));
+            pmax = fmax(pmax, fmax(me, mo));
+            pmin = fmin(pmin, fmin(me, mo));
+            region[r] = make_double2(pe, po);
+        }
+        __syncthreads();
+        LINE_STAMP(lucas_line_stamps_col, stamp, 2);
+        // Carry-save restoration + single ripple x_j = d_j + h_{j-1}
+        // (loadIntToDoubleIBDWT, gpuLucas.cu:730-766), one work-item per row:
+        // the row's local digits u = 0 .. 2NL-1 (digit 2*(c0-H) + u) are kept
+        // top-down, so digit u's window (u, u-1, .., u-NC) is contiguous.
+        for (int r = tid; r < A; r += T) {
+            unsigned long long mag[2 * NL];
+            bool neg[2 * NL];
+            int wd[2 * NL];
+#pragma unroll
+            for (int lc = 0; lc < NL; ++lc) {
+                const int rr = (c0 - H + lc < 0) ? (r == 0 ? A - 1 : r - 1) : r;
+                const double2 v = lds[lc * RS + rr];
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    const unsigned long long bits = (unsigned long long)__double_as_longlong(e ? v.y : v.x);
+                    const int u = 2 * NL - 1 - (2 * lc + e);
+                    neg[u] = (bits >> 63) != 0;
+                    mag[u] = bits & ((1ull << 52) - 1);
+                    wd[u] = J.q + (int)((bits >> 62) & 1);
+                }
+            }
+            // carry out of the digit before the first own element
+            int hprev;
+            {
+                constexpr int u = 2 * NL - 1 - (2 * H - 1);
+                (void)restore_digit<NC>(mag + u, neg + u, wd + u, &hprev);
+            }
+#pragma unroll
+            for (int lo = 0; lo < K; ++lo) {
+                const int lc = H + lo;
+                const int u1 = 2 * NL - 1 - (2 * lc + 1), u0 = u1 + 1;
+                int h0, h1;
+                const int d1 = restore_digit<NC>(mag + u1, neg + u1, wd + u1, &h1);
+                const int d0 = restore_digit<NC>(mag + u0, neg + u0, wd + u0, &h0);
+                const int2 xv = make_int2(d0 + hprev, d1 + h0);
+                hprev = h1;
+                if (out_digits) {
+                    gstore2i(J.digits, (size_t)(c0 + lo) + (size_t)B * r, xv);
+                } else {
+                    lds[lc * RS + r] = weighted(lc, r, xv);
+                }
+            }
+        }
+        __syncthreads();
+        LINE_STAMP(lucas_line_stamps_col, stamp, 3);
+        if (out_digits) {
+            wave_max_err(err, seen, J.maxerr);
+            wave_max_bits(pmax, pmin, seen_b, seen_z, J.pbits);
+            return;
+        }
+        if (w >= H) {
+#pragma unroll
+            for (int e = 0; e < EA; ++e) x[e] = region[l + 64 * e];
+        }
+    }
+    // forward column transform of the own columns, outputs to the region
+    if (w >= H) {
+        F::fwd(x, region, T1, T2, l);
+        wave_sync();
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            const int beta = l + 64 * i;
+            if (F::ALL || beta < NA) {
+#pragma unroll
+                for (int d = 0; d < 8; ++d) region[d * NA + beta] = x[8 * i + d];
+            }
+        }
+    }
+    __syncthreads();
+    LINE_STAMP(lucas_line_stamps_col, stamp, 4);
+    // store row-major: the outer twiddle is the row kernel's
+    for (int t = tid; t < A * K; t += T) {
+        const int k = t / K, lo = t - k * K;
+        gstore2(J.spec, (size_t)k * B + c0 + lo, lds[(H + lo) * RS + F::out_pos(k)]);
+    }
+    LINE_STAMP(lucas_line_stamps_col, stamp, 5);
+    wave_max_err(err, seen, J.maxerr);
+    if (!in_digits) wave_max_bits(pmax, pmin, seen_b, seen_z, J.pbits);
+}
+
+// Column tile of workgroup `wg` of a single-job launch of n workgroups:
+// contiguous tile ranges per XCD (workgroup 8k + x runs tile k of XCD x's
+// range), so tiles that share halo lines share an L2.
+__device__ __forceinline__ int line_col_tile(int wg, int n) {
+    return (n % 8 == 0 && n >= 16) ? (n / 8) * (wg & 7) + (wg >> 3) : wg;
+}
+
+}  // namespace
+
+#define LUCAS_LINE_COL_KERNELS(EA, K, NC)                                                                        \
+    extern "C" __global__ void __launch_bounds__(64 * (K + lucas_line_halo(NC)))                                 \
+            lucas_lcol_##EA##_##K##_##NC(const LucasJobDesc* __restrict__ jobs, const int2* __restrict__ wgmap, \
+                                         int mode) {                                                             \
+        const int2 jm = wgmap[blockIdx.x];                                                                       \
+        line_col_body<EA, K, NC>(jobs[jm.x], jm.y, mode);                                                        \
+    }                                                                                                            \
+    extern "C" __global__ void __launch_bounds__(64 * (K + lucas_line_halo(NC)))                                 \
+            lucas_lcol1_##EA##_##K##_##NC(const LucasJobDesc J, int mode) {                                      \
+        line_col_body<EA, K, NC>(J, line_col_tile(blockIdx.x, J.col_wgs), mode);                                 \
+    }
+LUCAS_LINE_COLS(LUCAS_LINE_COL_KERNELS)
+
+#define LUCAS_LINE_ROW_KERNELS(EB)                                                                          \
+    extern "C" __global__ void __launch_bounds__(128)                                                       \
+            lucas_lrow_##EB(const LucasJobDesc* __restrict__ jobs, const int2* __restrict__ wgmap) {       \
+        const int2 jm = wgmap[blockIdx.x];                                                                  \
+        line_row_body<EB>(jobs[jm.x], jm.y);                                                                \
+    }                                                                                                       \
+    extern "C" __global__ void __launch_bounds__(128) lucas_lrow1_##EB(const LucasJobDesc J) {              \
+        line_row_body<EB>(J, blockIdx.x);                                                                   \
+    }
+LUCAS_LINE_ROWS(LUCAS_LINE_ROW_KERNELS)
+
+namespace lucas {
+
+namespace {
+typedef void (*LineColKernel)(const LucasJobDesc*, const int2*, int);
+typedef void (*LineCol1Kernel)(const LucasJobDesc, int);
+typedef void (*LineRowKernel)(const LucasJobDesc*, const int2*);
+typedef void (*LineRow1Kernel)(const LucasJobDesc);
+struct LineCol {
+    LineColKernel many;
+    LineCol1Kernel one;
+    int threads, lds_words;
+};
+struct LineRow {
+    LineRowKernel many;
+    LineRow1Kernel one;
+    int lds_words;
+};
+#define LUCAS_LINE_COL_ENTRY(EA, K, NC) \
+    {lucas_lcol_##EA##_##K##_##NC, lucas_lcol1_##EA##_##K##_##NC, 64 * (K + lucas_line_halo(NC)), \
+     lucas_line_col_words(EA, K, NC)},
+#define LUCAS_LINE_ROW_ENTRY(EB) {lucas_lrow_##EB, lucas_lrow1_##EB, lucas_line_row_words(EB)},
+const LineCol kLineCols[] = {LUCAS_LINE_COLS(LUCAS_LINE_COL_ENTRY)};
+const LineRow kLineRows[] = {LUCAS_LINE_ROWS(LUCAS_LINE_ROW_ENTRY)};
+}  // namespace
+
+hipError_t configure_line_kernels(size_t max_lds) {
+    for (const LineCol& k : kLineCols) {
+        if ((size_t)16 * k.lds_words > max_lds) return hipErrorInvalidValue;
+        hipError_t e = hipFuncSetAttribute((const void*)k.many, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           16 * k.lds_words);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void*)k.one, hipFuncAttributeMaxDynamicSharedMemorySize, 16 * k.lds_words);
+        if (e != hipSuccess) return e;
+    }
+    for (const LineRow& k : kLineRows) {
+        hipError_t e = hipFuncSetAttribute((const void*)k.many, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           16 * k.lds_words);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void*)k.one, hipFuncAttributeMaxDynamicSharedMemorySize, 16 * k.lds_words);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// Copies the line kernels' phase stamps (profiling library only).
+hipError_t read_line_phase_stamps(unsigned long long* col, int ncol, unsigned long long* row, int nrow) {
+#ifdef LUCAS_PHASE_STAMPS
+    if (ncol < 0 || nrow < 0 || ncol > LUCAS_LINE_STAMP_WGS || nrow > LUCAS_LINE_STAMP_WGS) return hipErrorInvalidValue;
+    hipError_t e =
+            hipMemcpyFromSymbol(col, HIP_SYMBOL(lucas_line_stamps_col), sizeof(unsigned long long) * 8 * ncol);
+    if (e == hipSuccess)
+        e = hipMemcpyFromSymbol(row, HIP_SYMBOL(lucas_line_stamps_row), sizeof(unsigned long long) * 8 * nrow);
+    return e;
+#else
+    (void)col, (void)ncol, (void)row, (void)nrow;
+    return hipErrorNotSupported;
+#endif
+}
+
+// Column launch of line instantiation `idx` (lucas_line_col_index): the
+// batched kernel over (jobs, map) when `one` is null, else the single-job one.
+hipError_t launch_line_col(int idx, const LucasJobDesc* jobs, const int2* map, const LucasJobDesc* one, int nwg,
+                           int mode, hipStream_t stream) {
+    const LineCol& k = kLineCols[idx];
+    if (one)
+        hipLaunchKernelGGL(k.one, dim3(nwg), dim3(k.threads), 16 * (size_t)k.lds_words, stream, *one, mode);
+    else
+        hipLaunchKernelGGL(k.many, dim3(nwg), dim3(k.threads), 16 * (size_t)k.lds_words, stream, jobs, map, mode);
+    return hipGetLastError();
+}
+
+hipError_t launch_line_row(int idx, const LucasJobDesc* jobs, const int2* map, const LucasJobDesc* one, int nwg,
+                           hipStream_t stream) {
+    const LineRow& k = kLineRows[idx];
+    if (one)
+        hipLaunchKernelGGL(k.one, dim3(nwg), dim3(128), 16 * (size_t)k.lds_words, stream, *one);
+    else
+        hipLaunchKernelGGL(k.many, dim3(nwg), dim3(128), 16 * (size_t)k.lds_words, stream, jobs, map);
+    return hipGetLastError();
+}
+
+}  // namespace lucas

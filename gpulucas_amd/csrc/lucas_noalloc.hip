@@ -30,6 +30,11 @@ hipError_t launch_col1_kernel(int threads, int p, int dense, const LucasJobDesc&
                               hipStream_t stream);
 hipError_t launch_row1_kernel(int threads, int p, const LucasJobDesc& job, int nwg, size_t lds, hipStream_t stream);
 hipError_t configure_kernels(size_t max_lds);
+hipError_t configure_line_kernels(size_t max_lds);
+hipError_t launch_line_col(int idx, const LucasJobDesc* jobs, const int2* map, const LucasJobDesc* one, int nwg,
+                           int mode, hipStream_t stream);
+hipError_t launch_line_row(int idx, const LucasJobDesc* jobs, const int2* map, const LucasJobDesc* one, int nwg,
+                           hipStream_t stream);
 }  // namespace lucas
 
 using namespace lucas;
@@ -62,7 +67,8 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 // Set-up of `device` (current): the device address of its `scratch`.
 int device_setup(int device, DeviceScratch* ds) {
     if (ds->ptr) return LUCAS_OK;
-    if (configure_kernels(kMaxLdsBytes) != hipSuccess) return LUCAS_ERR_DEVICE;
+    if (configure_kernels(kMaxLdsBytes) != hipSuccess || configure_line_kernels(kMaxLdsBytes) != hipSuccess)
+        return LUCAS_ERR_DEVICE;
     void* ptr = nullptr;
     if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
         (void)hipGetLastError();
@@ -137,9 +143,13 @@ extern "C" int lucas_step_digits_noalloc(int p, int N, int32_t* digits, long lon
         // the launch order of Engine::step: col(digits in), [row; col] x
         // (iters - 1), row, col(digits out)
         auto col = [&](int mode) {
+            if (pl.line) return launch_line_col(pl.line_col, nullptr, nullptr, &d, pl.col_wgs, mode, s);
             return launch_col1_kernel(pl.threads, pl.wA / 64, pl.col_dense, d, pl.col_wgs, pl.lds_col, mode, s);
         };
-        auto row = [&]() { return launch_row1_kernel(pl.threads, pl.wB / 64, d, pl.row_wgs, pl.lds_row, s); };
+        auto row = [&]() {
+            if (pl.line) return launch_line_row(pl.line_row, nullptr, nullptr, &d, pl.row_wgs, s);
+            return launch_row1_kernel(pl.threads, pl.wB / 64, d, pl.row_wgs, pl.lds_row, s);
+        };
         hipError_t e = col(LUCAS_IN_DIGITS);
         for (long long i = 1; i < iters && e == hipSuccess; ++i) {
             e = row();

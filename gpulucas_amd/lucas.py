@@ -48,7 +48,7 @@ class PlanInfo(C.Structure):
                 ("carry_depth", C.c_int), ("A", C.c_int), ("B", C.c_int), ("C", C.c_int),
                 ("halo", C.c_int), ("col_wgs", C.c_int), ("row_wgs", C.c_int),
                 ("threads", C.c_int), ("radix_pref", C.c_int), ("wave_A", C.c_int), ("wave_B", C.c_int), ("col_dense", C.c_int),
-                ("device_bytes", C.c_size_t)]
+                ("line", C.c_int), ("device_bytes", C.c_size_t)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -100,6 +100,7 @@ _SIGS = {
     "lucas_plan": (C.c_int, [C.c_int, C.c_int, _P(PlanInfo)]),
     "lucas_plan_shape": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _P(PlanInfo)]),
     "lucas_plan_list": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_int, _P(C.c_int)]),
+    "lucas_line_plan_list": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_int, _P(C.c_int)]),
     "lucas_autotune_shape": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _P(Options), _P(PlanInfo),
                                        _P(C.c_double)]),
     "lucas_digit_tables": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -213,6 +214,16 @@ def plan_list(p: int, N: int) -> list[dict]:
     arr = (PlanInfo * n.value)()
     _check(_lib().lucas_plan_list(p, N, arr, n.value, C.byref(n)), "plan_list")
     return [a.as_dict() for a in arr]
+
+
+def line_plan_list(p: int, N: int) -> list[dict]:
+    """The line-kernel plans of (p, N): one wave per row or column, shapes
+    compiled in (empty where none fits N and the carry depth)."""
+    n = C.c_int(0)
+    _check(_lib().lucas_line_plan_list(p, N, None, 0, C.byref(n)), f"line_plan_list(p={p}, N={N})")
+    arr = (PlanInfo * max(1, n.value))()
+    _check(_lib().lucas_line_plan_list(p, N, arr, n.value, C.byref(n)), "line_plan_list")
+    return [arr[i].as_dict() for i in range(n.value)]
 
 
 def autotune_shape(p: int, N: int = 0, iters: int = 200, max_candidates: int = 0, device: int = 0) -> tuple[dict, float]:

@@ -1,6 +1,7 @@
 """Phase profile of the two iteration kernels, from the profiling library.
 
     python -m gpulucas_amd.phases --case 2976221:163840 --case 37156667:2097152
+    python -m gpulucas_amd.phases --case 2976221:163840:256:2   # line kernels, A = 256, K = 2
 
 Loads libgpulucas_phases.so (the kernels with their phase stamps compiled
 in), runs `--iters` iterations of each exponent at its planned shape and
@@ -47,15 +48,18 @@ def summarize(stamps: np.ndarray, names: list[str]) -> dict:
     }
 
 
-def profile(p: int, N: int, iters: int = 20) -> dict:
+def profile(p: int, N: int, iters: int = 20, line: tuple[int, int] | None = None) -> dict:
+    """Per-phase times at the planned shape, or with `line` = (A, K) at the
+    line kernels' shape with A rows and K own columns per column workgroup."""
     from gpulucas_amd import lucas
 
-    with lucas.Context(p, N) as ctx:
+    plan = {"p": p, "N": N, "A": line[0], "C": line[1], "line": 1} if line else None
+    with lucas.Context(p, N, plan=plan) as ctx:
         info = ctx.info
         ctx.step(64)
         col, row = ctx.phase_stamps(iters)
     out = {"p": p, "N": N}
-    out.update({k: info[k] for k in ("A", "B", "C", "halo", "threads", "wave_A", "wave_B", "col_dense")})
+    out.update({k: info[k] for k in ("A", "B", "C", "halo", "threads", "wave_A", "wave_B", "col_dense", "line")})
     out["column_kernel"] = summarize(col, COL_PHASES)
     out["row_kernel"] = summarize(row, ROW_PHASES)
     return out
@@ -63,7 +67,7 @@ def profile(p: int, N: int, iters: int = 20) -> dict:
 
 def _table(r: dict) -> str:
     lines = [f"p={r['p']} N={r['N']} A={r['A']} B={r['B']} C={r['C']} H={r['halo']} T={r['threads']} "
-             f"W={r['wave_A']}/{r['wave_B']} dense={r['col_dense']}"]
+             f"W={r['wave_A']}/{r['wave_B']} dense={r['col_dense']} line={r['line']}"]
     for kern in ("column_kernel", "row_kernel"):
         s = r[kern]
         lines.append(f"  {kern}: {s['workgroups']} workgroups, span {s['span_us']} us, starts within "
@@ -76,7 +80,8 @@ def _table(r: dict) -> str:
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("--case", action="append", default=[], help="p:N (repeatable)")
+    ap.add_argument("--case", action="append", default=[],
+                    help="p:N, or p:N:A:K for the line kernels (repeatable)")
     ap.add_argument("--iters", type=int, default=20)
     ap.add_argument("--json", action="store_true")
     a = ap.parse_args(argv)
@@ -89,9 +94,11 @@ def main(argv=None) -> int:
     lucas.LIB_PATH = build.PHASES_LIB
     cases = a.case or ["2976221:163840"]
     for c in cases:
-        p, N = (int(x) for x in c.split(":"))
-        print(f"case p={p} N={N}", file=sys.stderr, flush=True)
-        r = profile(p, N, a.iters)
+        f = [int(x) for x in c.split(":")]
+        p, N = f[0], f[1]
+        line = (f[2], f[3]) if len(f) == 4 else None
+        print(f"case p={p} N={N}" + (f" line A={line[0]} K={line[1]}" if line else ""), file=sys.stderr, flush=True)
+        r = profile(p, N, a.iters, line)
         print(json.dumps(r) if a.json else _table(r), flush=True)
     return 0
 

@@ -48,6 +48,8 @@ hipError_t launch_col_kernel(int, int, int, const LucasJobDesc*, const int2*, in
 hipError_t launch_row_kernel(int, int, const LucasJobDesc*, const int2*, int, size_t, hipStream_t);
 hipError_t launch_col1_kernel(int, int, int, const LucasJobDesc&, int, size_t, int, hipStream_t);
 hipError_t launch_row1_kernel(int, int, const LucasJobDesc&, int, size_t, hipStream_t);
+hipError_t launch_line_col(int, const LucasJobDesc*, const int2*, const LucasJobDesc*, int, int, hipStream_t);
+hipError_t launch_line_row(int, const LucasJobDesc*, const int2*, const LucasJobDesc*, int, hipStream_t);
 }
 
 // 1: launch the single-job kernels (descriptor by value, tile from the
@@ -57,6 +59,10 @@ extern "C" void emu_set_single(int on) { g_single = on; }
 // Wave factors the plan must use (-1: the planner's first choice).
 static int g_wA = -1, g_wB = -1;
 extern "C" void emu_set_waves(int wA, int wB) { g_wA = wA; g_wB = wB; }
+
+// Line kernels (A rows, K own columns; A < 0: the tiled kernels).
+static int g_line_A = -1, g_line_K = 0;
+extern "C" void emu_set_line(int A, int K) { g_line_A = A; g_line_K = K; }
 
 // Product-size diagnostic of the last emu_run: max bit length, max 64-bit
 // leading-zero count of the rounded products.
@@ -70,10 +76,13 @@ extern "C" int emu_run(int p, int N, int A, int C, int T, const int32_t* digits_
                        int32_t* digits_out, double* maxerr) {
     using namespace lucas;
     Plan pl;
-    if (!(T || g_wA >= 0 || g_wB >= 0 ? make_plan_exact(p, N, A, C, T, 0, &pl, g_wA, g_wB)
-                                       : make_plan(p, N, A, C, &pl)).empty() &&
-        !make_plan_exact(p, N, A, C, T, 0, &pl, g_wA, g_wB, true).empty())
+    if (g_line_A >= 0) {
+        if (!make_line_plan(p, N, g_line_A, g_line_K, &pl).empty()) return -1;
+    } else if (!(T || g_wA >= 0 || g_wB >= 0 ? make_plan_exact(p, N, A, C, T, 0, &pl, g_wA, g_wB)
+                                              : make_plan(p, N, A, C, &pl)).empty() &&
+               !make_plan_exact(p, N, A, C, T, 0, &pl, g_wA, g_wB, true).empty()) {
         return -1;
+    }
     Tables t;
     make_tables(pl, &t);
     std::vector<double2> spec(pl.M), spec2(pl.M);
@@ -86,6 +95,7 @@ extern "C" int emu_run(int p, int N, int A, int C, int T, const int32_t* digits_
     d.ct1 = (const double2*)t.ct1.data(); d.ct2 = (const double2*)t.ct2.data();
     d.rt1 = (const double2*)t.rt1.data(); d.rt2 = (const double2*)t.rt2.data();
     d.twsA = (const double2*)t.twsA.data(); d.twsB = (const double2*)t.twsB.data();
+    d.lnA = (const double2*)t.lnA.data(); d.lnB = (const double2*)t.lnB.data();
     d.rlo = t.rlo.data(); d.rhi = t.rhi.data();
     d.wlo = t.wlo.data(); d.whi = t.whi.data(); d.iwlo = t.iwlo.data(); d.iwhi = t.iwhi.data();
     d.spec = spec.data(); d.spec2 = spec2.data(); d.digits = dig.data(); d.maxerr = &err;
@@ -95,13 +105,17 @@ extern "C" int emu_run(int p, int N, int A, int C, int T, const int32_t* digits_
     for (int w = 0; w < pl.col_wgs; ++w) cmap.push_back({0, w});
     for (int w = 0; w < pl.row_wgs; ++w) rmap.push_back({0, w});
     auto col = [&](int mode) {
-        if (g_single)
+        if (pl.line)
+            launch_line_col(pl.line_col, &d, cmap.data(), g_single ? &d : nullptr, pl.col_wgs, mode, nullptr);
+        else if (g_single)
             launch_col1_kernel(pl.threads, pl.wA / 64, pl.col_dense, d, pl.col_wgs, 0, mode, nullptr);
         else
             launch_col_kernel(pl.threads, pl.wA / 64, pl.col_dense, &d, cmap.data(), pl.col_wgs, 0, mode, nullptr);
     };
     auto row = [&]() {
-        if (g_single)
+        if (pl.line)
+            launch_line_row(pl.line_row, &d, rmap.data(), g_single ? &d : nullptr, pl.row_wgs, nullptr);
+        else if (g_single)
             launch_row1_kernel(pl.threads, pl.wB / 64, d, pl.row_wgs, 0, nullptr);
         else
             launch_row_kernel(pl.threads, pl.wB / 64, &d, rmap.data(), pl.row_wgs, 0, nullptr);

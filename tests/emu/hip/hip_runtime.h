@@ -41,6 +41,7 @@ typedef int hipError_t;
 typedef void* hipStream_t;
 #define hipSuccess 0
 #define hipErrorNotSupported 801
+#define hipErrorInvalidValue 1
 inline hipError_t hipGetLastError() { return 0; }
 #define hipFuncAttributeMaxDynamicSharedMemorySize 0
 inline hipError_t hipFuncSetAttribute(const void*, int, int) { return 0; }
@@ -64,6 +65,8 @@ inline int wave_read(int x, int src) {
 }
 }
 inline void __syncthreads() { emu::bar->arrive_and_wait(); }
+// the lanes of a wave run as separate threads here: a wave barrier is real
+inline void __builtin_amdgcn_wave_barrier() { emu::wave_bar[threadIdx.x >> 6]->arrive_and_wait(); }
 inline unsigned __lane_id() { return threadIdx.x & 63; }
 // DPP: quad_perm (ctrl < 0x100) and row_ror:n (0x121..0x12F) are the forms used
 inline int __builtin_amdgcn_update_dpp(int /*old*/, int src, int ctrl, int, int, bool) {

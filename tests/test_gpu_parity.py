@@ -78,9 +78,18 @@ def test_wave_factor_shapes_bit_exact_vs_oracle(p, N, A, C, T):
         assert ctx.max_roundoff() < ROUNDOFF_LIMIT
 
 
+def tiled_plan(p, N):
+    """The table's tiling of the general kernels for length N (the plan a
+    batch uses), as a plan for exponent p."""
+    from gpulucas_amd import tune
+    N_, A, C, T, pref, wA, wB, line, _ = [r for r in tune.tuned_entries() if r[0] == N and r[7] == 0][0]
+    return {"p": p, "N": N, "A": A, "C": C, "threads": T, "radix_pref": pref, "wave_A": wA, "wave_B": wB,
+            "col_dense": 0, "line": 0}
+
+
 def _tuned_pairs(max_n):
     from gpulucas_amd import tune
-    return sorted((row[7], row[0]) for row in tune.tuned_entries() if row[0] <= max_n)
+    return sorted({(row[8], row[0]) for row in tune.tuned_entries() if row[0] <= max_n})
 
 
 @pytest.mark.parametrize("p,N", _tuned_pairs(294912))
@@ -236,11 +245,12 @@ def test_long_transforms_vs_gmp(p, k, gmp_fixtures):
                                    (37156667, 2097152, 100), (37156667, 2097152, 2000)])
 def test_dense_column_kernel_vs_gmp(p, N, k, gmp_fixtures):
     """Lengths whose column grid has more 1024-item tiles than CUs run the
-    dense (64-VGPR, two workgroups per CU) column kernel: Res64 against GMP."""
-    info = lucas.plan(p, N)
-    assert info["col_dense"] == 1 and info["threads"] == 1024 and info["col_wgs"] > 256
+    dense (64-VGPR, two workgroups per CU) column kernel in their tiled plan
+    (the one batches use): Res64 against GMP."""
     print(f"case p={p} N={N} k={k}", flush=True)
-    with lucas.Context(p, N) as ctx:
+    with lucas.Context(p, N, plan=tiled_plan(p, N)) as ctx:
+        info = ctx.info
+        assert info["col_dense"] == 1 and info["threads"] == 1024 and info["col_wgs"] > 256
         ctx.step(k)
         want, zero = fixture_res64(gmp_fixtures, "large", p, k)
         assert ctx.res64() == (want, zero)
@@ -415,3 +425,54 @@ def test_prepare_rehearses_and_restores_the_state(gmp_fixtures):
             assert ctx.max_roundoff() == err and ctx.product_bits() == bits
         ctx.step(400)
         assert ctx.res64() == fixture_res64(gmp_fixtures, "c1_p86243", p, 1000)
+
+
+@pytest.mark.parametrize("p,N,A,K,iters", [
+    (299843, 16384, 64, 2, 300),      # E = 1 columns and rows
+    (449741, 24576, 192, 4, 300),     # radix-3 columns
+    (1049413, 57344, 448, 2, 300),    # radix-7 columns
+    (2976221, 163840, 256, 2, 300),   # the headline shape: radix-4 columns, radix-5 rows
+    (2976221, 163840, 320, 4, 300),   # radix-5 columns, radix-4 rows
+    (9068767, 524288, 512, 2, 200),   # E = 8
+    (37156667, 2097152, 1024, 4, 100),  # E = 16, carry depth 3, N = 2^21
+    (37156667, 2097152, 1024, 2, 100),
+])
+def test_line_kernels_equal_tiled_kernels(p, N, A, K, iters):
+    """The line kernels and the tiled kernels reach the same digits after
+    `iters` iterations from s0 = 4 (both restate the reference's
+    restoration, so their digit vectors are the same integers)."""
+    print(f"case p={p} N={N} line A={A} K={K} iters={iters}", flush=True)
+    from gpulucas_amd import tune
+    in_table = any(r[0] == N and r[7] == 0 for r in tune.tuned_entries())
+    with lucas.Context(p, N, plan=tiled_plan(p, N) if in_table else lucas.plan_list(p, N)[0]) as ctx:
+        assert ctx.info["line"] == 0
+        ctx.step(iters)
+        want = ctx.digits()
+    with lucas.Context(p, N, plan={"p": p, "N": N, "A": A, "C": K, "line": 1}) as ctx:
+        assert (ctx.info["line"], ctx.info["A"], ctx.info["C"]) == (1, A, K)
+        ctx.step(iters)
+        assert np.array_equal(ctx.digits(), want)
+        assert 0.0 < ctx.max_roundoff() < ROUNDOFF_LIMIT
+
+
+@pytest.mark.parametrize("p,N,A,K", [(141731, 8192, 64, 2), (2267549, 131072, 1024, 4), (599657, 32768, 256, 1)])
+def test_line_kernels_vs_oracle_steady_state(p, N, A, K):
+    """Line kernels from random digits: one and three iterations against the
+    oracle's restatement (carry depth 3, E = 16 columns, one own column)."""
+    print(f"case p={p} N={N} line A={A} K={K}", flush=True)
+    x = random_balanced(p, N, 17)
+    with lucas.Context(p, N, plan={"p": p, "N": N, "A": A, "C": K, "line": 1}) as ctx:
+        ctx.set_digits(x, 0)
+        ctx.step(1)
+        y, _ = oracle.ibdwt_iterate(p, N, x, 1)
+        assert np.array_equal(ctx.digits(), y)
+        ctx.step(2)
+        y, _ = oracle.ibdwt_iterate(p, N, y, 2)
+        assert np.array_equal(ctx.digits(), y)
+
+
+def test_headline_and_long_transform_plans_are_line_plans():
+    """The planner's single-exponent plans at the headline and 2^21 shapes
+    are the line kernels (the tuned table's entries)."""
+    assert lucas.plan(2976221, 163840)["line"] == 1
+    assert lucas.plan(37156667, 2097152)["line"] == 1

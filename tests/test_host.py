@@ -236,15 +236,23 @@ def test_miller_rabin_against_sieve():
 
 def test_tuned_table_entries_are_the_plans_chosen():
     """Every measured shape is admissible and is what the planner returns for
-    the exponent it was measured at."""
+    the exponent it was measured at: a length's line entry for a single
+    exponent, its tiling otherwise (and always for a batch's split)."""
     from gpulucas_amd import tune
     table = tune.tuned_entries()
     assert len(table) >= 40
-    assert len({row[0] for row in table}) == len(table)
-    for N, A, C, T, pref, wA, wB, p in table:
-        pl = lucas.plan(p, N)
-        assert (pl["A"], pl["C"], pl["threads"], pl["radix_pref"]) == (A, C, T, pref), N
-        assert (pl["wave_A"], pl["wave_B"]) == (wA, wB), N
+    assert len({(row[0], row[7]) for row in table}) == len(table)
+    line_n = {row[0] for row in table if row[7]}
+    assert {row[0] for row in table if not row[7]} >= line_n, "every line entry keeps its tiling for batches"
+    for N, A, C, T, pref, wA, wB, line, p in table:
+        if line:
+            pl = lucas.plan(p, N)
+            assert (pl["line"], pl["A"], pl["C"], pl["threads"]) == (1, A, C, T), N
+        elif N not in line_n:
+            pl = lucas.plan(p, N)
+            assert pl["line"] == 0
+            assert (pl["A"], pl["C"], pl["threads"], pl["radix_pref"]) == (A, C, T, pref), N
+            assert (pl["wave_A"], pl["wave_B"]) == (wA, wB), N
 
 
 @pytest.mark.parametrize("p", [86243, 1257787, 2976221, 37156667])

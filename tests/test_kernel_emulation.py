@@ -132,3 +132,33 @@ def test_emulated_product_bits_match_exact_convolution(p, N, A, C, iters):
     got, _ = emu.run(p, N, x, iters, A, C, 0)
     assert np.array_equal(got, y)
     assert emu.product_bits() == (want_b, want_z)
+
+
+# Line kernels: every row and column length instantiated (64E, E = 1, 2, 3,
+# 4, 5, 7, 8, 16), carry depths 2 and 3, K = 1, 2, 4 own columns, the halo
+# wrap of tile 0, one and several iterations, batched and single-job forms.
+@pytest.mark.parametrize("p,N,A,K,iters,single", [
+    (149921, 8192, 64, 2, 3, False),     # A = B = 64 (E = 1): several iterations from random digits
+    (141731, 8192, 64, 2, 1, True),      # carry depth 3 (q = 17)
+    (299843, 16384, 128, 2, 1, False),   # columns E = 2, rows E = 2
+    (449741, 24576, 192, 2, 1, True),    # columns E = 3 (radix 3), rows E = 2
+    (449741, 24576, 64, 4, 1, False),    # rows E = 3
+    (599657, 32768, 256, 1, 1, True),    # columns E = 4, one own column per workgroup
+    (749587, 40960, 320, 2, 2, False),   # columns E = 5 (radix 5)
+    (1049413, 57344, 448, 4, 1, True),   # columns E = 7 (radix 7)
+    (1199309, 65536, 512, 4, 1, False),  # columns E = 8 (lanes hold one radix-8 butterfly each)
+    (2267549, 131072, 1024, 4, 1, True),   # columns E = 16 (two butterflies per lane), depth 3
+    (2998279, 163840, 256, 2, 1, True),    # rows E = 5 (the headline shape)
+    (2098801, 114688, 128, 4, 1, False),   # rows E = 7
+    (2398633, 131072, 128, 4, 1, True),    # rows E = 8
+    (4797241, 262144, 128, 4, 1, True),    # rows E = 16
+])
+def test_emulated_line_kernels_match_oracle(p, N, A, K, iters, single):
+    w, _, _, _ = oracle.tables(p, N)
+    rng = np.random.default_rng(p % 1000)
+    half = (1 << (w - 1)).astype(np.int64)
+    x = (rng.integers(0, 2 * half) - half).astype(np.int32)
+    got, err = emu.run(p, N, x, iters, single=single, line=(A, K))
+    want, _ = oracle.ibdwt_iterate(p, N, x, iters)
+    assert np.array_equal(got, want)
+    assert 0.0 < err < 0.1
