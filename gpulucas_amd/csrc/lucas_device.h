@@ -28,6 +28,20 @@ __device__ __forceinline__ void gstore2(double2* p, size_t i, double2 x) {
     const lucas_v2d v = {x.x, x.y};
     ((LUCAS_GLOBAL lucas_v2d*)p)[i] = v;
 }
+// Spectrum word moved non-temporally when `nt` (the state outgrows the
+// caches: no allocation in L2, no write-back of dirty lines at kernel end).
+__device__ __forceinline__ double2 gload2_nt(const double2* p, size_t i, bool nt) {
+    if (nt) return make_double2(__builtin_nontemporal_load(&p[i].x), __builtin_nontemporal_load(&p[i].y));
+    return gload2(p, i);
+}
+__device__ __forceinline__ void gstore2_nt(double2* p, size_t i, double2 x, bool nt) {
+    if (nt) {
+        __builtin_nontemporal_store(x.x, &p[i].x);
+        __builtin_nontemporal_store(x.y, &p[i].y);
+    } else {
+        gstore2(p, i, x);
+    }
+}
 __device__ __forceinline__ int2 gload2i(const int* p, size_t i) {
     const lucas_v2i v = ((const LUCAS_GLOBAL lucas_v2i*)p)[i];
     return make_int2(v[0], v[1]);

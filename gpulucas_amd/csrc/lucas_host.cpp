@@ -243,20 +243,22 @@ std::vector<Plan> line_candidates(int p, int N) {
     return out;
 }
 
-std::string make_line_plan(int p, int N, int A, int K, Plan* out) {
+std::string make_line_plan(int p, int N, int A, int K, Plan* out, int nt) {
     Plan pl;
     std::string why = base_plan(p, N, &pl);
     if (!why.empty()) return why;
     for (const Plan& c : line_candidates(p, N)) {
         if ((A && c.A != A) || (K && c.C != K)) continue;
         *out = c;
+        out->nt = nt ? 1 : 0;
         return "";
     }
     return "no line-kernel shape for this N and carry depth";
 }
 
 // Measured fastest tilings per signal length (gpulucas_amd/tune.py); line: 1
-// for the line kernels (A rows, C own columns per column workgroup).
+// for the line kernels (A rows, C own columns per column workgroup), 2 for
+// the line kernels with non-temporal spectrum loads and stores.
 struct TunedShape {
     int N, A, C, threads, pref, wA, wB, line;
 };
@@ -288,7 +290,7 @@ std::string make_plan(int p, int N, int forceA, int forceC, Plan* out, bool shar
             if (t->N != N) continue;
             if (t->line) {
                 // a single exponent's latency plan (batches keep the tiled kernels)
-                if (!shared && make_line_plan(p, N, t->A, t->C, out).empty()) return "";
+                if (!shared && make_line_plan(p, N, t->A, t->C, out, t->line == 2).empty()) return "";
                 continue;
             }
             if (shared) {
@@ -436,6 +438,7 @@ void fill_desc_shape(const Plan& plan, LucasJobDesc* d) {
     d->pB = plan.wB / 64;
     d->ohA = outer_hi_count(plan.A);
     d->ohB = outer_hi_count(plan.B);
+    d->nt = plan.nt;
     stage_table(A1, plan.radA, d->stA);
     stage_table(B1, plan.radB, d->stB);
     d->m_A1 = magic_u32((unsigned)A1);

@@ -30,6 +30,7 @@ struct Plan {
     // in; line_col / line_row index their instantiations.  C is then the
     // column workgroup's own columns and threads its work-items.
     int line = 0, line_col = -1, line_row = -1;
+    int nt = 0;  // line kernels: spectra moved with non-temporal loads and stores
     std::vector<int> radA, radB;  // Stockham radices of the remaining factors A/wA, B/wB
     size_t lds_col = 0, lds_row = 0;  // bytes of LDS per workgroup
 };
@@ -64,7 +65,9 @@ std::string make_plan_exact(int p, int N, int A, int C, int threads, int pref, P
                             bool wide = false);
 // Line-kernel plan for (p, N) with A rows and K own columns per column
 // workgroup (A = 0 or K = 0: the first instantiated choice); "" on success.
-std::string make_line_plan(int p, int N, int A, int K, Plan* out);
+// `nt`: spectra moved with non-temporal loads and stores (faster once the
+// state outgrows the caches).
+std::string make_line_plan(int p, int N, int A, int K, Plan* out, int nt = 0);
 // Every line-kernel plan of (p, N).
 std::vector<Plan> line_candidates(int p, int N);
 // Default N for p: smallest admissible length under the bits-per-word rule.

@@ -282,7 +282,7 @@ __device__ __forceinline__ void line_row_body(const LucasJobDesc& J, int wg) {
     }
     double2 x[F::NX];
 #pragma unroll
-    for (int e = 0; e < EB; ++e) x[e] = gload2(J.spec, (size_t)R * B + l + 64 * e);
+    for (int e = 0; e < EB; ++e) x[e] = gload2_nt(J.spec, (size_t)R * B + l + 64 * e, J.nt);
     const double2 t1 = gload2(J.rt1, (size_t)R * 64 + l);
 #pragma unroll
     for (int j = 0; j < NTJ; ++j) {
@@ -351,7 +351,7 @@ __device__ __forceinline__ void line_row_body(const LucasJobDesc& J, int wg) {
 #pragma unroll
     for (int e = 0; e < EB; ++e) {
         const double2 tw = cmul(t1, gload2(J.rt2, (size_t)R * EB + e));
-        gstore2(J.spec2, (size_t)R * B + l + 64 * e, cmulc(x[e], tw));
+        gstore2_nt(J.spec2, (size_t)R * B + l + 64 * e, cmulc(x[e], tw), J.nt);
     }
     LINE_STAMP(lucas_line_stamps_row, true, 5);
 }
@@ -419,7 +419,7 @@ __device__ __forceinline__ void line_col_body(const LucasJobDesc& J, int tile, i
         for (int i = 0; i < EA; ++i) {
             const int t = tid + T * i;
             const int r = t / NL, lc = t - r * NL;
-            tile_v[i] = gload2(J.spec2, (size_t)r * B + gcol(lc));
+            tile_v[i] = gload2_nt(J.spec2, (size_t)r * B + gcol(lc), J.nt);
         }
     }
 #pragma unroll
@@ -575,7 +575,7 @@ __device__ __forceinline__ void line_col_body(const LucasJobDesc& J, int tile, i
     // store row-major: the outer twiddle is the row kernel's
     for (int t = tid; t < A * K; t += T) {
         const int k = t / K, lo = t - k * K;
-        gstore2(J.spec, (size_t)k * B + c0 + lo, lds[(H + lo) * RS + F::out_pos(k)]);
+        gstore2_nt(J.spec, (size_t)k * B + c0 + lo, lds[(H + lo) * RS + F::out_pos(k)], J.nt);
     }
     LINE_STAMP(lucas_line_stamps_col, stamp, 5);
     wave_max_err(err, seen, J.maxerr);

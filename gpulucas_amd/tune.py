@@ -13,7 +13,8 @@ before its cost model (rebuild the library afterwards).
 
 A length may hold two entries: the tiling of the general kernels (which
 batches always use) and, where the line kernels measured faster for a single
-exponent, a line entry (last field 1), which single exponents then use.
+exponent, a line entry (last field 1, or 2 with non-temporal spectrum loads
+and stores), which single exponents then use.
 """
 from __future__ import annotations
 
@@ -162,14 +163,15 @@ def measure_line(pairs, iters, out, margin=0.02):
             base = timed(ctx)
         best = None
         for c in cands:
-            with lucas.Context(p, N, plan={"p": p, "N": N, "A": c["A"], "C": c["C"], "line": 1}) as ctx:
-                us = timed(ctx)
-            if best is None or us < best[0]:
-                best = (us, c)
-        us, c = best
+            for kind in (1, 2):  # 2: non-temporal spectrum loads and stores
+                with lucas.Context(p, N, plan={"p": p, "N": N, "A": c["A"], "C": c["C"], "line": kind}) as ctx:
+                    us = timed(ctx)
+                if best is None or us < best[0]:
+                    best = (us, c, kind)
+        us, c, kind = best
         results[str(N)] = {"p": p, "N": N, "tiled_us": base, "line_us": us, "A": c["A"], "C": c["C"],
-                           "threads": c["threads"], "wins": us < base * (1 - margin)}
-        print(f"N={N} p={p}: tiled {base:.2f} us, line A={c['A']} K={c['C']} {us:.2f} us "
+                           "threads": c["threads"], "line": kind, "wins": us < base * (1 - margin)}
+        print(f"N={N} p={p}: tiled {base:.2f} us, line{kind} A={c['A']} K={c['C']} {us:.2f} us "
               f"({'line' if us < base * (1 - margin) else 'tiled'}, {time.time() - t0:.1f}s)", flush=True)
         with open(out, "w") as f:
             json.dump(results, f, indent=1)
@@ -186,8 +188,8 @@ def emit_line(paths):
     rows = [r for r in _current_rows() if r[1] == 0]
     for v in table.values():
         if v["wins"]:
-            rows.append((int(v["N"]), 1, int(v["A"]), int(v["C"]), int(v["threads"]), 8, 0, 0, float(v["line_us"]),
-                         int(v["p"])))
+            rows.append((int(v["N"]), int(v.get("line", 1)), int(v["A"]), int(v["C"]), int(v["threads"]), 8, 0, 0,
+                         float(v["line_us"]), int(v["p"])))
     _write_inc(rows)
 
 

@@ -61,7 +61,8 @@ typedef struct {
     int wave_A, wave_B; /* wave-register factor of the column / row transform (64, 128, 256; 0: none) */
     int col_dense;     /* 1: column kernel capped at 64 VGPRs, two 1024-item workgroups per CU */
     int line;          /* 1: line kernels (one wave per row or column, shapes compiled in);
-                          C = own columns and threads = work-items of a column workgroup */
+                          C = own columns and threads = work-items of a column workgroup;
+                          2: the same with non-temporal spectrum loads and stores */
     size_t device_bytes; /* device footprint of one job */
 } lucas_plan_info;
 

@@ -123,6 +123,10 @@ inline unsigned atomicMax(unsigned* p, unsigned v) {
     if (v > old) *p = v;
     return old;
 }
+template <typename T>
+inline T __builtin_nontemporal_load(const T* p) { return *p; }
+template <typename T>
+inline void __builtin_nontemporal_store(T v, T* p) { *p = v; }
 using std::fma; using std::fmax; using std::fmin; using std::fabs; using std::rint;
 inline float fmaxf(float a, float b) { return std::fmax(a, b); }
 

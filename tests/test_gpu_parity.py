@@ -427,29 +427,29 @@ def test_prepare_rehearses_and_restores_the_state(gmp_fixtures):
         assert ctx.res64() == fixture_res64(gmp_fixtures, "c1_p86243", p, 1000)
 
 
-@pytest.mark.parametrize("p,N,A,K,iters", [
-    (299843, 16384, 64, 2, 300),      # E = 1 columns and rows
-    (449741, 24576, 192, 4, 300),     # radix-3 columns
-    (1049413, 57344, 448, 2, 300),    # radix-7 columns
-    (2976221, 163840, 256, 2, 300),   # the headline shape: radix-4 columns, radix-5 rows
-    (2976221, 163840, 320, 4, 300),   # radix-5 columns, radix-4 rows
-    (9068767, 524288, 512, 2, 200),   # E = 8
-    (37156667, 2097152, 1024, 4, 100),  # E = 16, carry depth 3, N = 2^21
-    (37156667, 2097152, 1024, 2, 100),
+@pytest.mark.parametrize("p,N,A,K,iters,kind", [
+    (299843, 16384, 64, 2, 300, 1),      # E = 1 columns and rows
+    (449741, 24576, 192, 4, 300, 1),     # radix-3 columns
+    (1049413, 57344, 448, 2, 300, 1),    # radix-7 columns
+    (2976221, 163840, 256, 2, 300, 1),   # the headline shape: radix-4 columns, radix-5 rows
+    (2976221, 163840, 320, 4, 300, 2),   # radix-5 columns, radix-4 rows; non-temporal spectra
+    (9068767, 524288, 512, 2, 200, 1),   # E = 8
+    (37156667, 2097152, 1024, 4, 100, 2),  # E = 16, carry depth 3, N = 2^21, non-temporal spectra
+    (37156667, 2097152, 1024, 2, 100, 1),
 ])
-def test_line_kernels_equal_tiled_kernels(p, N, A, K, iters):
+def test_line_kernels_equal_tiled_kernels(p, N, A, K, iters, kind):
     """The line kernels and the tiled kernels reach the same digits after
     `iters` iterations from s0 = 4 (both restate the reference's
     restoration, so their digit vectors are the same integers)."""
-    print(f"case p={p} N={N} line A={A} K={K} iters={iters}", flush=True)
+    print(f"case p={p} N={N} line{kind} A={A} K={K} iters={iters}", flush=True)
     from gpulucas_amd import tune
     in_table = any(r[0] == N and r[7] == 0 for r in tune.tuned_entries())
     with lucas.Context(p, N, plan=tiled_plan(p, N) if in_table else lucas.plan_list(p, N)[0]) as ctx:
         assert ctx.info["line"] == 0
         ctx.step(iters)
         want = ctx.digits()
-    with lucas.Context(p, N, plan={"p": p, "N": N, "A": A, "C": K, "line": 1}) as ctx:
-        assert (ctx.info["line"], ctx.info["A"], ctx.info["C"]) == (1, A, K)
+    with lucas.Context(p, N, plan={"p": p, "N": N, "A": A, "C": K, "line": kind}) as ctx:
+        assert (ctx.info["line"], ctx.info["A"], ctx.info["C"]) == (kind, A, K)
         ctx.step(iters)
         assert np.array_equal(ctx.digits(), want)
         assert 0.0 < ctx.max_roundoff() < ROUNDOFF_LIMIT
@@ -474,5 +474,5 @@ def test_line_kernels_vs_oracle_steady_state(p, N, A, K):
 def test_headline_and_long_transform_plans_are_line_plans():
     """The planner's single-exponent plans at the headline and 2^21 shapes
     are the line kernels (the tuned table's entries)."""
-    assert lucas.plan(2976221, 163840)["line"] == 1
-    assert lucas.plan(37156667, 2097152)["line"] == 1
+    assert lucas.plan(2976221, 163840)["line"] >= 1
+    assert lucas.plan(37156667, 2097152)["line"] >= 1

@@ -247,7 +247,7 @@ def test_tuned_table_entries_are_the_plans_chosen():
     for N, A, C, T, pref, wA, wB, line, p in table:
         if line:
             pl = lucas.plan(p, N)
-            assert (pl["line"], pl["A"], pl["C"], pl["threads"]) == (1, A, C, T), N
+            assert (pl["line"], pl["A"], pl["C"], pl["threads"]) == (line, A, C, T), N
         elif N not in line_n:
             pl = lucas.plan(p, N)
             assert pl["line"] == 0
