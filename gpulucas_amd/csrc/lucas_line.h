@@ -38,8 +38,9 @@ inline constexpr int lucas_line_col_words(int EA, int K, int NC) {
     return NL * (A + 1) + A + 64 + A + (A + 3) / 4 + 2 * NL + (2 * NL + 3) / 4;
 }
 // LDS of a row workgroup (one row pair, two waves): two row regions, the row
-// transform's twiddles omega_B^(l*k1) [k1][l] and omega_64^t.
-inline constexpr int lucas_line_row_words(int EB) { return 3 * 64 * EB + 64; }
+// transform's twiddles omega_B^(l*k1) [k1][l] and omega_64^t, and the outer
+// twiddle factors rt2 of both rows.
+inline constexpr int lucas_line_row_words(int EB) { return 3 * 64 * EB + 64 + 2 * EB; }
 
 // Index of an instantiation in the lists above, or -1.
 inline int lucas_line_col_index(int EA, int K, int NC) {

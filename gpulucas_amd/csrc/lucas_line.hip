@@ -109,6 +109,38 @@ __device__ __forceinline__ double2 mul_w8(double2 a) {
 __device__ __forceinline__ int s1_pos(int k1, int n) { return k1 * 64 + (n ^ ((k1 & 1) << 3)); }
 __device__ __forceinline__ int s2_pos(int k1, int c, int a) { return k1 * 64 + ((c ^ (k1 & 1)) << 3) + (a ^ c); }
 
+// Twiddles of a line transform (see Line): read from the workgroup's LDS copy
+// of the line table (long lines), or held in registers, loaded once per
+// kernel from the global table (lines of up to 8 words per lane, whose
+// transforms are short chains: no LDS round trip for a twiddle).
+struct TwLds {
+    const double2* T1;
+    const double2* T2;
+    int l;
+    __device__ __forceinline__ double2 t1(int k) const { return T1[k * 64 + l]; }
+    __device__ __forceinline__ double2 t2(int, int c, int a) const { return T2[c * 8 + a]; }
+};
+template <int E>
+struct TwReg {
+    static constexpr int NB = E >= 8 ? E / 8 : 1;
+    double2 a1[E > 1 ? E - 1 : 1];
+    double2 a2[NB][7];
+    // tab: omega_L^(n*k1) at [k1*64 + n], omega_64^(a*c) at [64E + c*8 + a]
+    __device__ __forceinline__ void load(const double2* tab, int l) {
+#pragma unroll
+        for (int k = 1; k < E; ++k) a1[k - 1] = gload2(tab, k * 64 + l);
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            const int beta = l + 64 * i;
+            const int a = (8 * E >= 64 || beta < 8 * E) ? (beta & 7) : 0;
+#pragma unroll
+            for (int c = 1; c < 8; ++c) a2[i][c - 1] = gload2(tab, 64 * E + c * 8 + a);
+        }
+    }
+    __device__ __forceinline__ double2 t1(int k) const { return a1[k - 1]; }
+    __device__ __forceinline__ double2 t2(int i, int c, int) const { return a2[i][c - 1]; }
+};
+
 // Transforms of one line of length L = 64E held by one wave.
 //   Input distribution (natural): x[e] = element l + 64e of lane l.
 //   Output distribution: butterfly beta = l + 64i (i < NB, active iff
@@ -116,7 +148,8 @@ __device__ __forceinline__ int s2_pos(int k1, int c, int a) { return k1 * 64 + (
 //   k1 = beta >> 3, c = beta & 7.
 // fwd maps natural input to that distribution (forward DFT, unnormalised);
 // inv maps it back (inverse DFT, unnormalised: inv(fwd(x)) = L x).
-// Twiddles: T1[k1*64 + n] = omega_L^(n*k1), T2[c*8 + a] = omega_64^(a*c).
+// Twiddles (tw, a TwLds or TwReg): t1(k1) = omega_L^(n*k1) for lane n,
+// t2(i, c, a) = omega_64^(a*c).
 template <int E>
 struct Line {
     static_assert(E < 8 || E % 8 == 0, "line length: 64E with E < 8 or a multiple of 8");
@@ -137,11 +170,11 @@ struct Line {
         return d * NA + 8 * k1 + c;
     }
 
-    __device__ __forceinline__ static void fwd(double2 (&x)[NX], double2* R, const double2* T1, const double2* T2,
-                                               int l) {
+    template <class TW>
+    __device__ __forceinline__ static void fwd(double2 (&x)[NX], double2* R, const TW& tw, int l) {
         Dft<E, -1>::run(x);
 #pragma unroll
-        for (int k = 1; k < E; ++k) x[k] = cmul(x[k], T1[k * 64 + l]);
+        for (int k = 1; k < E; ++k) x[k] = cmul(x[k], tw.t1(k));
         wave_sync();
 #pragma unroll
         for (int k = 0; k < E; ++k) R[s1_pos(k, l)] = x[k];
@@ -155,7 +188,7 @@ struct Line {
                 for (int b = 0; b < 8; ++b) x[8 * i + b] = R[s1_pos(k1, a + 8 * b)];
                 Dft<8, -1>::run(&x[8 * i]);
 #pragma unroll
-                for (int c = 1; c < 8; ++c) x[8 * i + c] = cmul(x[8 * i + c], T2[c * 8 + a]);
+                for (int c = 1; c < 8; ++c) x[8 * i + c] = cmul(x[8 * i + c], tw.t2(i, c, a));
             }
         }
         wave_sync();
@@ -181,8 +214,8 @@ struct Line {
         }
     }
 
-    __device__ __forceinline__ static void inv(double2 (&x)[NX], double2* R, const double2* T1, const double2* T2,
-                                               int l) {
+    template <class TW>
+    __device__ __forceinline__ static void inv(double2 (&x)[NX], double2* R, const TW& tw, int l) {
 #pragma unroll
         for (int i = 0; i < NB; ++i)
             if (ALL || l + 64 * i < NA) Dft<8, 1>::run(&x[8 * i]);
@@ -205,7 +238,7 @@ struct Line {
 #pragma unroll
                 for (int c = 0; c < 8; ++c) x[8 * i + c] = R[s2_pos(k1, c, a)];
 #pragma unroll
-                for (int c = 1; c < 8; ++c) x[8 * i + c] = cmulc(x[8 * i + c], T2[c * 8 + a]);
+                for (int c = 1; c < 8; ++c) x[8 * i + c] = cmulc(x[8 * i + c], tw.t2(i, c, a));
                 Dft<8, 1>::run(&x[8 * i]);
             }
         }
@@ -223,7 +256,7 @@ struct Line {
 #pragma unroll
         for (int k = 0; k < E; ++k) x[k] = R[s1_pos(k, l)];
 #pragma unroll
-        for (int k = 1; k < E; ++k) x[k] = cmulc(x[k], T1[k * 64 + l]);
+        for (int k = 1; k < E; ++k) x[k] = cmulc(x[k], tw.t1(k));
         Dft<E, 1>::run(x);
     }
 };
@@ -262,6 +295,7 @@ __device__ __forceinline__ void line_row_body(const LucasJobDesc& J, int wg) {
     double2* const lds = lucas_dynamic_lds();
     double2* const T1 = lds + 2 * B;
     double2* const T2 = T1 + B;
+    double2* const T2R = T2 + 64;  // rt2 rows of the two rows, [wave][e]
     const int A = J.A;
     const int tid = threadIdx.x, l = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -273,28 +307,47 @@ __device__ __forceinline__ void line_row_body(const LucasJobDesc& J, int wg) {
     // table loads first: the load counter drains in issue order, so waiting
     // for them never waits for the row
     LINE_STAMP(lucas_line_stamps_row, true, 0);
-    constexpr int NTAB = B + 64, NTJ = (NTAB + 127) / 128;
-    double2 tv[NTJ];
+    // the line twiddles: in registers for short rows, else an LDS copy
+    constexpr bool REG = EB <= 8;
+    constexpr int NTAB = REG ? 0 : B + 64, NTJ = (NTAB + 127) / 128;
+    double2 tv[NTJ > 0 ? NTJ : 1];
 #pragma unroll
     for (int j = 0; j < NTJ; ++j) {
         const int t = tid + 128 * j;
         if (t < NTAB) tv[j] = gload2(J.lnB, t);
     }
+    TwReg<EB> twr;
+    if constexpr (REG) twr.load(J.lnB, l);
+    const TwLds twl{T1, T2, l};
     double2 x[F::NX];
 #pragma unroll
     for (int e = 0; e < EB; ++e) x[e] = gload2_nt(J.spec, (size_t)R * B + l + 64 * e, J.nt);
+    // Every twiddle the kernel needs is loaded here, so no later phase waits
+    // on a global load: the outer twiddle omega_M^(R*c), c = l + 64e, is
+    // rt1[R][l] * rt2[R][e] (the rt2 row goes to LDS for the store), and the
+    // square's omega_M^K = omega_M^R * omega_B^(k1 + EB*c) * omega_8^d.
     const double2 t1 = gload2(J.rt1, (size_t)R * 64 + l);
+    double2 t2v = make_double2(0.0, 0.0);
+    if (l < EB) t2v = gload2(J.rt2, (size_t)R * EB + l);
+    const double2 wR = gload2(J.twMA, R);
+    double2 wbv[NB];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        const int beta = l + 64 * i;
+        wbv[i] = gload2(J.twB, (F::ALL || beta < NA) ? (beta >> 3) + EB * (beta & 7) : 0);
+    }
 #pragma unroll
     for (int j = 0; j < NTJ; ++j) {
         const int t = tid + 128 * j;
         if (t < NTAB) T1[t] = tv[j];
     }
-    // outer twiddle omega_M^(R*c), c = l + 64e: rt1[R][l] * rt2[R][e]
-#pragma unroll
-    for (int e = 0; e < EB; ++e) x[e] = cmul(x[e], cmul(t1, gload2(J.rt2, (size_t)R * EB + e)));
+    if (l < EB) T2R[w * EB + l] = t2v;
     __syncthreads();
+#pragma unroll
+    for (int e = 0; e < EB; ++e) x[e] = cmul(x[e], cmul(t1, T2R[w * EB + e]));
     LINE_STAMP(lucas_line_stamps_row, true, 1);
-    F::fwd(x, mine, T1, T2, l);
+    if constexpr (REG) F::fwd(x, mine, twr, l);
+    else F::fwd(x, mine, twl, l);
     wave_sync();
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
@@ -307,13 +360,12 @@ __device__ __forceinline__ void line_row_body(const LucasJobDesc& J, int wg) {
     __syncthreads();
     LINE_STAMP(lucas_line_stamps_row, true, 2);
     {
-        const double2 wR = gload2(J.twMA, R);  // omega_M^R
 #pragma unroll
         for (int i = 0; i < NB; ++i) {
             const int beta = l + 64 * i;
             if (F::ALL || beta < NA) {
                 // omega_B^kk = omega_B^(k1 + EB*c) * omega_8^d
-                const double2 wb = cmul(wR, gload2(J.twB, (beta >> 3) + EB * (beta & 7)));
+                const double2 wb = cmul(wR, wbv[i]);
 #pragma unroll
                 for (int d = 0; d < 8; ++d) {
                     const int kk = F::out_index(beta, d);
@@ -346,11 +398,12 @@ __device__ __forceinline__ void line_row_body(const LucasJobDesc& J, int wg) {
     }
     __syncthreads();  // every partner read is done before the regions are reused
     LINE_STAMP(lucas_line_stamps_row, true, 3);
-    F::inv(x, mine, T1, T2, l);
+    if constexpr (REG) F::inv(x, mine, twr, l);
+    else F::inv(x, mine, twl, l);
     LINE_STAMP(lucas_line_stamps_row, true, 4);
 #pragma unroll
     for (int e = 0; e < EB; ++e) {
-        const double2 tw = cmul(t1, gload2(J.rt2, (size_t)R * EB + e));
+        const double2 tw = cmul(t1, T2R[w * EB + e]);
         gstore2_nt(J.spec2, (size_t)R * B + l + 64 * e, cmulc(x[e], tw), J.nt);
     }
     LINE_STAMP(lucas_line_stamps_row, true, 5);
@@ -366,6 +419,7 @@ __device__ __forceinline__ void line_row_body(const LucasJobDesc& J, int wg) {
 // ---------------------------------------------------------------------------
 template <int EA, int K, int NC>
 __device__ __forceinline__ void line_col_body(const LucasJobDesc& J, int tile, int mode) {
+    static_assert(NC >= 2 && NC <= 3, "the row restoration assumes widths of at least 16 bits");
     using F = Line<EA>;
     constexpr int A = 64 * EA, H = lucas_line_halo(NC), NL = K + H, T = 64 * NL, RS = A + 1;
     constexpr int NA = F::NA, NB = F::NB;
@@ -394,14 +448,19 @@ __device__ __forceinline__ void line_col_body(const LucasJobDesc& J, int tile, i
     const bool stamp = mode == 0;
     LINE_STAMP(lucas_line_stamps_col, stamp, 0);
     // Tables (T1, T2; per row; per local digit), loads first, then the tile.
+    // the line twiddles: in registers for short columns, else an LDS copy
+    constexpr bool REG = EA <= 8;
     constexpr int NTAB = 2 * A + 64 + 2 * NL, NTJ = (NTAB + T - 1) / T;
     double2 tv[NTJ];
     int ti[NTJ];
+    TwReg<EA> twr;
+    if constexpr (REG) twr.load(J.lnA, l);
+    const TwLds twl{T1, T2, l};
 #pragma unroll
     for (int j = 0; j < NTJ; ++j) {
         const int t = tid + T * j;
         if (t < A + 64) {
-            tv[j] = gload2(J.lnA, t);
+            if (!REG) tv[j] = gload2(J.lnA, t);
         } else if (t < 2 * A + 64) {
             const int r = t - A - 64;
             tv[j] = make_double2(gload(J.whi, r), gload(J.iwhi, r));
@@ -426,7 +485,7 @@ __device__ __forceinline__ void line_col_body(const LucasJobDesc& J, int tile, i
     for (int j = 0; j < NTJ; ++j) {
         const int t = tid + T * j;
         if (t < A + 64) {
-            T1[t] = tv[j];
+            if (!REG) T1[t] = tv[j];
         } else if (t < 2 * A + 64) {
             roww[t - A - 64] = tv[j];
             rowr[t - A - 64] = ti[j];
@@ -475,7 +534,8 @@ __device__ __forceinline__ void line_col_body(const LucasJobDesc& J, int tile, i
                 for (int d = 0; d < 8; ++d) x[8 * i + d] = region[d * NA + beta];
             }
         }
-        F::inv(x, region, T1, T2, l);
+        if constexpr (REG) F::inv(x, region, twr, l);
+        else F::inv(x, region, twl, l);
         // unweight, -2 on digit 0, round half-even; the rounded products go
         // back as round_pack words (natural row order)
         const int gc = gcol(w);
@@ -503,39 +563,70 @@ __device__ __forceinline__ void line_col_body(const LucasJobDesc& J, int tile, i
         __syncthreads();
         LINE_STAMP(lucas_line_stamps_col, stamp, 2);
         // Carry-save restoration + single ripple x_j = d_j + h_{j-1}
-        // (loadIntToDoubleIBDWT, gpuLucas.cu:730-766), one work-item per row:
-        // the row's local digits u = 0 .. 2NL-1 (digit 2*(c0-H) + u) are kept
-        // top-down, so digit u's window (u, u-1, .., u-NC) is contiguous.
+        // (llintToIrrBal<n>, IrrBaseBalanced.cu:52-110, and
+        // loadIntToDoubleIBDWT, gpuLucas.cu:730-766), one work-item per row
+        // over the row's local digits u = 0 .. 2NL-1 (digit 2*(c0-H) + u).
+        // Digit t sums, for i = 0..NC, the field of product t-i that starts
+        // i digits above it: bits [w_{t-i} + .. + w_{t-1}, + w_t) of |c_{t-i}|.
+        // Each product is therefore cut once into its successive digit-width
+        // remainders |c_u| >> (w_u + .. + w_{u+i-1}), whose low 32 bits hold
+        // every field (w_t <= 21).
+        const unsigned int mq = (1u << J.q) - 1u, mq1 = (2u << J.q) - 1u;
         for (int r = tid; r < A; r += T) {
-            unsigned long long mag[2 * NL];
-            bool neg[2 * NL];
-            int wd[2 * NL];
+            unsigned int rem[2 * NL][NC + 1];
+            unsigned int hw[2 * NL];
+            int sg[2 * NL];  // +1 / -1
+            bool wide[2 * NL];
 #pragma unroll
             for (int lc = 0; lc < NL; ++lc) {
                 const int rr = (c0 - H + lc < 0) ? (r == 0 ? A - 1 : r - 1) : r;
                 const double2 v = lds[lc * RS + rr];
 #pragma unroll
                 for (int e = 0; e < 2; ++e) {
-                    const unsigned long long bits = (unsigned long long)__double_as_longlong(e ? v.y : v.x);
-                    const int u = 2 * NL - 1 - (2 * lc + e);
-                    neg[u] = (bits >> 63) != 0;
-                    mag[u] = bits & ((1ull << 52) - 1);
-                    wd[u] = J.q + (int)((bits >> 62) & 1);
+                    const int u = 2 * lc + e;
+                    hw[u] = (unsigned int)__double2hiint(e ? v.y : v.x);
+                    rem[u][0] = (unsigned int)__double2loint(e ? v.y : v.x);
+                    sg[u] = ((int)hw[u] >> 31) | 1;
+                    wide[u] = (hw[u] >> 30) & 1u;
                 }
             }
-            // carry out of the digit before the first own element
-            int hprev;
-            {
-                constexpr int u = 2 * NL - 1 - (2 * H - 1);
-                (void)restore_digit<NC>(mag + u, neg + u, wd + u, &hprev);
+            // |c_u| = (h:lo) with h < 2^20; every width is >= 16 (NC <= 3),
+            // so after two shifts the remainder is a 32-bit value
+#pragma unroll
+            for (int u = 0; u < 2 * NL; ++u) {
+                unsigned int h = hw[u] & 0xFFFFFu, lo = rem[u][0];
+#pragma unroll
+                for (int k = 1; k <= NC; ++k) {
+                    if (u + k - 1 < 2 * NL) {
+                        const int wk = J.q + (int)wide[u + k - 1];
+                        lo = __builtin_amdgcn_alignbit(h, lo, wk);
+                        h >>= wk;
+                        rem[u][k] = lo;
+                    }
+                }
             }
+            // digit t from the fields of products t, t-1, .., t-NC; returns the
+            // balanced digit and the reference's carry (sum + half) >> W for
+            // sum >= half, -((half - sum) >> W) for sum < -half, else 0
+            auto restore = [&](int t, int* carry) {
+                const unsigned int M = wide[t] ? mq1 : mq;
+                const int W = J.q + (int)wide[t];
+                int sum = 0;
+#pragma unroll
+                for (int k = 0; k <= NC; ++k) sum += __mul24(sg[t - k], (int)(rem[t - k][k] & M));
+                const int half = (int)((M + 1u) >> 1);
+                const int h = (sum + half + (sum >> 31) + (sum == -half ? 1 : 0)) >> W;
+                *carry = h;
+                return sum - (h << W);
+            };
+            int hprev;
+            (void)restore(2 * H - 1, &hprev);
 #pragma unroll
             for (int lo = 0; lo < K; ++lo) {
                 const int lc = H + lo;
-                const int u1 = 2 * NL - 1 - (2 * lc + 1), u0 = u1 + 1;
                 int h0, h1;
-                const int d1 = restore_digit<NC>(mag + u1, neg + u1, wd + u1, &h1);
-                const int d0 = restore_digit<NC>(mag + u0, neg + u0, wd + u0, &h0);
+                const int d0 = restore(2 * lc, &h0);
+                const int d1 = restore(2 * lc + 1, &h1);
                 const int2 xv = make_int2(d0 + hprev, d1 + h0);
                 hprev = h1;
                 if (out_digits) {
@@ -559,7 +650,8 @@ __device__ __forceinline__ void line_col_body(const LucasJobDesc& J, int tile, i
     }
     // forward column transform of the own columns, outputs to the region
     if (w >= H) {
-        F::fwd(x, region, T1, T2, l);
+        if constexpr (REG) F::fwd(x, region, twr, l);
+        else F::fwd(x, region, twl, l);
         wave_sync();
 #pragma unroll
         for (int i = 0; i < NB; ++i) {

@@ -109,6 +109,10 @@ inline unsigned __brev(unsigned x) {
     for (int i = 0; i < 32; ++i) r |= ((x >> i) & 1u) << (31 - i);
     return r;
 }
+inline int __mul24(int a, int b) { return a * b; }
+inline unsigned __builtin_amdgcn_alignbit(unsigned hi, unsigned lo, unsigned s) {
+    return (unsigned)((((unsigned long long)hi << 32) | lo) >> (s & 31));
+}
 inline unsigned __umulhi(unsigned a, unsigned b) { return (unsigned)(((unsigned long long)a * b) >> 32); }
 inline unsigned __float_as_uint(float f) { unsigned u; std::memcpy(&u, &f, 4); return u; }
 #define __HIP_MEMORY_SCOPE_AGENT 3
