@@ -269,17 +269,18 @@ __device__ __forceinline__ int restore_digit(const unsigned long long* mag, cons
 // lies in [2^52, 2^53) where doubles are the integers, so the addition rounds
 // |v| to |c| (rint is odd-symmetric) and t's mantissa field holds |c|; the
 // packing only rewrites t's sign and exponent bits.  *err receives |v - c|,
-// and *mag |c|.  |v| >= 2^52 (or NaN) has no meaningful rounding at all: it
-// reports roundoff 1 and packs 0, so the gate fires.
+// and *mag |c|.  |v| >= 2^52 (or NaN) has no meaningful rounding at all: t's
+// exponent then leaves that of [2^52, 2^53), the roundoff reported is 1 and
+// the gate fires.
 __device__ __forceinline__ double round_pack(double v, bool wide, float* err, double* mag) {
     const double av = fabs(v);
-    const bool fits = av < 0x1p52;
-    const double t = (fits ? av : 0.0) + 0x1p52;
+    const double t = av + 0x1p52;
     const double c = t - 0x1p52;
-    *err = fits ? (float)fabs(av - c) : 1.0f;
+    const unsigned int th = (unsigned int)__double2hiint(t);
+    *err = th < 0x43400000u ? (float)fabs(av - c) : 1.0f;
     *mag = c;
-    const unsigned int hi = ((unsigned int)__double2hiint(t) & 0xFFFFFu) | ((unsigned int)wide << 30) |
-                            (v < 0.0 ? 0x80000000u : 0u);
+    const unsigned int hi = (th & 0xFFFFFu) | ((unsigned int)wide << 30) |
+                            ((unsigned int)__double2hiint(v) & 0x80000000u);
     return __hiloint2double((int)hi, __double2loint(t));
 }
 

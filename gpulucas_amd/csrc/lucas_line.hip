@@ -560,89 +560,90 @@ __device__ __forceinline__ void line_col_body(const LucasJobDesc& J, int tile, i
             pmin = fmin(pmin, fmin(me, mo));
             region[r] = make_double2(pe, po);
         }
+        // the maxima are reduced here, not at the end: held to the end they
+        // would keep every element's intermediates live through the carry
+        wave_max_err(err, seen, J.maxerr);
+        wave_max_bits(pmax, pmin, seen_b, seen_z, J.pbits);
         __syncthreads();
         LINE_STAMP(lucas_line_stamps_col, stamp, 2);
         // Carry-save restoration + single ripple x_j = d_j + h_{j-1}
         // (llintToIrrBal<n>, IrrBaseBalanced.cu:52-110, and
         // loadIntToDoubleIBDWT, gpuLucas.cu:730-766), one work-item per row
         // over the row's local digits u = 0 .. 2NL-1 (digit 2*(c0-H) + u).
-        // Digit t sums, for i = 0..NC, the field of product t-i that starts
-        // i digits above it: bits [w_{t-i} + .. + w_{t-1}, + w_t) of |c_{t-i}|.
-        // Each product is therefore cut once into its successive digit-width
-        // remainders |c_u| >> (w_u + .. + w_{u+i-1}), whose low 32 bits hold
-        // every field (w_t <= 21).
+        // Digit t sums, for k = 0..NC, the field of product t-k that starts k
+        // digits above it: bits [w_{t-k} + .. + w_{t-1}, + w_t) of |c_{t-k}|.
+        // Walking up the row, a window holds the NC+1 products that feed the
+        // current digit, each shifted down by the widths of the digits passed
+        // (w_t <= 21, so every field is in the low 32 bits).
         const unsigned int mq = (1u << J.q) - 1u, mq1 = (2u << J.q) - 1u;
         for (int r = tid; r < A; r += T) {
-            unsigned int rem[2 * NL][NC + 1];
-            unsigned int hw[2 * NL];
-            int sg[2 * NL];  // +1 / -1
-            bool wide[2 * NL];
+            unsigned int hw[2 * NL], lw[2 * NL];
 #pragma unroll
             for (int lc = 0; lc < NL; ++lc) {
                 const int rr = (c0 - H + lc < 0) ? (r == 0 ? A - 1 : r - 1) : r;
                 const double2 v = lds[lc * RS + rr];
-#pragma unroll
-                for (int e = 0; e < 2; ++e) {
-                    const int u = 2 * lc + e;
-                    hw[u] = (unsigned int)__double2hiint(e ? v.y : v.x);
-                    rem[u][0] = (unsigned int)__double2loint(e ? v.y : v.x);
-                    sg[u] = ((int)hw[u] >> 31) | 1;
-                    wide[u] = (hw[u] >> 30) & 1u;
-                }
+                hw[2 * lc] = (unsigned int)__double2hiint(v.x);
+                lw[2 * lc] = (unsigned int)__double2loint(v.x);
+                hw[2 * lc + 1] = (unsigned int)__double2hiint(v.y);
+                lw[2 * lc + 1] = (unsigned int)__double2loint(v.y);
             }
-            // |c_u| = (h:lo) with h < 2^20; every width is >= 16 (NC <= 3),
-            // so after two shifts the remainder is a 32-bit value
+            // window slot k holds product t-k: (wh[k]:wl[k]) its remaining
+            // magnitude, ws[k] its sign (+1 / -1); wh is 0 once a product has
+            // been shifted by one width (its magnitude is below 2^36)
+            unsigned int wh[NC + 1], wl[NC + 1];
+            int ws[NC + 1];
 #pragma unroll
-            for (int u = 0; u < 2 * NL; ++u) {
-                unsigned int h = hw[u] & 0xFFFFFu, lo = rem[u][0];
+            for (int k = 0; k <= NC; ++k) wh[k] = wl[k] = 0u, ws[k] = 1;
+            int hprev = 0, h0 = 0;
+            int dlo = 0;
 #pragma unroll
-                for (int k = 1; k <= NC; ++k) {
-                    if (u + k - 1 < 2 * NL) {
-                        const int wk = J.q + (int)wide[u + k - 1];
-                        lo = __builtin_amdgcn_alignbit(h, lo, wk);
-                        h >>= wk;
-                        rem[u][k] = lo;
+            for (int t = 2 * H - 1 - NC; t < 2 * NL; ++t) {
+                if (t > 2 * H - 1 - NC) {
+                    // pass digit t-1: every product in the window moves down
+                    // by its width; the oldest leaves
+                    const int wp = J.q + (int)((hw[t - 1] >> 30) & 1u);
+#pragma unroll
+                    for (int k = NC; k >= 1; --k) {
+                        wl[k] = __builtin_amdgcn_alignbit(wh[k - 1], wl[k - 1], wp);
+                        wh[k] = wh[k - 1] >> wp;
+                        ws[k] = ws[k - 1];
                     }
                 }
-            }
-            // digit t from the fields of products t, t-1, .., t-NC; returns the
-            // balanced digit and the reference's carry (sum + half) >> W for
-            // sum >= half, -((half - sum) >> W) for sum < -half, else 0
-            auto restore = [&](int t, int* carry) {
-                const unsigned int M = wide[t] ? mq1 : mq;
-                const int W = J.q + (int)wide[t];
+                wh[0] = hw[t] & 0xFFFFFu;
+                wl[0] = lw[t];
+                ws[0] = ((int)hw[t] >> 31) | 1;
+                if (t < 2 * H - 1) continue;  // the window fills below the first digit restored
+                const bool wide = (hw[t] >> 30) & 1u;
+                const unsigned int M = wide ? mq1 : mq;
+                const int W = J.q + (int)wide;
                 int sum = 0;
 #pragma unroll
-                for (int k = 0; k <= NC; ++k) sum += __mul24(sg[t - k], (int)(rem[t - k][k] & M));
+                for (int k = 0; k <= NC; ++k) sum += __mul24(ws[k], (int)(wl[k] & M));
+                // the reference's carry: (sum + half) >> W for sum >= half,
+                // -((half - sum) >> W) for sum < -half, else 0
                 const int half = (int)((M + 1u) >> 1);
                 const int h = (sum + half + (sum >> 31) + (sum == -half ? 1 : 0)) >> W;
-                *carry = h;
-                return sum - (h << W);
-            };
-            int hprev;
-            (void)restore(2 * H - 1, &hprev);
-#pragma unroll
-            for (int lo = 0; lo < K; ++lo) {
-                const int lc = H + lo;
-                int h0, h1;
-                const int d0 = restore(2 * lc, &h0);
-                const int d1 = restore(2 * lc + 1, &h1);
-                const int2 xv = make_int2(d0 + hprev, d1 + h0);
-                hprev = h1;
-                if (out_digits) {
-                    gstore2i(J.digits, (size_t)(c0 + lo) + (size_t)B * r, xv);
+                const int d = sum - (h << W);
+                if (t == 2 * H - 1) {
+                    hprev = h;  // carry into the first own element
+                } else if ((t & 1) == 0) {
+                    dlo = d;
+                    h0 = h;
                 } else {
-                    lds[lc * RS + r] = weighted(lc, r, xv);
+                    const int lc = t >> 1, lo = lc - H;
+                    const int2 xv = make_int2(dlo + hprev, d + h0);
+                    hprev = h;
+                    if (out_digits) {
+                        gstore2i(J.digits, (size_t)(c0 + lo) + (size_t)B * r, xv);
+                    } else {
+                        lds[lc * RS + r] = weighted(lc, r, xv);
+                    }
                 }
             }
         }
         __syncthreads();
         LINE_STAMP(lucas_line_stamps_col, stamp, 3);
-        if (out_digits) {
-            wave_max_err(err, seen, J.maxerr);
-            wave_max_bits(pmax, pmin, seen_b, seen_z, J.pbits);
-            return;
-        }
+        if (out_digits) return;
         if (w >= H) {
 #pragma unroll
             for (int e = 0; e < EA; ++e) x[e] = region[l + 64 * e];
@@ -670,8 +671,6 @@ __device__ __forceinline__ void line_col_body(const LucasJobDesc& J, int tile, i
         gstore2_nt(J.spec, (size_t)k * B + c0 + lo, lds[(H + lo) * RS + F::out_pos(k)], J.nt);
     }
     LINE_STAMP(lucas_line_stamps_col, stamp, 5);
-    wave_max_err(err, seen, J.maxerr);
-    if (!in_digits) wave_max_bits(pmax, pmin, seen_b, seen_z, J.pbits);
 }
 
 // Column tile of workgroup `wg` of a single-job launch of n workgroups:
