@@ -37,10 +37,10 @@ inline constexpr int lucas_line_col_words(int EA, int K, int NC) {
     const int A = 64 * EA, NL = K + lucas_line_halo(NC);
     return NL * (A + 1) + A + 64 + A + (A + 3) / 4 + 2 * NL + (2 * NL + 3) / 4;
 }
-// LDS of a row workgroup (one row pair, two waves): two row regions, the row
-// transform's twiddles omega_B^(l*k1) [k1][l] and omega_64^t, and the outer
-// twiddle factors rt2 of both rows.
-inline constexpr int lucas_line_row_words(int EB) { return 3 * 64 * EB + 64 + 2 * EB; }
+// LDS of a row workgroup (one row pair, two waves): two row regions and the
+// outer twiddle factors rt2 of both rows (the transform's twiddles are held
+// in registers).
+inline constexpr int lucas_line_row_words(int EB) { return 2 * 64 * EB + 2 * EB; }
 
 // Index of an instantiation in the lists above, or -1.
 inline int lucas_line_col_index(int EA, int K, int NC) {

@@ -122,23 +122,19 @@ struct TwLds {
 };
 template <int E>
 struct TwReg {
-    static constexpr int NB = E >= 8 ? E / 8 : 1;
     double2 a1[E > 1 ? E - 1 : 1];
-    double2 a2[NB][7];
+    // omega_64^(a*c) depends on a = lane mod 8 only (the same for every
+    // butterfly of a lane)
+    double2 a2[7];
     // tab: omega_L^(n*k1) at [k1*64 + n], omega_64^(a*c) at [64E + c*8 + a]
     __device__ __forceinline__ void load(const double2* tab, int l) {
 #pragma unroll
         for (int k = 1; k < E; ++k) a1[k - 1] = gload2(tab, k * 64 + l);
 #pragma unroll
-        for (int i = 0; i < NB; ++i) {
-            const int beta = l + 64 * i;
-            const int a = (8 * E >= 64 || beta < 8 * E) ? (beta & 7) : 0;
-#pragma unroll
-            for (int c = 1; c < 8; ++c) a2[i][c - 1] = gload2(tab, 64 * E + c * 8 + a);
-        }
+        for (int c = 1; c < 8; ++c) a2[c - 1] = gload2(tab, 64 * E + c * 8 + (l & 7));
     }
     __device__ __forceinline__ double2 t1(int k) const { return a1[k - 1]; }
-    __device__ __forceinline__ double2 t2(int i, int c, int) const { return a2[i][c - 1]; }
+    __device__ __forceinline__ double2 t2(int, int c, int) const { return a2[c - 1]; }
 };
 
 // Transforms of one line of length L = 64E held by one wave.
@@ -293,9 +289,7 @@ __device__ __forceinline__ void line_row_body(const LucasJobDesc& J, int wg) {
     using F = Line<EB>;
     constexpr int B = 64 * EB, NA = F::NA, NB = F::NB;
     double2* const lds = lucas_dynamic_lds();
-    double2* const T1 = lds + 2 * B;
-    double2* const T2 = T1 + B;
-    double2* const T2R = T2 + 64;  // rt2 rows of the two rows, [wave][e]
+    double2* const T2R = lds + 2 * B;  // rt2 rows of the two rows, [wave][e]
     const int A = J.A;
     const int tid = threadIdx.x, l = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -307,18 +301,10 @@ __device__ __forceinline__ void line_row_body(const LucasJobDesc& J, int wg) {
     // table loads first: the load counter drains in issue order, so waiting
     // for them never waits for the row
     LINE_STAMP(lucas_line_stamps_row, true, 0);
-    // the line twiddles: in registers for short rows, else an LDS copy
-    constexpr bool REG = EB <= 8;
-    constexpr int NTAB = REG ? 0 : B + 64, NTJ = (NTAB + 127) / 128;
-    double2 tv[NTJ > 0 ? NTJ : 1];
-#pragma unroll
-    for (int j = 0; j < NTJ; ++j) {
-        const int t = tid + 128 * j;
-        if (t < NTAB) tv[j] = gload2(J.lnB, t);
-    }
+    // the line twiddles, in registers (a row workgroup has two waves per CU
+    // slot, so registers are plentiful; measured faster than an LDS copy)
     TwReg<EB> twr;
-    if constexpr (REG) twr.load(J.lnB, l);
-    const TwLds twl{T1, T2, l};
+    twr.load(J.lnB, l);
     double2 x[F::NX];
 #pragma unroll
     for (int e = 0; e < EB; ++e) x[e] = gload2_nt(J.spec, (size_t)R * B + l + 64 * e, J.nt);
@@ -336,18 +322,12 @@ __device__ __forceinline__ void line_row_body(const LucasJobDesc& J, int wg) {
         const int beta = l + 64 * i;
         wbv[i] = gload2(J.twB, (F::ALL || beta < NA) ? (beta >> 3) + EB * (beta & 7) : 0);
     }
-#pragma unroll
-    for (int j = 0; j < NTJ; ++j) {
-        const int t = tid + 128 * j;
-        if (t < NTAB) T1[t] = tv[j];
-    }
     if (l < EB) T2R[w * EB + l] = t2v;
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < EB; ++e) x[e] = cmul(x[e], cmul(t1, T2R[w * EB + e]));
     LINE_STAMP(lucas_line_stamps_row, true, 1);
-    if constexpr (REG) F::fwd(x, mine, twr, l);
-    else F::fwd(x, mine, twl, l);
+    F::fwd(x, mine, twr, l);
     wave_sync();
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
@@ -398,8 +378,7 @@ __device__ __forceinline__ void line_row_body(const LucasJobDesc& J, int wg) {
     }
     __syncthreads();  // every partner read is done before the regions are reused
     LINE_STAMP(lucas_line_stamps_row, true, 3);
-    if constexpr (REG) F::inv(x, mine, twr, l);
-    else F::inv(x, mine, twl, l);
+    F::inv(x, mine, twr, l);
     LINE_STAMP(lucas_line_stamps_row, true, 4);
 #pragma unroll
     for (int e = 0; e < EB; ++e) {
