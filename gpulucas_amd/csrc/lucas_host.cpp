@@ -228,7 +228,7 @@ std::vector<Plan> line_candidates(int p, int N) {
             c.A = A;
             c.B = B;
             c.C = K;
-            c.threads = 64 * (K + pl.halo);
+            c.threads = 64 * lucas_line_col_waves(K, pl.ncarry);
             c.col_wgs = B / K;
             c.row_wgs = A / 2;
             c.line = 1;

@@ -28,6 +28,10 @@
 // reads the products of the n+1 digits before them.
 inline constexpr int lucas_line_halo(int nc) { return (nc + 2) / 2; }
 
+// Waves of a column workgroup: one per local column, rounded up to a
+// multiple of the CU's 4 SIMDs (the extra waves take a share of the row-wise
+// carry and of the tile loads and stores, so every SIMD carries the same).
+inline constexpr int lucas_line_col_waves(int K, int NC) { return (K + lucas_line_halo(NC) + 3) / 4 * 4; }
 // LDS of a column workgroup, in 16-byte words: the NL = K + H column regions
 // (A + 1 words each, the pad keeping neighbouring columns in different bank
 // groups); the column transform's twiddles omega_A^(l*k1) [k1][l] and

@@ -400,7 +400,8 @@ template <int EA, int K, int NC>
 __device__ __forceinline__ void line_col_body(const LucasJobDesc& J, int tile, int mode) {
     static_assert(NC >= 2 && NC <= 3, "the row restoration assumes widths of at least 16 bits");
     using F = Line<EA>;
-    constexpr int A = 64 * EA, H = lucas_line_halo(NC), NL = K + H, T = 64 * NL, RS = A + 1;
+    constexpr int A = 64 * EA, H = lucas_line_halo(NC), NL = K + H, RS = A + 1;
+    constexpr int NW = lucas_line_col_waves(K, NC), T = 64 * NW;  // waves NL.. are helpers
     constexpr int NA = F::NA, NB = F::NB;
     double2* const lds = lucas_dynamic_lds();
     double2* const T1 = lds + NL * RS;
@@ -419,7 +420,8 @@ __device__ __forceinline__ void line_col_body(const LucasJobDesc& J, int tile, i
         const int g = c0 - H + lc;
         return g < 0 ? g + B : g;
     };
-    double2* const region = lds + w * RS;
+    const bool colw = w < NL;  // this wave holds a local column (else a helper)
+    double2* const region = lds + (colw ? w : 0) * RS;
     const unsigned int seen = __hip_atomic_load(J.maxerr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned int seen_b = __hip_atomic_load(J.pbits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned int seen_z = __hip_atomic_load(J.pbits + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -451,13 +453,16 @@ __device__ __forceinline__ void line_col_body(const LucasJobDesc& J, int tile, i
             ti[j] = gload(J.rlo, jj);
         }
     }
-    double2 tile_v[EA];
+    constexpr int NTILE = (A * NL + T - 1) / T;
+    double2 tile_v[NTILE];
     if (!in_digits) {
 #pragma unroll
-        for (int i = 0; i < EA; ++i) {
+        for (int i = 0; i < NTILE; ++i) {
             const int t = tid + T * i;
-            const int r = t / NL, lc = t - r * NL;
-            tile_v[i] = gload2_nt(J.spec2, (size_t)r * B + gcol(lc), J.nt);
+            if (t < A * NL) {
+                const int r = t / NL, lc = t - r * NL;
+                tile_v[i] = gload2_nt(J.spec2, (size_t)r * B + gcol(lc), J.nt);
+            }
         }
     }
 #pragma unroll
@@ -475,10 +480,12 @@ __device__ __forceinline__ void line_col_body(const LucasJobDesc& J, int tile, i
     }
     if (!in_digits) {
 #pragma unroll
-        for (int i = 0; i < EA; ++i) {
+        for (int i = 0; i < NTILE; ++i) {
             const int t = tid + T * i;
-            const int r = t / NL, lc = t - r * NL;
-            lds[lc * RS + F::out_pos(r)] = tile_v[i];
+            if (t < A * NL) {
+                const int r = t / NL, lc = t - r * NL;
+                lds[lc * RS + F::out_pos(r)] = tile_v[i];
+            }
         }
     }
     __syncthreads();
@@ -498,51 +505,54 @@ __device__ __forceinline__ void line_col_body(const LucasJobDesc& J, int tile, i
 
     if (in_digits) {
         // the own columns' digits, weighted, straight into the forward transform
-        if (w >= H) {
+        if (w >= H && colw) {
             const int gc = c0 + w - H;
 #pragma unroll
             for (int e = 0; e < EA; ++e) x[e] = weighted(w, l + 64 * e, gload2i(J.digits, (size_t)gc + (size_t)B * (l + 64 * e)));
         }
     } else {
         // inverse column transform of every local column
+        if (colw) {
 #pragma unroll
-        for (int i = 0; i < NB; ++i) {
-            const int beta = l + 64 * i;
-            if (F::ALL || beta < NA) {
+            for (int i = 0; i < NB; ++i) {
+                const int beta = l + 64 * i;
+                if (F::ALL || beta < NA) {
 #pragma unroll
-                for (int d = 0; d < 8; ++d) x[8 * i + d] = region[d * NA + beta];
+                    for (int d = 0; d < 8; ++d) x[8 * i + d] = region[d * NA + beta];
+                }
             }
-        }
-        if constexpr (REG) F::inv(x, region, twr, l);
-        else F::inv(x, region, twl, l);
-        // unweight, -2 on digit 0, round half-even; the rounded products go
-        // back as round_pack words (natural row order)
-        const int gc = gcol(w);
-        const int rle = locr[2 * w], rlo = locr[2 * w + 1];
-        const double ile = locw[2 * w].y, ilo = locw[2 * w + 1].y;
-        wave_sync();
+            if constexpr (REG) F::inv(x, region, twr, l);
+            else F::inv(x, region, twl, l);
+            // unweight, -2 on digit 0, round half-even; the rounded products
+            // go back as round_pack words (natural row order)
+            const int gc = gcol(w);
+            const int rle = locr[2 * w], rlo = locr[2 * w + 1];
+            const double ile = locw[2 * w].y, ilo = locw[2 * w + 1].y;
+            wave_sync();
 #pragma unroll
-        for (int e = 0; e < EA; ++e) {
-            const int r = l + 64 * e;
-            const int rh = rowr[r];
-            const double iwh = roww[r].y;
-            const int rje = rle + rh, rjo = rlo + rh;
-            double ve = x[e].x * (ile * iwh * (rje >= N ? 2.0 : 1.0));
-            const double vo = x[e].y * (ilo * iwh * (rjo >= N ? 2.0 : 1.0));
-            if (gc == 0 && r == 0) ve -= 2.0;
-            float ee, eo;
-            double me, mo;
-            const double pe = round_pack(ve, (rje >= N ? rje - N : rje) < J.b, &ee, &me);
-            const double po = round_pack(vo, (rjo >= N ? rjo - N : rjo) < J.b, &eo, &mo);
-            err = fmaxf(err, fmaxf(ee, eo));
-            pmax = fmax(pmax, fmax(me, mo));
-            pmin = fmin(pmin, fmin(me, mo));
-            region[r] = make_double2(pe, po);
+            for (int e = 0; e < EA; ++e) {
+                const int r = l + 64 * e;
+                const int rh = rowr[r];
+                const double iwh = roww[r].y;
+                const int rje = rle + rh, rjo = rlo + rh;
+                double ve = x[e].x * (ile * iwh * (rje >= N ? 2.0 : 1.0));
+                const double vo = x[e].y * (ilo * iwh * (rjo >= N ? 2.0 : 1.0));
+                if (gc == 0 && r == 0) ve -= 2.0;
+                float ee, eo;
+                double me, mo;
+                const double pe = round_pack(ve, (rje >= N ? rje - N : rje) < J.b, &ee, &me);
+                const double po = round_pack(vo, (rjo >= N ? rjo - N : rjo) < J.b, &eo, &mo);
+                err = fmaxf(err, fmaxf(ee, eo));
+                pmax = fmax(pmax, fmax(me, mo));
+                pmin = fmin(pmin, fmin(me, mo));
+                region[r] = make_double2(pe, po);
+            }
+            // the maxima are reduced here, not at the end: held to the end
+            // they would keep every element's intermediates live through the
+            // carry
+            wave_max_err(err, seen, J.maxerr);
+            wave_max_bits(pmax, pmin, seen_b, seen_z, J.pbits);
         }
-        // the maxima are reduced here, not at the end: held to the end they
-        // would keep every element's intermediates live through the carry
-        wave_max_err(err, seen, J.maxerr);
-        wave_max_bits(pmax, pmin, seen_b, seen_z, J.pbits);
         __syncthreads();
         LINE_STAMP(lucas_line_stamps_col, stamp, 2);
         // Carry-save restoration + single ripple x_j = d_j + h_{j-1}
@@ -623,13 +633,13 @@ __device__ __forceinline__ void line_col_body(const LucasJobDesc& J, int tile, i
         __syncthreads();
         LINE_STAMP(lucas_line_stamps_col, stamp, 3);
         if (out_digits) return;
-        if (w >= H) {
+        if (w >= H && colw) {
 #pragma unroll
             for (int e = 0; e < EA; ++e) x[e] = region[l + 64 * e];
         }
     }
     // forward column transform of the own columns, outputs to the region
-    if (w >= H) {
+    if (w >= H && colw) {
         if constexpr (REG) F::fwd(x, region, twr, l);
         else F::fwd(x, region, twl, l);
         wave_sync();
@@ -662,13 +672,13 @@ __device__ __forceinline__ int line_col_tile(int wg, int n) {
 }  // namespace
 
 #define LUCAS_LINE_COL_KERNELS(EA, K, NC)                                                                        \
-    extern "C" __global__ void __launch_bounds__(64 * (K + lucas_line_halo(NC)))                                 \
+    extern "C" __global__ void __launch_bounds__(64 * lucas_line_col_waves(K, NC))                              \
             lucas_lcol_##EA##_##K##_##NC(const LucasJobDesc* __restrict__ jobs, const int2* __restrict__ wgmap, \
                                          int mode) {                                                             \
         const int2 jm = wgmap[blockIdx.x];                                                                       \
         line_col_body<EA, K, NC>(jobs[jm.x], jm.y, mode);                                                        \
     }                                                                                                            \
-    extern "C" __global__ void __launch_bounds__(64 * (K + lucas_line_halo(NC)))                                 \
+    extern "C" __global__ void __launch_bounds__(64 * lucas_line_col_waves(K, NC))                              \
             lucas_lcol1_##EA##_##K##_##NC(const LucasJobDesc J, int mode) {                                      \
         line_col_body<EA, K, NC>(J, line_col_tile(blockIdx.x, J.col_wgs), mode);                                 \
     }
@@ -703,7 +713,7 @@ struct LineRow {
     int lds_words;
 };
 #define LUCAS_LINE_COL_ENTRY(EA, K, NC) \
-    {lucas_lcol_##EA##_##K##_##NC, lucas_lcol1_##EA##_##K##_##NC, 64 * (K + lucas_line_halo(NC)), \
+    {lucas_lcol_##EA##_##K##_##NC, lucas_lcol1_##EA##_##K##_##NC, 64 * lucas_line_col_waves(K, NC), \
      lucas_line_col_words(EA, K, NC)},
 #define LUCAS_LINE_ROW_ENTRY(EB) {lucas_lrow_##EB, lucas_lrow1_##EB, lucas_line_row_words(EB)},
 const LineCol kLineCols[] = {LUCAS_LINE_COLS(LUCAS_LINE_COL_ENTRY)};
