@@ -36,19 +36,22 @@ def log(msg):
 def kernel_times(ctx, iters):
     """Mean duration of each kernel in the steady-state iteration.
 
-    The iteration time comes from `iters` graph-launched iterations (graphs
-    built beforehand), the same launches a step makes.  HIP events recorded
-    after every launch of the row/col sequence on the engine's stream give
-    each kernel's share of the iteration; the markers lengthen every
-    interval, so only their ratio is used: col = t_iter * col_i / (col_i +
-    row_i).  Returns (col_us, row_us, t_iter_us, raw)."""
+    The iteration time t_iter comes from `iters` graph-launched iterations
+    (graphs built beforehand), the same launches a step makes.  HIP events
+    recorded after every launch of the row/col sequence on the engine's
+    stream give each kernel's interval; every interval also holds one
+    marker's own cost m, the same for both, so m = (col_i + row_i - t_iter)/2
+    and each kernel is its interval less m.  Neither figure depends on
+    --steps.  Returns (col_us, row_us, t_iter_us, raw)."""
     t_iter = 1e3 * ctx.time_iterations(iters)
     col_i, row_i = ctx.time_kernels(200)
-    share = col_i / (col_i + row_i)
+    m = max(0.0, (col_i + row_i - t_iter) / 2)
+    col, row = col_i - m, row_i - m
     raw = {"iteration_us": round(t_iter, 3), "col_interval_us": round(col_i, 3),
-           "row_interval_us": round(row_i, 3), "graph_iterations": iters,
-           "method": "graph-launched iteration time split by the ratio of event-timed launch intervals"}
-    return t_iter * share, t_iter * (1 - share), t_iter, raw
+           "row_interval_us": round(row_i, 3), "marker_us": round(m, 3), "graph_iterations": iters,
+           "method": "graph-launched iteration time; event-timed launch intervals less the per-marker cost "
+                     "(col_i + row_i - t_iter) / 2"}
+    return col, row, t_iter, raw
 
 
 def headline(lucas, steps, warmup):
@@ -127,7 +130,7 @@ def measured_traffic(info):
             k, v = line[len("traffic:"):].split("=", 1)
             vals[k.strip()] = v.strip()
     shape = (f"A={info['A']} B={info['B']} C={info['C']} T={info['threads']} R={info['radix_pref']} "
-             f"W={info['wave_A']}/{info['wave_B']}")
+             f"W={info['wave_A']}/{info['wave_B']} L={info['line']}")
     if vals.get("plan") != shape:
         return None
     return {k: int(v) for k, v in vals.items() if k != "plan"}

@@ -313,8 +313,16 @@ __device__ __forceinline__ void line_row_body(const LucasJobDesc& J, int wg) {
     // rt1[R][l] * rt2[R][e] (the rt2 row goes to LDS for the store), and the
     // square's omega_M^K = omega_M^R * omega_B^(k1 + EB*c) * omega_8^d.
     const double2 t1 = gload2(J.rt1, (size_t)R * 64 + l);
+    // short rows keep the whole factor row (wave-uniform) in registers
+    constexpr bool T2REG = EB <= 8;
+    double2 t2r[T2REG ? EB : 1];
     double2 t2v = make_double2(0.0, 0.0);
-    if (l < EB) t2v = gload2(J.rt2, (size_t)R * EB + l);
+    if constexpr (T2REG) {
+#pragma unroll
+        for (int e = 0; e < EB; ++e) t2r[e] = gload2(J.rt2, (size_t)R * EB + e);
+    } else if (l < EB) {
+        t2v = gload2(J.rt2, (size_t)R * EB + l);
+    }
     const double2 wR = gload2(J.twMA, R);
     double2 wbv[NB];
 #pragma unroll
@@ -322,10 +330,17 @@ __device__ __forceinline__ void line_row_body(const LucasJobDesc& J, int wg) {
         const int beta = l + 64 * i;
         wbv[i] = gload2(J.twB, (F::ALL || beta < NA) ? (beta >> 3) + EB * (beta & 7) : 0);
     }
-    if (l < EB) T2R[w * EB + l] = t2v;
+    if (!T2REG && l < EB) T2R[w * EB + l] = t2v;
     __syncthreads();
 #pragma unroll
-    for (int e = 0; e < EB; ++e) x[e] = cmul(x[e], cmul(t1, T2R[w * EB + e]));
+    for (int e = 0; e < EB; ++e) {
+        if constexpr (T2REG) {
+            t2r[e] = cmul(t1, t2r[e]);  // kept for the store
+            x[e] = cmul(x[e], t2r[e]);
+        } else {
+            x[e] = cmul(x[e], cmul(t1, T2R[w * EB + e]));
+        }
+    }
     LINE_STAMP(lucas_line_stamps_row, true, 1);
     F::fwd(x, mine, twr, l);
     wave_sync();
@@ -382,7 +397,7 @@ __device__ __forceinline__ void line_row_body(const LucasJobDesc& J, int wg) {
     LINE_STAMP(lucas_line_stamps_row, true, 4);
 #pragma unroll
     for (int e = 0; e < EB; ++e) {
-        const double2 tw = cmul(t1, T2R[w * EB + e]);
+        const double2 tw = T2REG ? t2r[e] : cmul(t1, T2R[w * EB + e]);
         gstore2_nt(J.spec2, (size_t)R * B + l + 64 * e, cmulc(x[e], tw), J.nt);
     }
     LINE_STAMP(lucas_line_stamps_row, true, 5);
@@ -547,11 +562,14 @@ __device__ __forceinline__ void line_col_body(const LucasJobDesc& J, int tile, i
                 pmin = fmin(pmin, fmin(me, mo));
                 region[r] = make_double2(pe, po);
             }
-            // the maxima are reduced here, not at the end: held to the end
-            // they would keep every element's intermediates live through the
-            // carry
-            wave_max_err(err, seen, J.maxerr);
-            wave_max_bits(pmax, pmin, seen_b, seen_z, J.pbits);
+            // long columns reduce the maxima here, not at the end: held to
+            // the end they would keep every element's intermediates live
+            // through the carry; short ones defer the reduction off the
+            // critical path
+            if constexpr (!REG) {
+                wave_max_err(err, seen, J.maxerr);
+                wave_max_bits(pmax, pmin, seen_b, seen_z, J.pbits);
+            }
         }
         __syncthreads();
         LINE_STAMP(lucas_line_stamps_col, stamp, 2);
@@ -632,7 +650,13 @@ __device__ __forceinline__ void line_col_body(const LucasJobDesc& J, int tile, i
         }
         __syncthreads();
         LINE_STAMP(lucas_line_stamps_col, stamp, 3);
-        if (out_digits) return;
+        if (out_digits) {
+            if (REG && colw) {
+                wave_max_err(err, seen, J.maxerr);
+                wave_max_bits(pmax, pmin, seen_b, seen_z, J.pbits);
+            }
+            return;
+        }
         if (w >= H && colw) {
 #pragma unroll
             for (int e = 0; e < EA; ++e) x[e] = region[l + 64 * e];
@@ -660,6 +684,10 @@ __device__ __forceinline__ void line_col_body(const LucasJobDesc& J, int tile, i
         gstore2_nt(J.spec, (size_t)k * B + c0 + lo, lds[(H + lo) * RS + F::out_pos(k)], J.nt);
     }
     LINE_STAMP(lucas_line_stamps_col, stamp, 5);
+    if (REG && colw && !in_digits) {
+        wave_max_err(err, seen, J.maxerr);
+        wave_max_bits(pmax, pmin, seen_b, seen_z, J.pbits);
+    }
 }
 
 // Column tile of workgroup `wg` of a single-job launch of n workgroups:
