@@ -14,6 +14,10 @@
 #ifndef LUCAS_GLOBAL
 #define LUCAS_GLOBAL __attribute__((address_space(1)))
 #endif
+// Register budget of a kernel: compiled to fit n waves per SIMD.
+#ifndef LUCAS_WAVES_PER_SIMD
+#define LUCAS_WAVES_PER_SIMD(n) __attribute__((amdgpu_waves_per_eu(n)))
+#endif
 typedef double lucas_v2d __attribute__((vector_size(16)));
 typedef int lucas_v2i __attribute__((vector_size(8)));
 template <typename T>

@@ -219,36 +219,39 @@ std::vector<Plan> line_candidates(int p, int N) {
         const int A = 64 * EA;
         if (pl.M % A || (pl.M / A) % 64) continue;
         const int B = pl.M / A, EB = B / 64;
-        const int ir = lucas_line_row_index(EB);
-        if (ir < 0) continue;
-        for (int K = 1; K <= 16; ++K) {
-            const int ic = lucas_line_col_index(EA, K, pl.ncarry);
-            if (ic < 0 || B % K) continue;
-            Plan c = pl;
-            c.A = A;
-            c.B = B;
-            c.C = K;
-            c.threads = 64 * lucas_line_col_waves(K, pl.ncarry);
-            c.col_wgs = B / K;
-            c.row_wgs = A / 2;
-            c.line = 1;
-            c.line_col = ic;
-            c.line_row = ir;
-            c.lds_col = 16 * (size_t)lucas_line_col_words(EA, K, pl.ncarry);
-            c.lds_row = 16 * (size_t)lucas_line_row_words(EB);
-            if (c.lds_col > kMaxLds || c.lds_row > kMaxLds) continue;
-            out.push_back(c);
+        for (int S = 1; S <= 2; ++S) {
+            const int ir = lucas_line_row_index(EB, S);
+            if (ir < 0) continue;
+            for (int K = 1; K <= 16; ++K) {
+                const int ic = lucas_line_col_index(EA, K, pl.ncarry);
+                if (ic < 0 || B % K) continue;
+                Plan c = pl;
+                c.A = A;
+                c.B = B;
+                c.C = K;
+                c.threads = 64 * lucas_line_col_waves(K, pl.ncarry);
+                c.col_wgs = B / K;
+                c.row_wgs = A / 2;
+                c.line = 1;
+                c.line_col = ic;
+                c.line_row = ir;
+                c.line_split = S - 1;
+                c.lds_col = 16 * (size_t)lucas_line_col_words(EA, K, pl.ncarry);
+                c.lds_row = 16 * (size_t)lucas_line_row_words(EB);
+                if (c.lds_col > kMaxLds || c.lds_row > kMaxLds) continue;
+                out.push_back(c);
+            }
         }
     }
     return out;
 }
 
-std::string make_line_plan(int p, int N, int A, int K, Plan* out, int nt) {
+std::string make_line_plan(int p, int N, int A, int K, Plan* out, int nt, int split) {
     Plan pl;
     std::string why = base_plan(p, N, &pl);
     if (!why.empty()) return why;
     for (const Plan& c : line_candidates(p, N)) {
-        if ((A && c.A != A) || (K && c.C != K)) continue;
+        if ((A && c.A != A) || (K && c.C != K) || (split >= 0 && c.line_split != split)) continue;
         *out = c;
         out->nt = nt ? 1 : 0;
         return "";
@@ -258,7 +261,8 @@ std::string make_line_plan(int p, int N, int A, int K, Plan* out, int nt) {
 
 // Measured fastest tilings per signal length (gpulucas_amd/tune.py); line: 1
 // for the line kernels (A rows, C own columns per column workgroup), 2 for
-// the line kernels with non-temporal spectrum loads and stores.
+// the line kernels with non-temporal spectrum loads and stores, 3 and 4 the
+// same with rows split over two waves.
 struct TunedShape {
     int N, A, C, threads, pref, wA, wB, line;
 };
@@ -290,7 +294,8 @@ std::string make_plan(int p, int N, int forceA, int forceC, Plan* out, bool shar
             if (t->N != N) continue;
             if (t->line) {
                 // a single exponent's latency plan (batches keep the tiled kernels)
-                if (!shared && make_line_plan(p, N, t->A, t->C, out, t->line == 2).empty()) return "";
+                if (!shared && make_line_plan(p, N, t->A, t->C, out, (t->line - 1) & 1, (t->line - 1) >> 1).empty())
+                    return "";
                 continue;
             }
             if (shared) {
@@ -588,7 +593,7 @@ void make_tables(const Plan& pl, Tables* t) {
             for (int u = 0; u < 64; ++u) put(L + u, (long long)(u & 7) * (u >> 3), 64);
         };
         line_tw(pl.A, &t->lnA);
-        line_tw(pl.B, &t->lnB);
+        line_tw(pl.B >> pl.line_split, &t->lnB);  // split rows: the half-length transform
     }
     root_table(pl.A, pl.A, &t->twA);
     root_table(pl.B, pl.B, &t->twB);

@@ -31,6 +31,7 @@ struct Plan {
     // column workgroup's own columns and threads its work-items.
     int line = 0, line_col = -1, line_row = -1;
     int nt = 0;  // line kernels: spectra moved with non-temporal loads and stores
+    int line_split = 0;  // line kernels: each row held by two waves (a radix-2 step joins them)
     std::vector<int> radA, radB;  // Stockham radices of the remaining factors A/wA, B/wB
     size_t lds_col = 0, lds_row = 0;  // bytes of LDS per workgroup
 };
@@ -67,7 +68,8 @@ std::string make_plan_exact(int p, int N, int A, int C, int threads, int pref, P
 // workgroup (A = 0 or K = 0: the first instantiated choice); "" on success.
 // `nt`: spectra moved with non-temporal loads and stores (faster once the
 // state outgrows the caches).
-std::string make_line_plan(int p, int N, int A, int K, Plan* out, int nt = 0);
+// split: 0 one wave per row, 1 two, -1 either (the first candidate).
+std::string make_line_plan(int p, int N, int A, int K, Plan* out, int nt = 0, int split = -1);
 // Every line-kernel plan of (p, N).
 std::vector<Plan> line_candidates(int p, int N);
 // Default N for p: smallest admissible length under the bits-per-word rule.

@@ -21,8 +21,10 @@
     X(7, 2, 2) X(7, 2, 3) X(7, 4, 2) X(7, 4, 3)                                                             \
     X(8, 2, 2) X(8, 2, 3) X(8, 4, 2) X(8, 4, 3)                                                             \
     X(16, 2, 2) X(16, 2, 3) X(16, 4, 2) X(16, 4, 3)
-// X(EB)
-#define LUCAS_LINE_ROWS(X) X(1) X(2) X(3) X(4) X(5) X(7) X(8) X(16)
+// X(EB, S): rows of B = 64*EB words, S waves per row (S = 2: each wave holds
+// one half, EB/2 words per lane, and a radix-2 step joins the halves)
+#define LUCAS_LINE_ROWS(X) \
+    X(1, 1) X(2, 1) X(3, 1) X(4, 1) X(5, 1) X(7, 1) X(8, 1) X(16, 1) X(8, 2) X(16, 2) X(32, 2)
 
 // Halo columns of carry depth n: the restoration of a column's first digits
 // reads the products of the n+1 digits before them.
@@ -41,10 +43,19 @@ inline constexpr int lucas_line_col_words(int EA, int K, int NC) {
     const int A = 64 * EA, NL = K + lucas_line_halo(NC);
     return NL * (A + 1) + A + 64 + A + (A + 3) / 4 + 2 * NL + (2 * NL + 3) / 4;
 }
-// LDS of a row workgroup (one row pair, two waves): two row regions and the
-// outer twiddle factors rt2 of both rows (the transform's twiddles are held
-// in registers).
+// LDS of a row workgroup (one row pair, 2S waves): one region per wave (the
+// two rows) and the outer twiddle factors rt2 of both rows (the transform's
+// twiddles are held in registers).
 inline constexpr int lucas_line_row_words(int EB) { return 2 * 64 * EB + 2 * EB; }
+// Work-items of a row workgroup: S waves for each of the pair's two rows.
+inline constexpr int lucas_line_row_threads(int S) { return 128 * S; }
+
+// Waves per SIMD a column kernel is compiled for (its register budget): two
+// workgroups per CU where the LDS holds two, so that one workgroup's loads
+// and stores overlap the other's transforms; else one.
+inline constexpr int lucas_line_col_waves_per_simd(int EA, int K, int NC) {
+    return (2 * 16 * lucas_line_col_words(EA, K, NC) <= 160 * 1024 ? 2 : 1) * lucas_line_col_waves(K, NC) / 4;
+}
 
 // Index of an instantiation in the lists above, or -1.
 inline int lucas_line_col_index(int EA, int K, int NC) {
@@ -57,11 +68,11 @@ inline int lucas_line_col_index(int EA, int K, int NC) {
 #undef LUCAS_LINE_X
     return -1;
 }
-inline int lucas_line_row_index(int EB) {
+inline int lucas_line_row_index(int EB, int S) {
     int i = 0;
-#define LUCAS_LINE_X(b) \
-    if (EB == (b))      \
-        return i;       \
+#define LUCAS_LINE_X(b, s)          \
+    if (EB == (b) && S == (s)) \
+        return i;                 \
     ++i;
     LUCAS_LINE_ROWS(LUCAS_LINE_X)
 #undef LUCAS_LINE_X

@@ -3,7 +3,8 @@
 // mersenneIter, gpuLucas.cu:845-942) as two launches, each one HBM round trip
 // of the 8N-byte state, with every transform held by single waves:
 //
-//   row kernel    : one row pair (k, A-k) per workgroup, one row per wave:
+//   row kernel    : one row pair (k, A-k) per workgroup, one row per wave
+//                   (long rows: two waves per row, joined by a radix-2 step):
 //                   outer twiddle, forward row transform, real-signal square
 //                   (ComplexPointwiseSqr, gpuLucas.cu:647-665), inverse row
 //                   transform, inverse outer twiddle.
@@ -404,6 +405,175 @@ __device__ __forceinline__ void line_row_body(const LucasJobDesc& J, int wg) {
 }
 
 // ---------------------------------------------------------------------------
+// Row kernel, split rows: each row of the pair is held by two waves, wave
+// (r, h) holding elements h*B/2 + n (n < B/2) of row r, so a long row has
+// twice the waves of the one-wave-per-row kernel.  A radix-2 step joins the
+// halves through LDS: the even frequencies kk = 2k are the half-length
+// transform of y0[n] = x[n] + x[n + B/2], the odd ones kk = 2k + 1 that of
+// y1[n] = (x[n] - x[n + B/2]) omega_B^n, and wave h transforms y_h.  The
+// square's partner of kk = 2k + h, column B-1-kk of the partner row, is
+// 2(B/2-1-k) + (1-h): the other half of the partner row at the reversed
+// position.  Row 0 pairs kk with -kk mod B, in the same half: k with
+// -k mod B/2 for h = 0, with B/2-1-k for h = 1.  The inverse runs the steps
+// backwards: half-length inverses u_h, then z[n] = u0 + u1 conj(omega_B^n)
+// and z[n + B/2] = u0 - u1 conj(omega_B^n).
+// ---------------------------------------------------------------------------
+// omega_64^t = exp(-2*pi*i*t/64) for a compile-time t (after unrolling): the
+// quarter-wave cosines cos(2*pi*j/64), j = 0..16, and the quadrant symmetries.
+__device__ constexpr double kCos64[17] = {1.0, 0.9951847266721969, 0.9807852804032304, 0.9569403357322088,
+                                          0.9238795325112867, 0.881921264348355, 0.8314696123025452,
+                                          0.773010453362737, 0.7071067811865476, 0.6343932841636455,
+                                          0.5555702330196022, 0.47139673682599764, 0.3826834323650898,
+                                          0.2902846772544624, 0.19509032201612828, 0.0980171403295606, 0.0};
+__device__ __forceinline__ double2 omega64(int t) {
+    const int q = (t >> 4) & 3, r = t & 15;
+    const double c = kCos64[r], sn = kCos64[16 - r];  // cos, sin of 2*pi*r/64
+    if (q == 0) return make_double2(c, -sn);
+    if (q == 1) return make_double2(-sn, -c);
+    if (q == 2) return make_double2(-c, sn);
+    return make_double2(sn, c);
+}
+
+template <int EB>
+__device__ __forceinline__ void line_row_split_body(const LucasJobDesc& J, int wg) {
+    constexpr int EH = EB / 2;
+    using F = Line<EH>;
+    constexpr int B = 64 * EB, LH = 64 * EH, NA = F::NA, NB = F::NB;
+    double2* const lds = lucas_dynamic_lds();
+    double2* const T2R = lds + 2 * B;  // each wave's half of its row's rt2 row, [wave][e]
+    const int A = J.A;
+    const int tid = threadIdx.x, l = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int h = w & 1;
+    const bool self = wg == 0;
+    const int R = self ? ((w >> 1) ? A / 2 : 0) : ((w >> 1) ? A - wg : wg);
+    double2* const mine = lds + w * LH;
+    double2* const sib = lds + (w ^ 1) * LH;  // the other half of this row
+    // the square's partners: the other half of the partner row (w ^ 3), of
+    // this row for row A/2, this half for row 0
+    double2* const part = self ? (R == 0 ? mine : sib) : lds + (w ^ 3) * LH;
+    const bool neg = self && R == 0 && h == 0;  // partner -k mod B/2
+
+    LINE_STAMP(lucas_line_stamps_row, true, 0);
+    TwReg<EH> twr;
+    twr.load(J.lnB, l);
+    double2 x[F::NX];
+    const size_t base = (size_t)R * B + (size_t)h * LH;
+#pragma unroll
+    for (int e = 0; e < EH; ++e) x[e] = gload2_nt(J.spec, base + l + 64 * e, J.nt);
+    // outer twiddle omega_M^(R*c), c = h*B/2 + l + 64e: rt1[R][l] * rt2[R][h*EH + e]
+    const double2 t1 = gload2(J.rt1, (size_t)R * 64 + l);
+    constexpr bool T2REG = EH <= 8;
+    double2 t2r[T2REG ? EH : 1];
+    double2 t2v = make_double2(0.0, 0.0);
+    if constexpr (T2REG) {
+#pragma unroll
+        for (int e = 0; e < EH; ++e) t2r[e] = gload2(J.rt2, (size_t)R * EB + h * EH + e);
+    } else if (l < EH) {
+        t2v = gload2(J.rt2, (size_t)R * EB + h * EH + l);
+    }
+    // the radix-2 step's omega_B^n, n = l + 64e: omega_B^l * omega_B^(64e),
+    // the second factor omega_64^(e*64/EB) a constant
+    static_assert(64 % EB == 0, "split rows: EB divides 64");
+    const double2 wl = gload2(J.twB, l);
+    const double2 wR = gload2(J.twMA, R);
+    double2 wbv[NB];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        const int beta = l + 64 * i;
+        wbv[i] = gload2(J.twB, (F::ALL || beta < NA) ? h + 2 * ((beta >> 3) + EH * (beta & 7)) : 0);
+    }
+    if (!T2REG && l < EH) T2R[w * EH + l] = t2v;
+    wave_sync();
+#pragma unroll
+    for (int e = 0; e < EH; ++e) {
+        if constexpr (T2REG) {
+            t2r[e] = cmul(t1, t2r[e]);  // kept for the store
+            x[e] = cmul(x[e], t2r[e]);
+        } else {
+            x[e] = cmul(x[e], cmul(t1, T2R[w * EH + e]));
+        }
+        mine[l + 64 * e] = x[e];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < EH; ++e) {
+        const double2 o = sib[l + 64 * e];
+        if (h == 0) x[e] = cadd(x[e], o);
+        else x[e] = cmul(csub(o, x[e]), cmul(wl, omega64(e * (64 / EB))));
+    }
+    __syncthreads();  // the sibling's reads are done before the region is reused
+    LINE_STAMP(lucas_line_stamps_row, true, 1);
+    F::fwd(x, mine, twr, l);
+    wave_sync();
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        const int beta = l + 64 * i;
+        if (F::ALL || beta < NA) {
+#pragma unroll
+            for (int d = 0; d < 8; ++d) mine[d * NA + beta] = x[8 * i + d];
+        }
+    }
+    __syncthreads();
+    LINE_STAMP(lucas_line_stamps_row, true, 2);
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        const int beta = l + 64 * i;
+        if (F::ALL || beta < NA) {
+            // omega_B^kk = omega_B^(h + 2(k1 + EH*c)) * omega_8^d
+            const double2 wb = cmul(wR, wbv[i]);
+#pragma unroll
+            for (int d = 0; d < 8; ++d) {
+                const int k = F::out_index(beta, d);
+                const int pos = neg ? F::out_pos(k == 0 ? 0 : LH - k) : LH - 1 - (d * NA + beta);
+                const double2 zk = x[8 * i + d], zm = part[pos];
+                double2 wk;
+                switch (d) {
+                    case 0: wk = mul_w8<0>(wb); break;
+                    case 1: wk = mul_w8<1>(wb); break;
+                    case 2: wk = mul_w8<2>(wb); break;
+                    case 3: wk = mul_w8<3>(wb); break;
+                    case 4: wk = mul_w8<4>(wb); break;
+                    case 5: wk = mul_w8<5>(wb); break;
+                    case 6: wk = mul_w8<6>(wb); break;
+                    default: wk = mul_w8<7>(wb); break;
+                }
+                const double2 Ev = cadd(zk, cconj(zm));
+                const double2 Ov = mul_si<-1>(csub(zk, cconj(zm)));
+                const double2 P = cadd(cmul(Ev, Ev), cmul(wk, cmul(Ov, Ov)));
+                const double2 Q = cmul(Ev, Ov);
+                x[8 * i + d] = make_double2(P.x - 2.0 * Q.y, P.y + 2.0 * Q.x);
+            }
+        }
+    }
+    __syncthreads();  // every partner read is done before the regions are reused
+    LINE_STAMP(lucas_line_stamps_row, true, 3);
+    F::inv(x, mine, twr, l);
+    wave_sync();
+#pragma unroll
+    for (int e = 0; e < EH; ++e) {
+        if (h) x[e] = cmulc(x[e], cmul(wl, omega64(e * (64 / EB))));
+        mine[l + 64 * e] = x[e];
+    }
+    __syncthreads();
+    LINE_STAMP(lucas_line_stamps_row, true, 4);
+#pragma unroll
+    for (int e = 0; e < EH; ++e) {
+        const double2 o = sib[l + 64 * e];
+        const double2 z = h ? csub(o, x[e]) : cadd(x[e], o);
+        const double2 tw = T2REG ? t2r[e] : cmul(t1, T2R[w * EH + e]);
+        gstore2_nt(J.spec2, base + l + 64 * e, cmulc(z, tw), J.nt);
+    }
+    LINE_STAMP(lucas_line_stamps_row, true, 5);
+}
+
+template <int EB, int S>
+__device__ __forceinline__ void line_row_any(const LucasJobDesc& J, int wg) {
+    if constexpr (S == 1) line_row_body<EB>(J, wg);
+    else line_row_split_body<EB>(J, wg);
+}
+
+// ---------------------------------------------------------------------------
 // Column kernel: workgroup `tile` owns columns c0 = K*tile .. c0+K-1; wave w
 // holds local column w, global column c0 - H + w (mod B).  Local columns
 // 0..H-1 are the halo: the restoration of digit j reads the rounded products
@@ -701,25 +871,28 @@ __device__ __forceinline__ int line_col_tile(int wg, int n) {
 
 #define LUCAS_LINE_COL_KERNELS(EA, K, NC)                                                                        \
     extern "C" __global__ void __launch_bounds__(64 * lucas_line_col_waves(K, NC))                              \
+            LUCAS_WAVES_PER_SIMD(lucas_line_col_waves_per_simd(EA, K, NC))                                        \
             lucas_lcol_##EA##_##K##_##NC(const LucasJobDesc* __restrict__ jobs, const int2* __restrict__ wgmap, \
                                          int mode) {                                                             \
         const int2 jm = wgmap[blockIdx.x];                                                                       \
         line_col_body<EA, K, NC>(jobs[jm.x], jm.y, mode);                                                        \
     }                                                                                                            \
     extern "C" __global__ void __launch_bounds__(64 * lucas_line_col_waves(K, NC))                              \
+            LUCAS_WAVES_PER_SIMD(lucas_line_col_waves_per_simd(EA, K, NC))                                        \
             lucas_lcol1_##EA##_##K##_##NC(const LucasJobDesc J, int mode) {                                      \
         line_col_body<EA, K, NC>(J, line_col_tile(blockIdx.x, J.col_wgs), mode);                                 \
     }
 LUCAS_LINE_COLS(LUCAS_LINE_COL_KERNELS)
 
-#define LUCAS_LINE_ROW_KERNELS(EB)                                                                          \
-    extern "C" __global__ void __launch_bounds__(128)                                                       \
-            lucas_lrow_##EB(const LucasJobDesc* __restrict__ jobs, const int2* __restrict__ wgmap) {       \
-        const int2 jm = wgmap[blockIdx.x];                                                                  \
-        line_row_body<EB>(jobs[jm.x], jm.y);                                                                \
-    }                                                                                                       \
-    extern "C" __global__ void __launch_bounds__(128) lucas_lrow1_##EB(const LucasJobDesc J) {              \
-        line_row_body<EB>(J, blockIdx.x);                                                                   \
+#define LUCAS_LINE_ROW_KERNELS(EB, S)                                                                         \
+    extern "C" __global__ void __launch_bounds__(lucas_line_row_threads(S))                                   \
+            lucas_lrow_##EB##_##S(const LucasJobDesc* __restrict__ jobs, const int2* __restrict__ wgmap) {   \
+        const int2 jm = wgmap[blockIdx.x];                                                                    \
+        line_row_any<EB, S>(jobs[jm.x], jm.y);                                                                \
+    }                                                                                                         \
+    extern "C" __global__ void __launch_bounds__(lucas_line_row_threads(S))                                   \
+            lucas_lrow1_##EB##_##S(const LucasJobDesc J) {                                                    \
+        line_row_any<EB, S>(J, blockIdx.x);                                                                   \
     }
 LUCAS_LINE_ROWS(LUCAS_LINE_ROW_KERNELS)
 
@@ -738,12 +911,13 @@ struct LineCol {
 struct LineRow {
     LineRowKernel many;
     LineRow1Kernel one;
-    int lds_words;
+    int threads, lds_words;
 };
 #define LUCAS_LINE_COL_ENTRY(EA, K, NC) \
     {lucas_lcol_##EA##_##K##_##NC, lucas_lcol1_##EA##_##K##_##NC, 64 * lucas_line_col_waves(K, NC), \
      lucas_line_col_words(EA, K, NC)},
-#define LUCAS_LINE_ROW_ENTRY(EB) {lucas_lrow_##EB, lucas_lrow1_##EB, lucas_line_row_words(EB)},
+#define LUCAS_LINE_ROW_ENTRY(EB, S) \
+    {lucas_lrow_##EB##_##S, lucas_lrow1_##EB##_##S, lucas_line_row_threads(S), lucas_line_row_words(EB)},
 const LineCol kLineCols[] = {LUCAS_LINE_COLS(LUCAS_LINE_COL_ENTRY)};
 const LineRow kLineRows[] = {LUCAS_LINE_ROWS(LUCAS_LINE_ROW_ENTRY)};
 }  // namespace
@@ -798,9 +972,9 @@ hipError_t launch_line_row(int idx, const LucasJobDesc* jobs, const int2* map, c
                            hipStream_t stream) {
     const LineRow& k = kLineRows[idx];
     if (one)
-        hipLaunchKernelGGL(k.one, dim3(nwg), dim3(128), 16 * (size_t)k.lds_words, stream, *one);
+        hipLaunchKernelGGL(k.one, dim3(nwg), dim3(k.threads), 16 * (size_t)k.lds_words, stream, *one);
     else
-        hipLaunchKernelGGL(k.many, dim3(nwg), dim3(128), 16 * (size_t)k.lds_words, stream, jobs, map);
+        hipLaunchKernelGGL(k.many, dim3(nwg), dim3(k.threads), 16 * (size_t)k.lds_words, stream, jobs, map);
     return hipGetLastError();
 }
 

@@ -2,6 +2,7 @@
 
     python -m gpulucas_amd.phases --case 2976221:163840 --case 37156667:2097152
     python -m gpulucas_amd.phases --case 2976221:163840:256:2   # line kernels, A = 256, K = 2
+    python -m gpulucas_amd.phases --case 37156667:2097152:512:4:4  # ... line kind 4 (split rows, non-temporal)
 
 Loads libgpulucas_phases.so (the kernels with their phase stamps compiled
 in), runs `--iters` iterations of each exponent at its planned shape and
@@ -48,12 +49,13 @@ def summarize(stamps: np.ndarray, names: list[str]) -> dict:
     }
 
 
-def profile(p: int, N: int, iters: int = 20, line: tuple[int, int] | None = None) -> dict:
-    """Per-phase times at the planned shape, or with `line` = (A, K) at the
-    line kernels' shape with A rows and K own columns per column workgroup."""
+def profile(p: int, N: int, iters: int = 20, line: tuple[int, ...] | None = None) -> dict:
+    """Per-phase times at the planned shape, or with `line` = (A, K[, kind])
+    at the line kernels' shape with A rows and K own columns per column
+    workgroup (kind as lucas_plan_info.line, default 1)."""
     from gpulucas_amd import lucas
 
-    plan = {"p": p, "N": N, "A": line[0], "C": line[1], "line": 1} if line else None
+    plan = {"p": p, "N": N, "A": line[0], "C": line[1], "line": line[2] if len(line) > 2 else 1} if line else None
     with lucas.Context(p, N, plan=plan) as ctx:
         info = ctx.info
         ctx.step(64)
@@ -81,7 +83,7 @@ def _table(r: dict) -> str:
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--case", action="append", default=[],
-                    help="p:N, or p:N:A:K for the line kernels (repeatable)")
+                    help="p:N, or p:N:A:K[:kind] for the line kernels (repeatable)")
     ap.add_argument("--iters", type=int, default=20)
     ap.add_argument("--json", action="store_true")
     a = ap.parse_args(argv)
@@ -96,8 +98,8 @@ def main(argv=None) -> int:
     for c in cases:
         f = [int(x) for x in c.split(":")]
         p, N = f[0], f[1]
-        line = (f[2], f[3]) if len(f) == 4 else None
-        print(f"case p={p} N={N}" + (f" line A={line[0]} K={line[1]}" if line else ""), file=sys.stderr, flush=True)
+        line = tuple(f[2:]) if len(f) >= 4 else None
+        print(f"case p={p} N={N}" + (f" line {line}" if line else ""), file=sys.stderr, flush=True)
         r = profile(p, N, a.iters, line)
         print(json.dumps(r) if a.json else _table(r), flush=True)
     return 0

@@ -14,7 +14,8 @@ before its cost model (rebuild the library afterwards).
 A length may hold two entries: the tiling of the general kernels (which
 batches always use) and, where the line kernels measured faster for a single
 exponent, a line entry (last field 1, or 2 with non-temporal spectrum loads
-and stores), which single exponents then use.
+and stores; 3 and 4 the same with every row split over two waves), which
+single exponents then use.
 """
 from __future__ import annotations
 
@@ -162,15 +163,19 @@ def measure_line(pairs, iters, out, margin=0.02):
         with lucas.Context(p, N, plan=plan) as ctx:
             base = timed(ctx)
         best = None
+        every = []
         for c in cands:
-            for kind in (1, 2):  # 2: non-temporal spectrum loads and stores
+            # c["line"]: 1 one wave per row, 3 two; +1 non-temporal spectrum loads and stores
+            for kind in (c["line"], c["line"] + 1):
                 with lucas.Context(p, N, plan={"p": p, "N": N, "A": c["A"], "C": c["C"], "line": kind}) as ctx:
                     us = timed(ctx)
+                every.append({"A": c["A"], "C": c["C"], "line": kind, "us": round(us, 3)})
                 if best is None or us < best[0]:
                     best = (us, c, kind)
         us, c, kind = best
         results[str(N)] = {"p": p, "N": N, "tiled_us": base, "line_us": us, "A": c["A"], "C": c["C"],
-                           "threads": c["threads"], "line": kind, "wins": us < base * (1 - margin)}
+                           "threads": c["threads"], "line": kind, "wins": us < base * (1 - margin),
+                           "every": every}
         print(f"N={N} p={p}: tiled {base:.2f} us, line{kind} A={c['A']} K={c['C']} {us:.2f} us "
               f"({'line' if us < base * (1 - margin) else 'tiled'}, {time.time() - t0:.1f}s)", flush=True)
         with open(out, "w") as f:

@@ -21,6 +21,7 @@
 #define __launch_bounds__(...)
 #define __shared__
 #define LUCAS_GLOBAL
+#define LUCAS_WAVES_PER_SIMD(n)
 
 struct double2 { double x, y; };
 struct int2 { int x, y; };

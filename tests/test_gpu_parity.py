@@ -436,6 +436,11 @@ def test_prepare_rehearses_and_restores_the_state(gmp_fixtures):
     (9068767, 524288, 512, 2, 200, 1),   # E = 8
     (37156667, 2097152, 1024, 4, 100, 2),  # E = 16, carry depth 3, N = 2^21, non-temporal spectra
     (37156667, 2097152, 1024, 2, 100, 1),
+    (37156667, 2097152, 1024, 4, 100, 4),  # rows split over two waves, non-temporal spectra
+    (37156667, 2097152, 512, 4, 100, 3),   # rows of 2048 split over two waves
+    (18984731, 1048576, 512, 4, 100, 4),
+    (9068767, 524288, 512, 2, 100, 3),     # rows of 512 split: two 256-word halves
+    (9068767, 524288, 128, 2, 100, 3),     # rows of 2048 split, E = 2 columns
 ])
 def test_line_kernels_equal_tiled_kernels(p, N, A, K, iters, kind):
     """The line kernels and the tiled kernels reach the same digits after
@@ -455,13 +460,17 @@ def test_line_kernels_equal_tiled_kernels(p, N, A, K, iters, kind):
         assert 0.0 < ctx.max_roundoff() < ROUNDOFF_LIMIT
 
 
-@pytest.mark.parametrize("p,N,A,K", [(141731, 8192, 64, 2), (2267549, 131072, 1024, 4), (599657, 32768, 256, 1)])
-def test_line_kernels_vs_oracle_steady_state(p, N, A, K):
+@pytest.mark.parametrize("p,N,A,K,kind", [(141731, 8192, 64, 2, 1), (2267549, 131072, 1024, 4, 1),
+                                          (599657, 32768, 256, 1, 1), (2398633, 131072, 64, 2, 3),
+                                          (4535099, 262144, 64, 2, 4)])
+def test_line_kernels_vs_oracle_steady_state(p, N, A, K, kind):
     """Line kernels from random digits: one and three iterations against the
-    oracle's restatement (carry depth 3, E = 16 columns, one own column)."""
-    print(f"case p={p} N={N} line A={A} K={K}", flush=True)
+    oracle's restatement (carry depth 3, E = 16 columns, one own column,
+    rows of 1024 and 2048 split over two waves)."""
+    print(f"case p={p} N={N} line{kind} A={A} K={K}", flush=True)
     x = random_balanced(p, N, 17)
-    with lucas.Context(p, N, plan={"p": p, "N": N, "A": A, "C": K, "line": 1}) as ctx:
+    with lucas.Context(p, N, plan={"p": p, "N": N, "A": A, "C": K, "line": kind}) as ctx:
+        assert ctx.info["line"] == kind
         ctx.set_digits(x, 0)
         ctx.step(1)
         y, _ = oracle.ibdwt_iterate(p, N, x, 1)

@@ -136,29 +136,32 @@ def test_emulated_product_bits_match_exact_convolution(p, N, A, C, iters):
 
 # Line kernels: every row and column length instantiated (64E, E = 1, 2, 3,
 # 4, 5, 7, 8, 16), carry depths 2 and 3, K = 1, 2, 4 own columns, the halo
-# wrap of tile 0, one and several iterations, batched and single-job forms.
-@pytest.mark.parametrize("p,N,A,K,iters,single", [
-    (149921, 8192, 64, 2, 3, False),     # A = B = 64 (E = 1): several iterations from random digits
-    (141731, 8192, 64, 2, 1, True),      # carry depth 3 (q = 17)
-    (299843, 16384, 128, 2, 1, False),   # columns E = 2, rows E = 2
-    (449741, 24576, 192, 2, 1, True),    # columns E = 3 (radix 3), rows E = 2
-    (449741, 24576, 64, 4, 1, False),    # rows E = 3
-    (599657, 32768, 256, 1, 1, True),    # columns E = 4, one own column per workgroup
-    (749587, 40960, 320, 2, 2, False),   # columns E = 5 (radix 5)
-    (1049413, 57344, 448, 4, 1, True),   # columns E = 7 (radix 7)
-    (1199309, 65536, 512, 4, 1, False),  # columns E = 8 (lanes hold one radix-8 butterfly each)
-    (2267549, 131072, 1024, 4, 1, True),   # columns E = 16 (two butterflies per lane), depth 3
-    (2998279, 163840, 256, 2, 1, True),    # rows E = 5 (the headline shape)
-    (2098801, 114688, 128, 4, 1, False),   # rows E = 7
-    (2398633, 131072, 128, 4, 1, True),    # rows E = 8
-    (4797241, 262144, 128, 4, 1, True),    # rows E = 16
+# wrap of tile 0, one and several iterations, batched and single-job forms,
+# rows of 1024 and 2048 words split over two waves.
+@pytest.mark.parametrize("p,N,A,K,iters,single,split", [
+    (149921, 8192, 64, 2, 3, False, 0),     # A = B = 64 (E = 1): several iterations from random digits
+    (141731, 8192, 64, 2, 1, True, 0),      # carry depth 3 (q = 17)
+    (299843, 16384, 128, 2, 1, False, 0),   # columns E = 2, rows E = 2
+    (449741, 24576, 192, 2, 1, True, 0),    # columns E = 3 (radix 3), rows E = 2
+    (449741, 24576, 64, 4, 1, False, 0),    # rows E = 3
+    (599657, 32768, 256, 1, 1, True, 0),    # columns E = 4, one own column per workgroup
+    (749587, 40960, 320, 2, 2, False, 0),   # columns E = 5 (radix 5)
+    (1049413, 57344, 448, 4, 1, True, 0),   # columns E = 7 (radix 7)
+    (1199309, 65536, 512, 4, 1, False, 0),  # columns E = 8 (lanes hold one radix-8 butterfly each)
+    (2267549, 131072, 1024, 4, 1, True, 0),   # columns E = 16 (two butterflies per lane), depth 3
+    (2998279, 163840, 256, 2, 1, True, 0),    # rows E = 5 (the headline shape)
+    (2098801, 114688, 128, 4, 1, False, 0),   # rows E = 7
+    (2398633, 131072, 128, 4, 1, True, 0),    # rows E = 8
+    (4797241, 262144, 128, 4, 1, True, 0),    # rows E = 16
+    (2398633, 131072, 64, 2, 1, True, 1),   # rows of 1024 split: two 512-word halves (register twiddles)
+    (4535099, 262144, 64, 2, 1, False, 1),  # rows of 2048 split: two 1024-word halves, carry depth 3
 ])
-def test_emulated_line_kernels_match_oracle(p, N, A, K, iters, single):
+def test_emulated_line_kernels_match_oracle(p, N, A, K, iters, single, split):
     w, _, _, _ = oracle.tables(p, N)
     rng = np.random.default_rng(p % 1000)
     half = (1 << (w - 1)).astype(np.int64)
     x = (rng.integers(0, 2 * half) - half).astype(np.int32)
-    got, err = emu.run(p, N, x, iters, single=single, line=(A, K))
+    got, err = emu.run(p, N, x, iters, single=single, line=(A, K, split))
     want, _ = oracle.ibdwt_iterate(p, N, x, iters)
     assert np.array_equal(got, want)
     assert 0.0 < err < 0.1
