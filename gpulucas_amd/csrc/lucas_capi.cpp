@@ -58,7 +58,7 @@ void fill_info(const Plan& pl, lucas_plan_info* out) {
     out->wave_A = pl.wA;
     out->wave_B = pl.wB;
     out->col_dense = pl.col_dense;
-    out->line = pl.line ? 1 + pl.nt + 2 * pl.line_split : 0;
+    out->line = line_kind(pl);
     out->device_bytes = job_bytes(pl);
 }
 
@@ -373,9 +373,7 @@ int lucas_ctx_create_plan(const lucas_plan_info* plan, int device, lucas_ctx** o
     if (!plan || !out) return LUCAS_ERR_INVALID_ARG;
     Plan pl;
     if (plan->line) {
-        if (plan->line < 1 || plan->line > 4 ||
-            !make_line_plan(plan->p, plan->N, plan->A, plan->C, &pl, (plan->line - 1) & 1, (plan->line - 1) >> 1)
-                     .empty())
+        if (!make_line_plan_kind(plan->p, plan->N, plan->A, plan->C, plan->line, &pl).empty())
             return LUCAS_ERR_INVALID_ARG;
         return ctx_new(pl, device, out);
     }

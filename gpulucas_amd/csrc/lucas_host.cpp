@@ -222,20 +222,22 @@ std::vector<Plan> line_candidates(int p, int N) {
         for (int S = 1; S <= 2; ++S) {
             const int ir = lucas_line_row_index(EB, S);
             if (ir < 0) continue;
+            for (int CS = 1; CS <= 2; ++CS)
             for (int K = 1; K <= 16; ++K) {
-                const int ic = lucas_line_col_index(EA, K, pl.ncarry);
+                const int ic = lucas_line_col_index(EA, K, pl.ncarry, CS);
                 if (ic < 0 || B % K) continue;
                 Plan c = pl;
                 c.A = A;
                 c.B = B;
                 c.C = K;
-                c.threads = 64 * lucas_line_col_waves(K, pl.ncarry);
+                c.threads = 64 * lucas_line_col_waves(K, pl.ncarry, CS);
                 c.col_wgs = B / K;
                 c.row_wgs = A / 2;
                 c.line = 1;
                 c.line_col = ic;
                 c.line_row = ir;
                 c.line_split = S - 1;
+                c.line_col_split = CS - 1;
                 c.lds_col = 16 * (size_t)lucas_line_col_words(EA, K, pl.ncarry);
                 c.lds_row = 16 * (size_t)lucas_line_row_words(EB);
                 if (c.lds_col > kMaxLds || c.lds_row > kMaxLds) continue;
@@ -246,12 +248,14 @@ std::vector<Plan> line_candidates(int p, int N) {
     return out;
 }
 
-std::string make_line_plan(int p, int N, int A, int K, Plan* out, int nt, int split) {
+std::string make_line_plan(int p, int N, int A, int K, Plan* out, int nt, int split, int col_split) {
     Plan pl;
     std::string why = base_plan(p, N, &pl);
     if (!why.empty()) return why;
     for (const Plan& c : line_candidates(p, N)) {
-        if ((A && c.A != A) || (K && c.C != K) || (split >= 0 && c.line_split != split)) continue;
+        if ((A && c.A != A) || (K && c.C != K) || (split >= 0 && c.line_split != split) ||
+            (col_split >= 0 && c.line_col_split != col_split))
+            continue;
         *out = c;
         out->nt = nt ? 1 : 0;
         return "";
@@ -259,10 +263,19 @@ std::string make_line_plan(int p, int N, int A, int K, Plan* out, int nt, int sp
     return "no line-kernel shape for this N and carry depth";
 }
 
-// Measured fastest tilings per signal length (gpulucas_amd/tune.py); line: 1
-// for the line kernels (A rows, C own columns per column workgroup), 2 for
-// the line kernels with non-temporal spectrum loads and stores, 3 and 4 the
-// same with rows split over two waves.
+int line_kind(const Plan& pl) {
+    return pl.line ? 1 + pl.nt + 2 * pl.line_split + 4 * pl.line_col_split : 0;
+}
+
+std::string make_line_plan_kind(int p, int N, int A, int K, int kind, Plan* out) {
+    if (kind < 1 || kind > 8) return "line kind out of range";
+    return make_line_plan(p, N, A, K, out, (kind - 1) & 1, ((kind - 1) >> 1) & 1, (kind - 1) >> 2);
+}
+
+// Measured fastest tilings per signal length (gpulucas_amd/tune.py); line:
+// the line kind (line_kind: 1 for the line kernels with A rows and C own
+// columns per column workgroup, +1 with non-temporal spectrum loads and
+// stores, +2 with rows split over two waves, +4 with columns split).
 struct TunedShape {
     int N, A, C, threads, pref, wA, wB, line;
 };
@@ -294,8 +307,7 @@ std::string make_plan(int p, int N, int forceA, int forceC, Plan* out, bool shar
             if (t->N != N) continue;
             if (t->line) {
                 // a single exponent's latency plan (batches keep the tiled kernels)
-                if (!shared && make_line_plan(p, N, t->A, t->C, out, (t->line - 1) & 1, (t->line - 1) >> 1).empty())
-                    return "";
+                if (!shared && make_line_plan_kind(p, N, t->A, t->C, t->line, out).empty()) return "";
                 continue;
             }
             if (shared) {
@@ -592,7 +604,7 @@ void make_tables(const Plan& pl, Tables* t) {
             for (int t = 0; t < L; ++t) put(t, (long long)(t & 63) * (t >> 6), L);
             for (int u = 0; u < 64; ++u) put(L + u, (long long)(u & 7) * (u >> 3), 64);
         };
-        line_tw(pl.A, &t->lnA);
+        line_tw(pl.A >> pl.line_col_split, &t->lnA);  // split columns: the half-length transform
         line_tw(pl.B >> pl.line_split, &t->lnB);  // split rows: the half-length transform
     }
     root_table(pl.A, pl.A, &t->twA);

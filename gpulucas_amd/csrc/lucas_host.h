@@ -31,7 +31,8 @@ struct Plan {
     // column workgroup's own columns and threads its work-items.
     int line = 0, line_col = -1, line_row = -1;
     int nt = 0;  // line kernels: spectra moved with non-temporal loads and stores
-    int line_split = 0;  // line kernels: each row held by two waves (a radix-2 step joins them)
+    int line_split = 0;      // line kernels: each row held by two waves (a radix-2 step joins them)
+    int line_col_split = 0;  // line kernels: each column held by two waves
     std::vector<int> radA, radB;  // Stockham radices of the remaining factors A/wA, B/wB
     size_t lds_col = 0, lds_row = 0;  // bytes of LDS per workgroup
 };
@@ -68,8 +69,14 @@ std::string make_plan_exact(int p, int N, int A, int C, int threads, int pref, P
 // workgroup (A = 0 or K = 0: the first instantiated choice); "" on success.
 // `nt`: spectra moved with non-temporal loads and stores (faster once the
 // state outgrows the caches).
-// split: 0 one wave per row, 1 two, -1 either (the first candidate).
-std::string make_line_plan(int p, int N, int A, int K, Plan* out, int nt = 0, int split = -1);
+// split / col_split: 0 one wave per row / column, 1 two, -1 either (the
+// first candidate).
+std::string make_line_plan(int p, int N, int A, int K, Plan* out, int nt = 0, int split = -1, int col_split = -1);
+// The line kind of a plan (lucas_plan_info.line): 1 + non-temporal +
+// 2 * split rows + 4 * split columns; 0 for the tiled kernels.
+int line_kind(const Plan& pl);
+// Plan of line kind `kind` (1..8) with A rows and K own columns.
+std::string make_line_plan_kind(int p, int N, int A, int K, int kind, Plan* out);
 // Every line-kernel plan of (p, N).
 std::vector<Plan> line_candidates(int p, int N);
 // Default N for p: smallest admissible length under the bits-per-word rule.

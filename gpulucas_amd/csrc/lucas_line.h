@@ -11,16 +11,18 @@
 // H = ceil((NC+1)/2) columns -- and the row kernel per EB (B = 64*EB).
 #pragma once
 
-// X(EA, K, NC)
+// X(EA, K, NC, S): S waves per column (S = 2: each wave holds one half, EA/2
+// words per lane, and a radix-2 step joins the halves)
 #define LUCAS_LINE_COLS(X)                                                                                  \
-    X(1, 2, 2) X(1, 2, 3) X(1, 4, 2) X(1, 4, 3)                                                             \
-    X(2, 2, 2) X(2, 2, 3) X(2, 4, 2) X(2, 4, 3)                                                             \
-    X(3, 2, 2) X(3, 2, 3) X(3, 4, 2) X(3, 4, 3)                                                             \
-    X(4, 1, 2) X(4, 2, 2) X(4, 2, 3) X(4, 4, 2) X(4, 4, 3)                                                  \
-    X(5, 2, 2) X(5, 2, 3) X(5, 4, 2) X(5, 4, 3)                                                             \
-    X(7, 2, 2) X(7, 2, 3) X(7, 4, 2) X(7, 4, 3)                                                             \
-    X(8, 2, 2) X(8, 2, 3) X(8, 4, 2) X(8, 4, 3)                                                             \
-    X(16, 2, 2) X(16, 2, 3) X(16, 4, 2) X(16, 4, 3)
+    X(1, 2, 2, 1) X(1, 2, 3, 1) X(1, 4, 2, 1) X(1, 4, 3, 1)                                                 \
+    X(2, 2, 2, 1) X(2, 2, 3, 1) X(2, 4, 2, 1) X(2, 4, 3, 1)                                                 \
+    X(3, 2, 2, 1) X(3, 2, 3, 1) X(3, 4, 2, 1) X(3, 4, 3, 1)                                                 \
+    X(4, 1, 2, 1) X(4, 2, 2, 1) X(4, 2, 3, 1) X(4, 4, 2, 1) X(4, 4, 3, 1)                                   \
+    X(5, 2, 2, 1) X(5, 2, 3, 1) X(5, 4, 2, 1) X(5, 4, 3, 1)                                                 \
+    X(7, 2, 2, 1) X(7, 2, 3, 1) X(7, 4, 2, 1) X(7, 4, 3, 1)                                                 \
+    X(8, 2, 2, 1) X(8, 2, 3, 1) X(8, 4, 2, 1) X(8, 4, 3, 1)                                                 \
+    X(16, 2, 2, 1) X(16, 2, 3, 1) X(16, 4, 2, 1) X(16, 4, 3, 1)                                             \
+    X(16, 2, 2, 2) X(16, 2, 3, 2) X(16, 4, 2, 2) X(16, 4, 3, 2)
 // X(EB, S): rows of B = 64*EB words, S waves per row (S = 2: each wave holds
 // one half, EB/2 words per lane, and a radix-2 step joins the halves)
 #define LUCAS_LINE_ROWS(X) \
@@ -30,10 +32,12 @@
 // reads the products of the n+1 digits before them.
 inline constexpr int lucas_line_halo(int nc) { return (nc + 2) / 2; }
 
-// Waves of a column workgroup: one per local column, rounded up to a
-// multiple of the CU's 4 SIMDs (the extra waves take a share of the row-wise
-// carry and of the tile loads and stores, so every SIMD carries the same).
-inline constexpr int lucas_line_col_waves(int K, int NC) { return (K + lucas_line_halo(NC) + 3) / 4 * 4; }
+// Waves of a column workgroup: S per local column, rounded up to a multiple
+// of the CU's 4 SIMDs (the extra waves take a share of the row-wise carry
+// and of the tile loads and stores, so every SIMD carries the same).
+inline constexpr int lucas_line_col_waves(int K, int NC, int S) {
+    return (S * (K + lucas_line_halo(NC)) + 3) / 4 * 4;
+}
 // LDS of a column workgroup, in 16-byte words: the NL = K + H column regions
 // (A + 1 words each, the pad keeping neighbouring columns in different bank
 // groups); the column transform's twiddles omega_A^(l*k1) [k1][l] and
@@ -53,16 +57,16 @@ inline constexpr int lucas_line_row_threads(int S) { return 128 * S; }
 // Waves per SIMD a column kernel is compiled for (its register budget): two
 // workgroups per CU where the LDS holds two, so that one workgroup's loads
 // and stores overlap the other's transforms; else one.
-inline constexpr int lucas_line_col_waves_per_simd(int EA, int K, int NC) {
-    return (2 * 16 * lucas_line_col_words(EA, K, NC) <= 160 * 1024 ? 2 : 1) * lucas_line_col_waves(K, NC) / 4;
+inline constexpr int lucas_line_col_waves_per_simd(int EA, int K, int NC, int S) {
+    return (2 * 16 * lucas_line_col_words(EA, K, NC) <= 160 * 1024 ? 2 : 1) * lucas_line_col_waves(K, NC, S) / 4;
 }
 
 // Index of an instantiation in the lists above, or -1.
-inline int lucas_line_col_index(int EA, int K, int NC) {
+inline int lucas_line_col_index(int EA, int K, int NC, int S) {
     int i = 0;
-#define LUCAS_LINE_X(a, k, n)             \
-    if (EA == (a) && K == (k) && NC == (n)) \
-        return i;                         \
+#define LUCAS_LINE_X(a, k, n, s)                          \
+    if (EA == (a) && K == (k) && NC == (n) && S == (s)) \
+        return i;                                       \
     ++i;
     LUCAS_LINE_COLS(LUCAS_LINE_X)
 #undef LUCAS_LINE_X

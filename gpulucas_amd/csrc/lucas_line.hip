@@ -574,19 +574,29 @@ __device__ __forceinline__ void line_row_any(const LucasJobDesc& J, int wg) {
 }
 
 // ---------------------------------------------------------------------------
-// Column kernel: workgroup `tile` owns columns c0 = K*tile .. c0+K-1; wave w
-// holds local column w, global column c0 - H + w (mod B).  Local columns
-// 0..H-1 are the halo: the restoration of digit j reads the rounded products
-// of digits j-NC .. j, and a tile's first digits need the products of the
-// H columns to its left.  A halo column left of column 0 wraps to column
-// B - H + w of the previous row (the cyclic wrap of IrrBaseBalanced.cu:64-70).
+// Column kernel: workgroup `tile` owns columns c0 = K*tile .. c0+K-1; local
+// column lc is global column c0 - H + lc (mod B).  Local columns 0..H-1 are
+// the halo: the restoration of digit j reads the rounded products of digits
+// j-NC .. j, and a tile's first digits need the products of the H columns to
+// its left.  A halo column left of column 0 wraps to column B - H + lc of the
+// previous row (the cyclic wrap of IrrBaseBalanced.cu:64-70).
+// S = 1: wave w holds local column w.  S = 2: waves 2lc and 2lc+1 hold the
+// halves of local column lc (rows h*A/2 + n of wave h), joined by a radix-2
+// step as in the split row kernel: the forward transform of half h is that
+// of y0[n] = x[n] + x[n + A/2] (even frequencies) or
+// y1[n] = (x[n] - x[n + A/2]) omega_A^n (odd ones); the inverse takes the
+// half-length inverses u_h to z[n] = u0 + u1 conj(omega_A^n),
+// z[n + A/2] = u0 - u1 conj(omega_A^n).  Frequency r of a column sits in
+// half r & 1 at the half-line position of r >> 1.
 // ---------------------------------------------------------------------------
-template <int EA, int K, int NC>
+template <int EA, int K, int NC, int S>
 __device__ __forceinline__ void line_col_body(const LucasJobDesc& J, int tile, int mode) {
     static_assert(NC >= 2 && NC <= 3, "the row restoration assumes widths of at least 16 bits");
-    using F = Line<EA>;
-    constexpr int A = 64 * EA, H = lucas_line_halo(NC), NL = K + H, RS = A + 1;
-    constexpr int NW = lucas_line_col_waves(K, NC), T = 64 * NW;  // waves NL.. are helpers
+    static_assert(S == 1 || (S == 2 && EA % 2 == 0 && EA / 2 <= 8), "split columns: halves of at most 512 words");
+    constexpr int EL = EA / S;  // words per lane of a wave's line
+    using F = Line<EL>;
+    constexpr int A = 64 * EA, LH = 64 * EL, H = lucas_line_halo(NC), NL = K + H, RS = A + 1;
+    constexpr int NW = lucas_line_col_waves(K, NC, S), T = 64 * NW;  // waves S*NL.. are helpers
     constexpr int NA = F::NA, NB = F::NB;
     double2* const lds = lucas_dynamic_lds();
     double2* const T1 = lds + NL * RS;
@@ -605,8 +615,13 @@ __device__ __forceinline__ void line_col_body(const LucasJobDesc& J, int tile, i
         const int g = c0 - H + lc;
         return g < 0 ? g + B : g;
     };
-    const bool colw = w < NL;  // this wave holds a local column (else a helper)
-    double2* const region = lds + (colw ? w : 0) * RS;
+    const int lcw = w / S, h = w % S;  // this wave's local column and half
+    const bool colw = lcw < NL;        // this wave holds (half) a local column (else a helper)
+    const bool own = colw && lcw >= H;  // ... one of the tile's own columns
+    double2* const column = lds + (colw ? lcw : 0) * RS;
+    double2* const region = column + h * LH;  // the wave's line
+    // region position of frequency r of a column
+    auto spec_pos = [](int r) { return S == 1 ? F::out_pos(r) : (r & 1) * LH + F::out_pos(r >> 1); };
     const unsigned int seen = __hip_atomic_load(J.maxerr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned int seen_b = __hip_atomic_load(J.pbits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned int seen_z = __hip_atomic_load(J.pbits + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -615,12 +630,14 @@ __device__ __forceinline__ void line_col_body(const LucasJobDesc& J, int tile, i
     LINE_STAMP(lucas_line_stamps_col, stamp, 0);
     // Tables (T1, T2; per row; per local digit), loads first, then the tile.
     // the line twiddles: in registers for short columns, else an LDS copy
-    constexpr bool REG = EA <= 8;
+    constexpr bool REG = EL <= 8;
     constexpr int NTAB = 2 * A + 64 + 2 * NL, NTJ = (NTAB + T - 1) / T;
     double2 tv[NTJ];
     int ti[NTJ];
-    TwReg<EA> twr;
+    TwReg<EL> twr;
     if constexpr (REG) twr.load(J.lnA, l);
+    // split columns: omega_A^n, n = l + 64e, as omega_A^l * omega_64^(e*64/EA)
+    const double2 wa = S == 2 ? gload2(J.twA, l) : make_double2(1.0, 0.0);
     const TwLds twl{T1, T2, l};
 #pragma unroll
     for (int j = 0; j < NTJ; ++j) {
@@ -669,7 +686,7 @@ __device__ __forceinline__ void line_col_body(const LucasJobDesc& J, int tile, i
             const int t = tid + T * i;
             if (t < A * NL) {
                 const int r = t / NL, lc = t - r * NL;
-                lds[lc * RS + F::out_pos(r)] = tile_v[i];
+                lds[lc * RS + spec_pos(r)] = tile_v[i];
             }
         }
     }
@@ -690,10 +707,19 @@ __device__ __forceinline__ void line_col_body(const LucasJobDesc& J, int tile, i
 
     if (in_digits) {
         // the own columns' digits, weighted, straight into the forward transform
-        if (w >= H && colw) {
-            const int gc = c0 + w - H;
+        if (own) {
+            const int gc = c0 + lcw - H;
+            auto dig = [&](int r) { return weighted(lcw, r, gload2i(J.digits, (size_t)gc + (size_t)B * r)); };
 #pragma unroll
-            for (int e = 0; e < EA; ++e) x[e] = weighted(w, l + 64 * e, gload2i(J.digits, (size_t)gc + (size_t)B * (l + 64 * e)));
+            for (int e = 0; e < EL; ++e) {
+                const int n = l + 64 * e;
+                if constexpr (S == 1) {
+                    x[e] = dig(n);
+                } else {
+                    const double2 a = dig(n), b = dig(n + LH);
+                    x[e] = h ? cmul(csub(a, b), cmul(wa, omega64(e * (64 / EA)))) : cadd(a, b);
+                }
+            }
         }
     } else {
         // inverse column transform of every local column
@@ -708,15 +734,37 @@ __device__ __forceinline__ void line_col_body(const LucasJobDesc& J, int tile, i
             }
             if constexpr (REG) F::inv(x, region, twr, l);
             else F::inv(x, region, twl, l);
+            if constexpr (S == 2) {
+                wave_sync();
+#pragma unroll
+                for (int e = 0; e < EL; ++e) {
+                    if (h) x[e] = cmulc(x[e], cmul(wa, omega64(e * (64 / EA))));
+                    region[l + 64 * e] = x[e];
+                }
+            }
+        }
+        if constexpr (S == 2) {
+            // the radix-2 step: each half reads the other's u
+            __syncthreads();
+            if (colw) {
+#pragma unroll
+                for (int e = 0; e < EL; ++e) {
+                    const double2 o = column[(h ^ 1) * LH + l + 64 * e];
+                    x[e] = h ? csub(o, x[e]) : cadd(x[e], o);
+                }
+            }
+            __syncthreads();  // every read done before the products overwrite the column
+        }
+        if (colw) {
             // unweight, -2 on digit 0, round half-even; the rounded products
             // go back as round_pack words (natural row order)
-            const int gc = gcol(w);
-            const int rle = locr[2 * w], rlo = locr[2 * w + 1];
-            const double ile = locw[2 * w].y, ilo = locw[2 * w + 1].y;
+            const int gc = gcol(lcw);
+            const int rle = locr[2 * lcw], rlo = locr[2 * lcw + 1];
+            const double ile = locw[2 * lcw].y, ilo = locw[2 * lcw + 1].y;
             wave_sync();
 #pragma unroll
-            for (int e = 0; e < EA; ++e) {
-                const int r = l + 64 * e;
+            for (int e = 0; e < EL; ++e) {
+                const int r = h * LH + l + 64 * e;
                 const int rh = rowr[r];
                 const double iwh = roww[r].y;
                 const int rje = rle + rh, rjo = rlo + rh;
@@ -730,7 +778,7 @@ __device__ __forceinline__ void line_col_body(const LucasJobDesc& J, int tile, i
                 err = fmaxf(err, fmaxf(ee, eo));
                 pmax = fmax(pmax, fmax(me, mo));
                 pmin = fmin(pmin, fmin(me, mo));
-                region[r] = make_double2(pe, po);
+                column[r] = make_double2(pe, po);
             }
             // long columns reduce the maxima here, not at the end: held to
             // the end they would keep every element's intermediates live
@@ -827,13 +875,23 @@ __device__ __forceinline__ void line_col_body(const LucasJobDesc& J, int tile, i
             }
             return;
         }
-        if (w >= H && colw) {
+        if (own) {
 #pragma unroll
-            for (int e = 0; e < EA; ++e) x[e] = region[l + 64 * e];
+            for (int e = 0; e < EL; ++e) {
+                const int n = l + 64 * e;
+                if constexpr (S == 1) {
+                    x[e] = column[n];
+                } else {
+                    const double2 a = column[n], b = column[n + LH];
+                    x[e] = h ? cmul(csub(a, b), cmul(wa, omega64(e * (64 / EA)))) : cadd(a, b);
+                }
+            }
         }
     }
+    // both halves read before the transforms reuse the column
+    if constexpr (S == 2) __syncthreads();
     // forward column transform of the own columns, outputs to the region
-    if (w >= H && colw) {
+    if (own) {
         if constexpr (REG) F::fwd(x, region, twr, l);
         else F::fwd(x, region, twl, l);
         wave_sync();
@@ -851,7 +909,7 @@ __device__ __forceinline__ void line_col_body(const LucasJobDesc& J, int tile, i
     // store row-major: the outer twiddle is the row kernel's
     for (int t = tid; t < A * K; t += T) {
         const int k = t / K, lo = t - k * K;
-        gstore2_nt(J.spec, (size_t)k * B + c0 + lo, lds[(H + lo) * RS + F::out_pos(k)], J.nt);
+        gstore2_nt(J.spec, (size_t)k * B + c0 + lo, lds[(H + lo) * RS + spec_pos(k)], J.nt);
     }
     LINE_STAMP(lucas_line_stamps_col, stamp, 5);
     if (REG && colw && !in_digits) {
@@ -869,18 +927,18 @@ __device__ __forceinline__ int line_col_tile(int wg, int n) {
 
 }  // namespace
 
-#define LUCAS_LINE_COL_KERNELS(EA, K, NC)                                                                        \
-    extern "C" __global__ void __launch_bounds__(64 * lucas_line_col_waves(K, NC))                              \
-            LUCAS_WAVES_PER_SIMD(lucas_line_col_waves_per_simd(EA, K, NC))                                        \
-            lucas_lcol_##EA##_##K##_##NC(const LucasJobDesc* __restrict__ jobs, const int2* __restrict__ wgmap, \
-                                         int mode) {                                                             \
+#define LUCAS_LINE_COL_KERNELS(EA, K, NC, S)                                                                   \
+    extern "C" __global__ void __launch_bounds__(64 * lucas_line_col_waves(K, NC, S))                           \
+            LUCAS_WAVES_PER_SIMD(lucas_line_col_waves_per_simd(EA, K, NC, S))                                     \
+            lucas_lcol_##EA##_##K##_##NC##_##S(const LucasJobDesc* __restrict__ jobs,                           \
+                                               const int2* __restrict__ wgmap, int mode) {                       \
         const int2 jm = wgmap[blockIdx.x];                                                                       \
-        line_col_body<EA, K, NC>(jobs[jm.x], jm.y, mode);                                                        \
+        line_col_body<EA, K, NC, S>(jobs[jm.x], jm.y, mode);                                                     \
     }                                                                                                            \
-    extern "C" __global__ void __launch_bounds__(64 * lucas_line_col_waves(K, NC))                              \
-            LUCAS_WAVES_PER_SIMD(lucas_line_col_waves_per_simd(EA, K, NC))                                        \
-            lucas_lcol1_##EA##_##K##_##NC(const LucasJobDesc J, int mode) {                                      \
-        line_col_body<EA, K, NC>(J, line_col_tile(blockIdx.x, J.col_wgs), mode);                                 \
+    extern "C" __global__ void __launch_bounds__(64 * lucas_line_col_waves(K, NC, S))                           \
+            LUCAS_WAVES_PER_SIMD(lucas_line_col_waves_per_simd(EA, K, NC, S))                                     \
+            lucas_lcol1_##EA##_##K##_##NC##_##S(const LucasJobDesc J, int mode) {                                \
+        line_col_body<EA, K, NC, S>(J, line_col_tile(blockIdx.x, J.col_wgs), mode);                              \
     }
 LUCAS_LINE_COLS(LUCAS_LINE_COL_KERNELS)
 
@@ -913,8 +971,8 @@ struct LineRow {
     LineRow1Kernel one;
     int threads, lds_words;
 };
-#define LUCAS_LINE_COL_ENTRY(EA, K, NC) \
-    {lucas_lcol_##EA##_##K##_##NC, lucas_lcol1_##EA##_##K##_##NC, 64 * lucas_line_col_waves(K, NC), \
+#define LUCAS_LINE_COL_ENTRY(EA, K, NC, S)                                                                      \
+    {lucas_lcol_##EA##_##K##_##NC##_##S, lucas_lcol1_##EA##_##K##_##NC##_##S, 64 * lucas_line_col_waves(K, NC, S), \
      lucas_line_col_words(EA, K, NC)},
 #define LUCAS_LINE_ROW_ENTRY(EB, S) \
     {lucas_lrow_##EB##_##S, lucas_lrow1_##EB##_##S, lucas_line_row_threads(S), lucas_line_row_words(EB)},

@@ -63,7 +63,8 @@ typedef struct {
     int line;          /* 1: line kernels (one wave per row or column, shapes compiled in);
                           C = own columns and threads = work-items of a column workgroup;
                           2: the same with non-temporal spectrum loads and stores;
-                          3, 4: as 1, 2 with every row held by two waves */
+                          +2: every row held by two waves; +4: every column
+                          held by two waves (values 1..8) */
     size_t device_bytes; /* device footprint of one job */
 } lucas_plan_info;
 
@@ -112,9 +113,9 @@ int lucas_plan_shape(int p, int N, int A, int C, lucas_plan_info* out);
 /* Every admissible (A, C) tiling of (p, N), cheapest modelled first; fills up
  * to `cap` entries and writes the total into *count. */
 int lucas_plan_list(int p, int N, lucas_plan_info* out, int cap, int* count);
-/* The line-kernel plans of (p, N) (line = 1, or 3 for split rows; none where
- * no instantiated shape fits N and the carry depth): *count = how many, the
- * first `cap` in out. */
+/* The line-kernel plans of (p, N) (line = 1, +2 for split rows, +4 for
+ * split columns; none where no instantiated shape fits N and the carry
+ * depth): *count = how many, the first `cap` in out. */
 int lucas_line_plan_list(int p, int N, lucas_plan_info* out, int cap, int* count);
 /* Per-digit widths and weights as the reference computes them
  * (computeBitsPerWord gpuLucas.cu:688-694, computeWeightVectors 673-686):

@@ -41,7 +41,7 @@ def lib():
         _lib.emu_run.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_longlong, C.c_void_p,
                                  C.POINTER(C.c_double)]
         _lib.emu_product_bits.argtypes = [C.c_void_p]
-        _lib.emu_set_line.argtypes = [C.c_int, C.c_int, C.c_int]
+        _lib.emu_set_line.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
     return _lib
 
 
@@ -49,11 +49,12 @@ def run(p, N, digits, iters, A=0, C_=0, T=0, single=False, waves=(-1, -1), line=
     """Digits after `iters` iterations of the emulated kernels; `single` uses
     the single-job kernels (descriptor by value) instead of the batched ones;
     `waves` forces the column / row wave factors (-1: planner's choice);
-    `line` = (A, K) or (A, K, split) runs the line kernels with A rows, K own
-    columns per column workgroup (0: first instantiated choice) and rows held
-    by one wave (split 0) or two (split 1; -1 or absent: the first choice)."""
+    `line` = (A, K[, split[, col_split]]) runs the line kernels with A rows,
+    K own columns per column workgroup (0: first instantiated choice) and
+    rows / columns held by one wave (0) or two (1; -1 or absent: the first
+    choice)."""
     lib().emu_set_single(1 if single else 0)
-    lib().emu_set_line(*((tuple(line) + (-1,))[:3] if line is not None else (-1, 0, -1)))
+    lib().emu_set_line(*((tuple(line) + (-1, -1))[:4] if line is not None else (-1, 0, -1, -1)))
     lib().emu_set_waves(*waves)
     x = np.ascontiguousarray(digits, dtype=np.int32)
     out = np.zeros(N, dtype=np.int32)

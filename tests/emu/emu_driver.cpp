@@ -61,8 +61,13 @@ static int g_wA = -1, g_wB = -1;
 extern "C" void emu_set_waves(int wA, int wB) { g_wA = wA; g_wB = wB; }
 
 // Line kernels (A rows, K own columns; A < 0: the tiled kernels).
-static int g_line_A = -1, g_line_K = 0, g_line_split = -1;
-extern "C" void emu_set_line(int A, int K, int split) { g_line_A = A; g_line_K = K; g_line_split = split; }
+static int g_line_A = -1, g_line_K = 0, g_line_split = -1, g_line_col_split = -1;
+extern "C" void emu_set_line(int A, int K, int split, int col_split) {
+    g_line_A = A;
+    g_line_K = K;
+    g_line_split = split;
+    g_line_col_split = col_split;
+}
 
 // Product-size diagnostic of the last emu_run: max bit length, max 64-bit
 // leading-zero count of the rounded products.
@@ -77,7 +82,7 @@ extern "C" int emu_run(int p, int N, int A, int C, int T, const int32_t* digits_
     using namespace lucas;
     Plan pl;
     if (g_line_A >= 0) {
-        if (!make_line_plan(p, N, g_line_A, g_line_K, &pl, 0, g_line_split).empty()) return -1;
+        if (!make_line_plan(p, N, g_line_A, g_line_K, &pl, 0, g_line_split, g_line_col_split).empty()) return -1;
     } else if (!(T || g_wA >= 0 || g_wB >= 0 ? make_plan_exact(p, N, A, C, T, 0, &pl, g_wA, g_wB)
                                               : make_plan(p, N, A, C, &pl)).empty() &&
                !make_plan_exact(p, N, A, C, T, 0, &pl, g_wA, g_wB, true).empty()) {

@@ -441,6 +441,9 @@ def test_prepare_rehearses_and_restores_the_state(gmp_fixtures):
     (18984731, 1048576, 512, 4, 100, 4),
     (9068767, 524288, 512, 2, 100, 3),     # rows of 512 split: two 256-word halves
     (9068767, 524288, 128, 2, 100, 3),     # rows of 2048 split, E = 2 columns
+    (37156667, 2097152, 1024, 4, 100, 8),  # rows and columns split over two waves, non-temporal spectra
+    (37156667, 2097152, 1024, 2, 100, 5),  # columns split
+    (18984731, 1048576, 1024, 4, 100, 6),  # columns split, one wave per row of 512
 ])
 def test_line_kernels_equal_tiled_kernels(p, N, A, K, iters, kind):
     """The line kernels and the tiled kernels reach the same digits after
@@ -462,11 +465,11 @@ def test_line_kernels_equal_tiled_kernels(p, N, A, K, iters, kind):
 
 @pytest.mark.parametrize("p,N,A,K,kind", [(141731, 8192, 64, 2, 1), (2267549, 131072, 1024, 4, 1),
                                           (599657, 32768, 256, 1, 1), (2398633, 131072, 64, 2, 3),
-                                          (4535099, 262144, 64, 2, 4)])
+                                          (4535099, 262144, 64, 2, 4), (2267549, 131072, 1024, 4, 5)])
 def test_line_kernels_vs_oracle_steady_state(p, N, A, K, kind):
     """Line kernels from random digits: one and three iterations against the
     oracle's restatement (carry depth 3, E = 16 columns, one own column,
-    rows of 1024 and 2048 split over two waves)."""
+    rows of 1024 and 2048 and columns of 1024 split over two waves)."""
     print(f"case p={p} N={N} line{kind} A={A} K={K}", flush=True)
     x = random_balanced(p, N, 17)
     with lucas.Context(p, N, plan={"p": p, "N": N, "A": A, "C": K, "line": kind}) as ctx:
