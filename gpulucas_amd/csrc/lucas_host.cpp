@@ -238,7 +238,7 @@ std::vector<Plan> line_candidates(int p, int N) {
                 c.line_row = ir;
                 c.line_split = S - 1;
                 c.line_col_split = CS - 1;
-                c.lds_col = 16 * (size_t)lucas_line_col_words(EA, K, pl.ncarry);
+                c.lds_col = 16 * (size_t)lucas_line_col_words(EA, K, pl.ncarry, CS);
                 c.lds_row = 16 * (size_t)lucas_line_row_words(EB);
                 if (c.lds_col > kMaxLds || c.lds_row > kMaxLds) continue;
                 out.push_back(c);

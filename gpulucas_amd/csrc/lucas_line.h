@@ -38,14 +38,17 @@ inline constexpr int lucas_line_halo(int nc) { return (nc + 2) / 2; }
 inline constexpr int lucas_line_col_waves(int K, int NC, int S) {
     return (S * (K + lucas_line_halo(NC)) + 3) / 4 * 4;
 }
+// The column transform's twiddles: an LDS copy for lines of more than 8
+// words per lane, else registers.
+inline constexpr bool lucas_line_col_twl(int EA, int S) { return EA / S > 8; }
 // LDS of a column workgroup, in 16-byte words: the NL = K + H column regions
 // (A + 1 words each, the pad keeping neighbouring columns in different bank
-// groups); the column transform's twiddles omega_A^(l*k1) [k1][l] and
-// omega_64^t; per row {2^(rhi/N), 2^(-rhi/N)/(4M)} and rhi; per local digit
-// {2^(rlo/N), 2^(-rlo/N)} and rlo.
-inline constexpr int lucas_line_col_words(int EA, int K, int NC) {
+// groups); the column transform's twiddles omega_L^(l*k1) [k1][l] and
+// omega_64^t (LDS twiddles only); per row {2^(rhi/N), 2^(-rhi/N)/(4M)} and
+// rhi; per local digit {2^(rlo/N), 2^(-rlo/N)} and rlo.
+inline constexpr int lucas_line_col_words(int EA, int K, int NC, int S) {
     const int A = 64 * EA, NL = K + lucas_line_halo(NC);
-    return NL * (A + 1) + A + 64 + A + (A + 3) / 4 + 2 * NL + (2 * NL + 3) / 4;
+    return NL * (A + 1) + (lucas_line_col_twl(EA, S) ? A + 64 : 0) + A + (A + 3) / 4 + 2 * NL + (2 * NL + 3) / 4;
 }
 // LDS of a row workgroup (one row pair, 2S waves): one region per wave (the
 // two rows) and the outer twiddle factors rt2 of both rows (the transform's
@@ -58,7 +61,7 @@ inline constexpr int lucas_line_row_threads(int S) { return 128 * S; }
 // workgroups per CU where the LDS holds two, so that one workgroup's loads
 // and stores overlap the other's transforms; else one.
 inline constexpr int lucas_line_col_waves_per_simd(int EA, int K, int NC, int S) {
-    return (2 * 16 * lucas_line_col_words(EA, K, NC) <= 160 * 1024 ? 2 : 1) * lucas_line_col_waves(K, NC, S) / 4;
+    return (2 * 16 * lucas_line_col_words(EA, K, NC, S) <= 160 * 1024 ? 2 : 1) * lucas_line_col_waves(K, NC, S) / 4;
 }
 
 // Index of an instantiation in the lists above, or -1.

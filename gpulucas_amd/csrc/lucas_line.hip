@@ -598,12 +598,14 @@ __device__ __forceinline__ void line_col_body(const LucasJobDesc& J, int tile, i
     constexpr int A = 64 * EA, LH = 64 * EL, H = lucas_line_halo(NC), NL = K + H, RS = A + 1;
     constexpr int NW = lucas_line_col_waves(K, NC, S), T = 64 * NW;  // waves S*NL.. are helpers
     constexpr int NA = F::NA, NB = F::NB;
+    // the line twiddles: in registers for short lines, else an LDS copy
+    constexpr bool REG = !lucas_line_col_twl(EA, S);
     double2* const lds = lucas_dynamic_lds();
     double2* const T1 = lds + NL * RS;
     double2* const T2 = T1 + A;
-    double2* const roww = T2 + 64;                          // {whi, iwhi} per row
-    int* const rowr = reinterpret_cast<int*>(roww + A);    // rhi per row
-    double2* const locw = roww + A + (A + 3) / 4;          // {wlo, iwlo} per local digit
+    double2* const roww = REG ? T1 : T2 + 64;                 // {whi, iwhi} per row
+    int* const rowr = reinterpret_cast<int*>(roww + A);      // rhi per row
+    double2* const locw = roww + A + (A + 3) / 4;            // {wlo, iwlo} per local digit
     int* const locr = reinterpret_cast<int*>(locw + 2 * NL);  // rlo per local digit
     const int B = J.B, N = J.N;
     const int c0 = tile * K;
@@ -629,8 +631,6 @@ __device__ __forceinline__ void line_col_body(const LucasJobDesc& J, int tile, i
     const bool stamp = mode == 0;
     LINE_STAMP(lucas_line_stamps_col, stamp, 0);
     // Tables (T1, T2; per row; per local digit), loads first, then the tile.
-    // the line twiddles: in registers for short columns, else an LDS copy
-    constexpr bool REG = EL <= 8;
     constexpr int NTAB = 2 * A + 64 + 2 * NL, NTJ = (NTAB + T - 1) / T;
     double2 tv[NTJ];
     int ti[NTJ];
@@ -973,7 +973,7 @@ struct LineRow {
 };
 #define LUCAS_LINE_COL_ENTRY(EA, K, NC, S)                                                                      \
     {lucas_lcol_##EA##_##K##_##NC##_##S, lucas_lcol1_##EA##_##K##_##NC##_##S, 64 * lucas_line_col_waves(K, NC, S), \
-     lucas_line_col_words(EA, K, NC)},
+     lucas_line_col_words(EA, K, NC, S)},
 #define LUCAS_LINE_ROW_ENTRY(EB, S) \
     {lucas_lrow_##EB##_##S, lucas_lrow1_##EB##_##S, lucas_line_row_threads(S), lucas_line_row_words(EB)},
 const LineCol kLineCols[] = {LUCAS_LINE_COLS(LUCAS_LINE_COL_ENTRY)};
