@@ -1,8 +1,12 @@
 // CPU emulation driver: runs the engine's column/row kernels (compiled from
-// gpulucas_amd/csrc/lucas_kernels.hip against tests/emu/hip/hip_runtime.h)
-// on host memory for one job.  Tests only.
+// gpulucas_amd/csrc/lucas_kernels.hip and lucas_line.hip against
+// tests/emu/hip/hip_runtime.h) on host memory for one job.  Tests only.
+#include <sys/mman.h>
+
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "lucas_desc.h"
@@ -10,38 +14,204 @@
 
 thread_local emu_idx threadIdx, blockIdx;
 emu_idx blockDim;
-double2 lucas_lds[163840 / sizeof(double2)];
+// the running workgroup's LDS (`extern __shared__` is thread_local here)
+thread_local double2 lucas_lds[163840 / sizeof(double2)];
+
+// x86-64 System V context switch: pushes the callee-saved registers and the
+// FP control words of the running context, stores its stack pointer in
+// *from, and resumes the context whose stack pointer is `to`.
+extern "C" void emu_ctx_swap(void** from, void* to);
+// first frame of a fiber: the fiber's Fiber* arrives in rbx
+extern "C" void emu_fiber_entry();
+asm(R"(
+    .pushsection .text
+    .globl emu_ctx_swap
+    .type emu_ctx_swap, @function
+emu_ctx_swap:
+    pushq %rbp
+    pushq %rbx
+    pushq %r12
+    pushq %r13
+    pushq %r14
+    pushq %r15
+    subq $8, %rsp
+    stmxcsr (%rsp)
+    fnstcw 4(%rsp)
+    movq %rsp, (%rdi)
+    movq %rsi, %rsp
+    ldmxcsr (%rsp)
+    fldcw 4(%rsp)
+    addq $8, %rsp
+    popq %r15
+    popq %r14
+    popq %r13
+    popq %r12
+    popq %rbx
+    popq %rbp
+    ret
+    .size emu_ctx_swap, .-emu_ctx_swap
+
+    .globl emu_fiber_entry
+    .type emu_fiber_entry, @function
+emu_fiber_entry:
+    movq %rbx, %rdi
+    andq $-16, %rsp
+    call emu_fiber_main@PLT
+    ud2
+    .size emu_fiber_entry, .-emu_fiber_entry
+    .popsection
+)");
 
 namespace emu {
-std::barrier<>* bar = nullptr;
-std::vector<float> shfl;
 std::mutex atom_mu;
-std::vector<std::unique_ptr<std::barrier<>>> wave_bar;
-std::vector<int> lane_buf;
 
-void run_grid(unsigned nblocks, unsigned nthreads, const std::function<void()>& body) {
-    std::barrier<> b(nthreads);
-    bar = &b;
-    shfl.assign(nthreads, 0.f);
-    lane_buf.assign(nthreads, 0);
-    wave_bar.clear();
-    for (unsigned w = 0; w < (nthreads + 63) / 64; ++w)
-        wave_bar.push_back(std::make_unique<std::barrier<>>(std::min(64u, nthreads - 64 * w)));
-    blockDim = {nthreads, 1, 1};
-    std::vector<std::thread> ts;
-    for (unsigned t = 0; t < nthreads; ++t)
-        ts.emplace_back([&, t] {
-            threadIdx = {t, 0, 0};
-            for (unsigned blk = 0; blk < nblocks; ++blk) {
+namespace {
+constexpr size_t kStack = 128 * 1024;  // per fiber (reserved, committed as touched)
+
+struct Group;
+struct Fiber {
+    Group* g;
+    unsigned t;
+};
+// One host thread's workgroup: its fibers, their saved stack pointers, the
+// scheduler's context and the barrier state.
+struct Group {
+    unsigned n = 0;
+    const std::function<void()>* body = nullptr;
+    std::vector<void*> sp;
+    std::vector<Fiber> fib;
+    std::vector<char> done;
+    void* sched = nullptr;
+    unsigned cur = 0;
+    unsigned bar_count = 0, bar_gen = 0;
+    std::vector<unsigned> w_count, w_gen;
+    std::vector<int> lanes;
+    std::vector<float> shfl;
+    char* stacks = nullptr;
+    size_t stacks_n = 0;  // fibers the stack block holds
+};
+thread_local Group* t_group = nullptr;
+
+void yield_to_scheduler() {
+    Group& G = *t_group;
+    emu_ctx_swap(&G.sp[G.cur], G.sched);
+}
+
+// a fresh context that starts emu_fiber_entry with rbx = &fib
+void* fresh_context(char* stack_top, Fiber* fib) {
+    uint64_t* sp = reinterpret_cast<uint64_t*>(reinterpret_cast<uintptr_t>(stack_top) & ~uintptr_t(15));
+    *--sp = 0;                                               // alignment slot
+    *--sp = reinterpret_cast<uint64_t>(&emu_fiber_entry);    // return address
+    *--sp = 0;                                               // rbp
+    *--sp = reinterpret_cast<uint64_t>(fib);                 // rbx
+    *--sp = 0;                                               // r12
+    *--sp = 0;                                               // r13
+    *--sp = 0;                                               // r14
+    *--sp = 0;                                               // r15
+    *--sp = (uint64_t)0x1F80 | ((uint64_t)0x037F << 32);     // mxcsr, x87 control word
+    return sp;
+}
+
+void run_blocks(Group& G, unsigned first, unsigned step, unsigned nblocks) {
+    t_group = &G;
+    for (unsigned blk = first; blk < nblocks; blk += step) {
+        for (unsigned t = 0; t < G.n; ++t) {
+            G.fib[t] = {&G, t};
+            G.sp[t] = fresh_context(G.stacks + (size_t)(t + 1) * kStack, &G.fib[t]);
+            G.done[t] = 0;
+        }
+        G.bar_count = 0;
+        std::fill(G.w_count.begin(), G.w_count.end(), 0u);
+        unsigned left = G.n;
+        while (left) {
+            for (unsigned t = 0; t < G.n; ++t) {
+                if (G.done[t]) continue;
+                G.cur = t;
+                threadIdx = {t, 0, 0};
                 blockIdx = {blk, 0, 0};
-                body();
-                bar->arrive_and_wait();  // the workgroup's LDS is reused by the next one
+                emu_ctx_swap(&G.sched, G.sp[t]);
+                if (G.done[t]) --left;
             }
-        });
+        }
+    }
+    t_group = nullptr;
+}
+
+std::vector<Group>& groups() {
+    static std::vector<Group> g;
+    return g;
+}
+}  // namespace
+
+void group_barrier() {
+    Group& G = *t_group;
+    const unsigned gen = G.bar_gen;
+    if (++G.bar_count == G.n) {
+        G.bar_count = 0;
+        ++G.bar_gen;
+        return;
+    }
+    while (G.bar_gen == gen) yield_to_scheduler();
+}
+
+void wave_barrier() {
+    Group& G = *t_group;
+    const unsigned w = G.cur >> 6, size = std::min(64u, G.n - 64 * w);
+    const unsigned gen = G.w_gen[w];
+    if (++G.w_count[w] == size) {
+        G.w_count[w] = 0;
+        ++G.w_gen[w];
+        return;
+    }
+    while (G.w_gen[w] == gen) yield_to_scheduler();
+}
+
+int* lane_buf() { return t_group->lanes.data(); }
+float* shfl_buf() { return t_group->shfl.data(); }
+
+// Runs `body` for every work-item of nblocks workgroups of nthreads: up to
+// 8 host threads, each taking every 8th workgroup, its work-items as fibers.
+void run_grid(unsigned nblocks, unsigned nthreads, const std::function<void()>& body) {
+    blockDim = {nthreads, 1, 1};
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const unsigned nw = std::max(1u, std::min({8u, hw, nblocks}));
+    auto& gs = groups();
+    if (gs.size() < nw) gs.resize(nw);
+    for (unsigned g = 0; g < nw; ++g) {
+        Group& G = gs[g];
+        G.n = nthreads;
+        G.body = &body;
+        G.sp.assign(nthreads, nullptr);
+        G.fib.assign(nthreads, Fiber{});
+        G.done.assign(nthreads, 0);
+        G.w_count.assign((nthreads + 63) / 64, 0u);
+        G.w_gen.assign((nthreads + 63) / 64, 0u);
+        G.lanes.assign(nthreads, 0);
+        G.shfl.assign(nthreads, 0.f);
+        if (G.stacks_n < nthreads) {
+            if (G.stacks) munmap(G.stacks, G.stacks_n * kStack);
+            void* m = mmap(nullptr, (size_t)nthreads * kStack, PROT_READ | PROT_WRITE,
+                           MAP_PRIVATE | MAP_ANONYMOUS | MAP_NORESERVE, -1, 0);
+            if (m == MAP_FAILED) std::abort();
+            G.stacks = static_cast<char*>(m);
+            G.stacks_n = nthreads;
+        }
+    }
+    std::vector<std::thread> ts;
+    for (unsigned g = 1; g < nw; ++g) ts.emplace_back([&, g] { run_blocks(gs[g], g, nw, nblocks); });
+    run_blocks(gs[0], 0, nw, nblocks);
     for (auto& th : ts) th.join();
-    bar = nullptr;
 }
 }  // namespace emu
+
+// body of a fiber: the kernel for its work-item, then back to the scheduler
+extern "C" void emu_fiber_main(emu::Fiber* f) {
+    (*f->g->body)();
+    f->g->done[f->t] = 1;
+    void* dead;
+    emu_ctx_swap(&dead, f->g->sched);
+    std::abort();
+}
 
 namespace lucas {
 hipError_t launch_col_kernel(int, int, int, const LucasJobDesc*, const int2*, int, size_t, int, hipStream_t);

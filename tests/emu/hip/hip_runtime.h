@@ -1,11 +1,13 @@
 // hip_runtime.h stand-in for the CPU emulation harness (tests only).
-// Runs the engine's kernel source on the host: one workgroup at a time, every
-// work-item a host thread, __syncthreads a real barrier, shuffles through a
-// shared buffer.  It exists so kernel indexing can be checked against the
-// oracle without a GPU; the product is never built against it.
+// Runs the engine's kernel source on the host.  Every work-item is a fiber
+// (a user-space context with its own stack); a workgroup's fibers run on one
+// host thread, switched at barriers, and several host threads each run their
+// own workgroups.  __syncthreads and wave barriers are real barriers over the
+// fibers, cross-lane reads go through a per-wave buffer.  It exists so kernel
+// indexing can be checked against the oracle without a GPU; the product is
+// never built against it.
 #pragma once
 #include <algorithm>
-#include <barrier>
 #include <cmath>
 #include <cstring>
 #include <functional>
@@ -19,7 +21,9 @@
 #define __host__
 #define __forceinline__ inline
 #define __launch_bounds__(...)
-#define __shared__
+// workgroup-shared memory: one copy per host thread, which runs one
+// workgroup at a time
+#define __shared__ thread_local
 #define LUCAS_GLOBAL
 #define LUCAS_WAVES_PER_SIMD(n)
 
@@ -48,26 +52,29 @@ inline hipError_t hipGetLastError() { return 0; }
 inline hipError_t hipFuncSetAttribute(const void*, int, int) { return 0; }
 
 namespace emu {
-extern std::barrier<>* bar;
-extern std::vector<float> shfl;
 extern std::mutex atom_mu;
-// cross-lane exchange within one 64-wide wave: one barrier per wave
-extern std::vector<std::unique_ptr<std::barrier<>>> wave_bar;
-extern std::vector<int> lane_buf;
 void run_grid(unsigned nblocks, unsigned nthreads, const std::function<void()>& body);
+// barrier over the calling fiber's workgroup, or over its wave
+void group_barrier();
+void wave_barrier();
+// per-workgroup exchange buffers of the calling host thread (one int / float
+// per work-item)
+int* lane_buf();
+float* shfl_buf();
 // value of `x` held by lane `src` of the calling work-item's wave
 inline int wave_read(int x, int src) {
     const unsigned w = threadIdx.x >> 6;
-    lane_buf[threadIdx.x] = x;
-    wave_bar[w]->arrive_and_wait();
-    const int r = lane_buf[(w << 6) + (src & 63)];
-    wave_bar[w]->arrive_and_wait();
+    int* const buf = lane_buf();
+    buf[threadIdx.x] = x;
+    wave_barrier();
+    const int r = buf[(w << 6) + (src & 63)];
+    wave_barrier();
     return r;
 }
 }
-inline void __syncthreads() { emu::bar->arrive_and_wait(); }
-// the lanes of a wave run as separate threads here: a wave barrier is real
-inline void __builtin_amdgcn_wave_barrier() { emu::wave_bar[threadIdx.x >> 6]->arrive_and_wait(); }
+inline void __syncthreads() { emu::group_barrier(); }
+// the lanes of a wave run as separate fibers here: a wave barrier is real
+inline void __builtin_amdgcn_wave_barrier() { emu::wave_barrier(); }
 inline unsigned __lane_id() { return threadIdx.x & 63; }
 // DPP: quad_perm (ctrl < 0x100) and row_ror:n (0x121..0x12F) are the forms used
 inline int __builtin_amdgcn_update_dpp(int /*old*/, int src, int ctrl, int, int, bool) {
@@ -96,9 +103,10 @@ inline double __hiloint2double(int hi, int lo) {
     return d;
 }
 inline float __shfl_xor(float v, int mask, int /*width*/) {
-    emu::shfl[threadIdx.x] = v;
+    float* const buf = emu::shfl_buf();
+    buf[threadIdx.x] = v;
     __syncthreads();
-    float r = emu::shfl[threadIdx.x ^ mask];
+    float r = buf[threadIdx.x ^ mask];
     __syncthreads();
     return r;
 }

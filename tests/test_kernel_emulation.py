@@ -137,7 +137,8 @@ def test_emulated_product_bits_match_exact_convolution(p, N, A, C, iters):
 # Line kernels: every row and column length instantiated (64E, E = 1, 2, 3,
 # 4, 5, 7, 8, 16), carry depths 2 and 3, K = 1, 2, 4 own columns, the halo
 # wrap of tile 0, one and several iterations, batched and single-job forms,
-# rows of 1024 and 2048 words and columns of 1024 words split over two waves.
+# rows of 1024 and 2048 words and columns of 1024 words split over two waves,
+# and the N = 2^21 plan itself.
 @pytest.mark.parametrize("p,N,A,K,iters,single,split", [
     (149921, 8192, 64, 2, 3, False, 0),     # A = B = 64 (E = 1): several iterations from random digits
     (141731, 8192, 64, 2, 1, True, 0),      # carry depth 3 (q = 17)
@@ -157,14 +158,15 @@ def test_emulated_product_bits_match_exact_convolution(p, N, A, C, iters):
     (4535099, 262144, 64, 2, 1, False, 1),  # rows of 2048 split: two 1024-word halves, carry depth 3
     (2267549, 131072, 1024, 4, 1, False, 2),  # columns of 1024 split: two 512-word halves, depth 3
     (2398633, 131072, 1024, 2, 2, True, 2),   # ... K = 2, depth 2, two iterations
+    (36280733, 2097152, 1024, 4, 1, True, 3),  # N = 2^21 as planned: rows and columns split, depth 3
 ])
 def test_emulated_line_kernels_match_oracle(p, N, A, K, iters, single, split):
     w, _, _, _ = oracle.tables(p, N)
     rng = np.random.default_rng(p % 1000)
     half = (1 << (w - 1)).astype(np.int64)
     x = (rng.integers(0, 2 * half) - half).astype(np.int32)
-    # split: 0 one wave per line, 1 rows split over two waves, 2 columns split
-    line = (A, K, 1 if split == 1 else 0, 1 if split == 2 else 0)
+    # split: bit 0 rows split over two waves, bit 1 columns split
+    line = (A, K, split & 1, split >> 1)
     got, err = emu.run(p, N, x, iters, single=single, line=line)
     want, _ = oracle.ibdwt_iterate(p, N, x, iters)
     assert np.array_equal(got, want)
