@@ -9,7 +9,9 @@
 //                   (ComplexPointwiseSqr, gpuLucas.cu:647-665), inverse row
 //                   transform, inverse outer twiddle.
 //   column kernel : K own columns and H halo columns per workgroup, one column
-//                   per wave: inverse column transform, unweight, -2, round
+//                   per wave (1024-word columns: two waves per column, joined
+//                   by a radix-2 step): inverse column transform, unweight,
+//                   -2, round
 //                   (invDWTproductMinus2, gpuLucas.cu:774-805), carry-save
 //                   restoration (llintToIrrBal<n>, IrrBaseBalanced.cu:52-110)
 //                   and ripple + weight (loadIntToDoubleIBDWT,
@@ -22,7 +24,8 @@
 // exchanged through the wave's own LDS region: no workgroup barrier, and
 // every shape constant (lengths, strides, divisions) is known at compile
 // time.  Workgroup barriers remain only where lines meet: loading and
-// storing tiles, the square's row pairs and the carry across columns.
+// storing tiles, the square's row pairs, the carry across columns and the
+// radix-2 steps of split lines.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
