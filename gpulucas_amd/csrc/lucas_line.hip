@@ -29,6 +29,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "lucas_desc.h"
 #include "lucas_device.h"
 #include "lucas_line.h"
@@ -804,70 +806,91 @@ __device__ __forceinline__ void line_col_body(const LucasJobDesc& J, int tile, i
         // current digit, each shifted down by the widths of the digits passed
         // (w_t <= 21, so every field is in the low 32 bits).
         const unsigned int mq = (1u << J.q) - 1u, mq1 = (2u << J.q) - 1u;
-        for (int r = tid; r < A; r += T) {
-            unsigned int hw[2 * NL], lw[2 * NL];
+        // RR rows at once (r0, r0 + T, ...), their chains interleaved
+        auto carry_rows = [&](auto rr_c, int r0) {
+            constexpr int RR = decltype(rr_c)::value;
+            unsigned int hw[RR][2 * NL], lw[RR][2 * NL];
 #pragma unroll
-            for (int lc = 0; lc < NL; ++lc) {
-                const int rr = (c0 - H + lc < 0) ? (r == 0 ? A - 1 : r - 1) : r;
-                const double2 v = lds[lc * RS + rr];
-                hw[2 * lc] = (unsigned int)__double2hiint(v.x);
-                lw[2 * lc] = (unsigned int)__double2loint(v.x);
-                hw[2 * lc + 1] = (unsigned int)__double2hiint(v.y);
-                lw[2 * lc + 1] = (unsigned int)__double2loint(v.y);
+            for (int i = 0; i < RR; ++i) {
+                const int r = r0 + i * T;
+#pragma unroll
+                for (int lc = 0; lc < NL; ++lc) {
+                    const int rr = (c0 - H + lc < 0) ? (r == 0 ? A - 1 : r - 1) : r;
+                    const double2 v = lds[lc * RS + rr];
+                    hw[i][2 * lc] = (unsigned int)__double2hiint(v.x);
+                    lw[i][2 * lc] = (unsigned int)__double2loint(v.x);
+                    hw[i][2 * lc + 1] = (unsigned int)__double2hiint(v.y);
+                    lw[i][2 * lc + 1] = (unsigned int)__double2loint(v.y);
+                }
             }
             // window slot k holds product t-k: (wh[k]:wl[k]) its remaining
             // magnitude, ws[k] its sign (+1 / -1); wh is 0 once a product has
             // been shifted by one width (its magnitude is below 2^36)
-            unsigned int wh[NC + 1], wl[NC + 1];
-            int ws[NC + 1];
+            unsigned int wh[RR][NC + 1], wl[RR][NC + 1];
+            int ws[RR][NC + 1];
+            int hprev[RR], h0[RR], dlo[RR];
 #pragma unroll
-            for (int k = 0; k <= NC; ++k) wh[k] = wl[k] = 0u, ws[k] = 1;
-            int hprev = 0, h0 = 0;
-            int dlo = 0;
+            for (int i = 0; i < RR; ++i) {
+                hprev[i] = h0[i] = dlo[i] = 0;
+#pragma unroll
+                for (int k = 0; k <= NC; ++k) wh[i][k] = wl[i][k] = 0u, ws[i][k] = 1;
+            }
 #pragma unroll
             for (int t = 2 * H - 1 - NC; t < 2 * NL; ++t) {
-                if (t > 2 * H - 1 - NC) {
-                    // pass digit t-1: every product in the window moves down
-                    // by its width; the oldest leaves
-                    const int wp = J.q + (int)((hw[t - 1] >> 30) & 1u);
 #pragma unroll
-                    for (int k = NC; k >= 1; --k) {
-                        wl[k] = __builtin_amdgcn_alignbit(wh[k - 1], wl[k - 1], wp);
-                        wh[k] = wh[k - 1] >> wp;
-                        ws[k] = ws[k - 1];
+                for (int i = 0; i < RR; ++i) {
+                    if (t > 2 * H - 1 - NC) {
+                        // pass digit t-1: every product in the window moves
+                        // down by its width; the oldest leaves
+                        const int wp = J.q + (int)((hw[i][t - 1] >> 30) & 1u);
+#pragma unroll
+                        for (int k = NC; k >= 1; --k) {
+                            wl[i][k] = __builtin_amdgcn_alignbit(wh[i][k - 1], wl[i][k - 1], wp);
+                            wh[i][k] = wh[i][k - 1] >> wp;
+                            ws[i][k] = ws[i][k - 1];
+                        }
                     }
+                    wh[i][0] = hw[i][t] & 0xFFFFFu;
+                    wl[i][0] = lw[i][t];
+                    ws[i][0] = ((int)hw[i][t] >> 31) | 1;
                 }
-                wh[0] = hw[t] & 0xFFFFFu;
-                wl[0] = lw[t];
-                ws[0] = ((int)hw[t] >> 31) | 1;
                 if (t < 2 * H - 1) continue;  // the window fills below the first digit restored
-                const bool wide = (hw[t] >> 30) & 1u;
-                const unsigned int M = wide ? mq1 : mq;
-                const int W = J.q + (int)wide;
-                int sum = 0;
 #pragma unroll
-                for (int k = 0; k <= NC; ++k) sum += __mul24(ws[k], (int)(wl[k] & M));
-                // the reference's carry: (sum + half) >> W for sum >= half,
-                // -((half - sum) >> W) for sum < -half, else 0
-                const int half = (int)((M + 1u) >> 1);
-                const int h = (sum + half + (sum >> 31) + (sum == -half ? 1 : 0)) >> W;
-                const int d = sum - (h << W);
-                if (t == 2 * H - 1) {
-                    hprev = h;  // carry into the first own element
-                } else if ((t & 1) == 0) {
-                    dlo = d;
-                    h0 = h;
-                } else {
-                    const int lc = t >> 1, lo = lc - H;
-                    const int2 xv = make_int2(dlo + hprev, d + h0);
-                    hprev = h;
-                    if (out_digits) {
-                        gstore2i(J.digits, (size_t)(c0 + lo) + (size_t)B * r, xv);
+                for (int i = 0; i < RR; ++i) {
+                    const bool wide = (hw[i][t] >> 30) & 1u;
+                    const unsigned int M = wide ? mq1 : mq;
+                    const int W = J.q + (int)wide;
+                    int sum = 0;
+#pragma unroll
+                    for (int k = 0; k <= NC; ++k) sum += __mul24(ws[i][k], (int)(wl[i][k] & M));
+                    // the reference's carry: (sum + half) >> W for sum >= half,
+                    // -((half - sum) >> W) for sum < -half, else 0
+                    const int half = (int)((M + 1u) >> 1);
+                    const int h = (sum + half + (sum >> 31) + (sum == -half ? 1 : 0)) >> W;
+                    const int d = sum - (h << W);
+                    if (t == 2 * H - 1) {
+                        hprev[i] = h;  // carry into the first own element
+                    } else if ((t & 1) == 0) {
+                        dlo[i] = d;
+                        h0[i] = h;
                     } else {
-                        lds[lc * RS + r] = weighted(lc, r, xv);
+                        const int lc = t >> 1, lo = lc - H, r = r0 + i * T;
+                        const int2 xv = make_int2(dlo[i] + hprev[i], d + h0[i]);
+                        hprev[i] = h;
+                        if (out_digits) {
+                            gstore2i(J.digits, (size_t)(c0 + lo) + (size_t)B * r, xv);
+                        } else {
+                            lds[lc * RS + r] = weighted(lc, r, xv);
+                        }
                     }
                 }
             }
+        };
+        // two rows per work-item where a second one exists (a wave-uniform
+        // choice: A - T is a multiple of 64), so no pass runs a few waves alone
+        for (int r = tid; r < A; r += 2 * T) {
+            if (r + T < A) carry_rows(std::integral_constant<int, 2>{}, r);
+            else carry_rows(std::integral_constant<int, 1>{}, r);
         }
         __syncthreads();
         LINE_STAMP(lucas_line_stamps_col, stamp, 3);
