@@ -698,9 +698,19 @@ int lucas_autotune_shape(int p, int N, int iters, int max_candidates, const luca
                          lucas_plan_info* out, double* best_ms) {
     if (!out || p < 7) return LUCAS_ERR_INVALID_ARG;
     if (N == 0) N = default_length(p);
-    // the line-kernel shapes first (few, and usually the fastest), then the
-    // tilings of the general kernels in modelled order
-    std::vector<Plan> cands = line_candidates(p, N);
+    // the line-kernel shapes first (few, and usually the fastest), each also
+    // with non-temporal spectrum loads and stores where the state outgrows
+    // the L2s (N >= 2^19), then the tilings of the general kernels in
+    // modelled order
+    std::vector<Plan> cands;
+    for (const Plan& c : line_candidates(p, N)) {
+        cands.push_back(c);
+        if (N >= 524288) {
+            Plan nt = c;
+            nt.nt = 1;
+            cands.push_back(nt);
+        }
+    }
     std::vector<Plan> tiled = plan_candidates(p, N);
     if (tiled.empty()) tiled = plan_candidates(p, N, true);
     cands.insert(cands.end(), tiled.begin(), tiled.end());

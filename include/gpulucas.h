@@ -251,10 +251,12 @@ int lucas_autotune_N(int p, int iters, const lucas_options* opt, lucas_plan_info
  * fills up to `cap` entries, ascending, and writes the total into *count. */
 int lucas_autotune_candidates(int p, int* out, int cap, int* count);
 
-/* Measured choice of tiling (radix split A x B, tile width C, workgroup
- * size) for (p, N): times `iters` iterations of up to `max_candidates`
- * tilings and keeps the fastest (replaces the hand-set signal-size and
- * slice constants, gpuLucas.cu:159-173, 323-343).  *best_ms = ms/iteration. */
+/* Measured choice of shape for (p, N): times `iters` iterations of up to
+ * `max_candidates` plans -- the line-kernel shapes first (from N = 2^19 each
+ * also with non-temporal spectra), then the tilings (radix split A x B, tile
+ * width C, workgroup size) -- and keeps the fastest (replaces the hand-set
+ * signal-size and slice constants, gpuLucas.cu:159-173, 323-343).
+ * *best_ms = ms/iteration. */
 int lucas_autotune_shape(int p, int N, int iters, int max_candidates, const lucas_options* opt,
                          lucas_plan_info* out, double* best_ms);
 

@@ -463,6 +463,24 @@ def test_line_kernels_equal_tiled_kernels(p, N, A, K, iters, kind):
         assert 0.0 < ctx.max_roundoff() < ROUNDOFF_LIMIT
 
 
+@pytest.mark.gpu
+def test_autotune_shape_picks_a_plan_with_the_same_digits():
+    """lucas_autotune_shape at N = 2^19 (line shapes with and without
+    non-temporal spectra first, then tilings): the plan it keeps reaches the
+    default plan's digits, and its time is a positive ms/iteration."""
+    p, N = 9068767, 524288
+    print(f"case autotune_shape p={p} N={N}", flush=True)
+    info, ms = lucas.autotune_shape(p, N, iters=40, max_candidates=8)
+    assert info["N"] == N and 0.0 < ms < 1.0
+    with lucas.Context(p, N) as ctx:
+        ctx.step(50)
+        want = ctx.digits()
+    with lucas.Context(p, N, plan=info) as ctx:
+        assert (ctx.info["A"], ctx.info["C"], ctx.info["line"]) == (info["A"], info["C"], info["line"])
+        ctx.step(50)
+        assert np.array_equal(ctx.digits(), want)
+
+
 @pytest.mark.parametrize("p,N,A,K,kind", [(141731, 8192, 64, 2, 1), (2267549, 131072, 1024, 4, 1),
                                           (599657, 32768, 256, 1, 1), (2398633, 131072, 64, 2, 3),
                                           (4535099, 262144, 64, 2, 4), (2267549, 131072, 1024, 4, 5)])
