@@ -249,7 +249,14 @@ void Engine::set_jobs(const std::vector<Job*>& jobs) {
 int Engine::prepare() {
     if (!dirty_) return LUCAS_OK;
     hipSetDevice(device_);
-    destroy_graphs();
+    // the slot graphs become stale (updated in place when next used); the
+    // whole-step graphs of single jobs are rebuilt
+    for (int l = 0; l < kGraphSlots; ++l) stale_[l] = gexec_[l] != nullptr;
+    for (Whole& w : whole_) {
+        if (w.exec) hipGraphExecDestroy(w.exec);
+        if (w.graph) hipGraphDestroy(w.graph);
+    }
+    whole_.clear();
     std::vector<LucasJobDesc> desc;
     std::map<int, std::vector<int2>> cmap, rmap;
     static const int kThreads[4] = {LUCAS_THREADS_SMALL, LUCAS_THREADS_LARGE, LUCAS_THREADS_MAX, LUCAS_THREADS_MAX};
@@ -376,6 +383,7 @@ void Engine::destroy_graphs() {
         if (graph_[l]) hipGraphDestroy(graph_[l]);
         gexec_[l] = nullptr;
         graph_[l] = nullptr;
+        stale_[l] = false;
     }
     for (Whole& w : whole_) {
         if (w.exec) hipGraphExecDestroy(w.exec);
@@ -429,7 +437,7 @@ int Engine::whole_graph(long long k, hipGraphExec_t* out) {
 // Slot l < kLevels: 2^l steady-state [row ; col] pairs; kHead: the column
 // kernel reading digits; kTail: row + column kernel writing digits.
 int Engine::build_graph(int slot) {
-    if (gexec_[slot]) return LUCAS_OK;
+    if (gexec_[slot] && !stale_[slot]) return LUCAS_OK;
     hipError_t e = hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return LUCAS_ERR_DEVICE;
     int rc = LUCAS_OK;
@@ -444,13 +452,31 @@ int Engine::build_graph(int slot) {
             if (rc == LUCAS_OK) rc = launch_col(0);
         }
     }
-    e = hipStreamEndCapture(stream_, &graph_[slot]);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(stream_, &g);
     if (rc != LUCAS_OK || e != hipSuccess) {
-        if (graph_[slot]) hipGraphDestroy(graph_[slot]);
-        graph_[slot] = nullptr;
+        if (g) hipGraphDestroy(g);
         (void)hipGetLastError();
         return LUCAS_ERR_DEVICE;
     }
+    if (gexec_[slot]) {
+        // a stale executable graph: the same kernels with new grid sizes and
+        // arguments update in place; another kernel sequence is rebuilt
+        hipGraphNode_t bad = nullptr;
+        hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+        const hipError_t u = hipGraphExecUpdate(gexec_[slot], g, &bad, &res);
+        stale_[slot] = false;
+        if (u == hipSuccess && res == hipGraphExecUpdateSuccess) {
+            if (graph_[slot]) hipGraphDestroy(graph_[slot]);
+            graph_[slot] = g;
+            return LUCAS_OK;
+        }
+        (void)hipGetLastError();
+        hipGraphExecDestroy(gexec_[slot]);
+        gexec_[slot] = nullptr;
+    }
+    if (graph_[slot]) hipGraphDestroy(graph_[slot]);
+    graph_[slot] = g;
     e = hipGraphInstantiate(&gexec_[slot], graph_[slot], nullptr, nullptr, 0);
     if (e == hipSuccess) e = hipGraphUpload(gexec_[slot], stream_);
     if (e != hipSuccess) {

@@ -118,9 +118,12 @@ private:
     bool single_ = false;  // one job: launch the by-value-descriptor kernels with one_
     LucasJobDesc one_;
     std::map<int, Group> colg_, rowg_;
-    // replay graphs by slot (see kHead, kTail), built on first use
+    // replay graphs by slot (see kHead, kTail), built on first use; after a
+    // change of jobs a slot's executable graph is kept as stale and updated
+    // in place from the new capture where the kernel sequence allows
     hipGraph_t graph_[kGraphSlots] = {};
     hipGraphExec_t gexec_[kGraphSlots] = {};
+    bool stale_[kGraphSlots] = {};
     // whole-step graphs of a single-job engine: one graph per short step
     // length k <= kWholeMax (head, k - 1 pairs, tail), the most recent
     // kWholeCache lengths kept
