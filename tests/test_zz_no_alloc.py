@@ -45,6 +45,19 @@ def test_noalloc_matches_context_path_twice_second_smaller():
         assert err == want_err and 0.0 < err < lucas.DEFAULT_ERROR_LIMIT
 
 
+def test_noalloc_split_row_plan_matches_context_path():
+    """N = 393216 plans the line kernels with rows split over two waves (its
+    tuned kind 3): the scratch-array path reaches the context path's digits."""
+    p, N, iters = 7149713, 393216, 20
+    assert lucas.plan(p, N)["line"] == 3
+    x = random_balanced(p, N, 43)
+    want, want_err = _reference(p, N, x, iters)
+    got, err = _noalloc(p, N, x, iters)
+    print(f"p={p} N={N} iters={iters} roundoff {err:.4f}", flush=True)
+    assert np.array_equal(got, want)
+    assert err == want_err
+
+
 def test_noalloc_from_s0_matches_oracle():
     p, N = 9941, 560
     x = oracle.s0_digits(N)
