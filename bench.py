@@ -272,8 +272,9 @@ def main():
                     help="only the headline iterations (for rocprofv3 kernel-trace / counter passes)")
     ap.add_argument("--p", type=int, default=P_HEAD, help="exponent for --profile-run")
     ap.add_argument("--N", type=int, default=N_HEAD, help="signal length for --profile-run (0: default)")
-    ap.add_argument("--line", default="", metavar="A:K",
-                    help="--profile-run on the line kernels with A rows and K own columns per column workgroup")
+    ap.add_argument("--line", default="", metavar="A:K[:KIND]",
+                    help="--profile-run on the line kernels with A rows and K own columns per column workgroup "
+                         "(KIND: lucas_plan_info.line, default 1)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only a device-to-device copy of BYTES (HBM counter calibration)")
     a = ap.parse_args()
@@ -299,8 +300,8 @@ def main():
     if a.profile_run:
         plan = None
         if a.line:
-            la, lk = (int(v) for v in a.line.split(":"))
-            plan = {"p": a.p, "N": a.N, "A": la, "C": lk, "line": 1}
+            f = [int(v) for v in a.line.split(":")]
+            plan = {"p": a.p, "N": a.N, "A": f[0], "C": f[1], "line": f[2] if len(f) > 2 else 1}
         with lucas.Context(a.p, a.N, plan=plan) as ctx:
             ctx.step(a.warmup + a.steps)
             print(json.dumps({"profile_run": ctx.info, "max_roundoff": ctx.max_roundoff()}), flush=True)
