@@ -58,6 +58,7 @@ struct LucasJobDesc {
     int pA, pB;             // P of the column (length A) and row (length B) transforms
     int ohA, ohB;           // ceil(A/64), ceil(B/64): row lengths of ct2 and rt2
     int nt;                 // line kernels: spectra loaded and stored non-temporally
+    int ld;                 // line kernels: row stride of the spectra, in words (>= B)
     LucasStage stA[LUCAS_MAX_RADICES];  // column transform Stockham stages (length A1)
     LucasStage stB[LUCAS_MAX_RADICES];  // row transform Stockham stages (length B1)
     unsigned int m_ncl, m_C;             // reciprocals of C+halo and C

@@ -122,7 +122,7 @@ int job_create(const Plan& plan, int device, hipStream_t stream, Job** out, Devi
     make_tables(plan, &t);
     // spectra, digits, then the table block (layout mirrors job_bytes)
     const size_t o_spec = 0;
-    const size_t o_spec2 = align256(sizeof(double2) * plan.M);
+    const size_t o_spec2 = align256(spec_bytes(plan));
     const size_t o_dig = 2 * o_spec2;
     const size_t o_tab = o_dig + align256(sizeof(int) * plan.N);
     const TableLayout lay = table_layout(plan);

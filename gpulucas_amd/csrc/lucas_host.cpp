@@ -238,6 +238,7 @@ std::vector<Plan> line_candidates(int p, int N) {
                 c.line_row = ir;
                 c.line_split = S - 1;
                 c.line_col_split = CS - 1;
+                c.ld = A >= 1024 && B % 256 == 0 ? B + 8 : B;
                 c.lds_col = 16 * (size_t)lucas_line_col_words(EA, K, pl.ncarry, CS);
                 c.lds_row = 16 * (size_t)lucas_line_row_words(EB);
                 if (c.lds_col > kMaxLds || c.lds_row > kMaxLds) continue;
@@ -445,6 +446,7 @@ void fill_desc_shape(const Plan& plan, LucasJobDesc* d) {
     d->C = plan.C;
     d->col_wgs = plan.col_wgs;
     d->row_wgs = plan.row_wgs;
+    d->ld = plan.ld ? plan.ld : plan.B;
     d->threads = plan.threads;
     const int A1 = plan.wA ? plan.A / plan.wA : plan.A, B1 = plan.wB ? plan.B / plan.wB : plan.B;
     d->nradA = (int)plan.radA.size();
@@ -552,9 +554,13 @@ void desc_tables(const TableLayout& l, char* m, LucasJobDesc* d) {
     d->pbits = d->maxerr + 1;
 }
 
+size_t spec_bytes(const Plan& pl) {
+    return sizeof(double) * 2 * (size_t)pl.A * (size_t)(pl.ld ? pl.ld : pl.B);
+}
+
 size_t job_bytes(const Plan& pl) {
     size_t s = 0;
-    s += align256(sizeof(double) * 2 * (size_t)pl.M) * 2;  // spec, spec2
+    s += align256(spec_bytes(pl)) * 2;  // spec, spec2
     s += align256(sizeof(int) * (size_t)pl.N);             // digits
     return s + table_layout(pl).bytes;
 }

@@ -33,6 +33,13 @@ struct Plan {
     int nt = 0;  // line kernels: spectra moved with non-temporal loads and stores
     int line_split = 0;      // line kernels: each row held by two waves (a radix-2 step joins them)
     int line_col_split = 0;  // line kernels: each column held by two waves
+    // Row stride of the spectra in words (0: B).  Line plans of 1024 or more
+    // rows pad rows whose length is a multiple of 4 KB by 128 bytes: a
+    // column tile's reads, 64-96 bytes of each of 1024 rows, then rotate
+    // over the memory channels instead of all landing at one offset modulo
+    // 4 KB (MI355X, N = 2^21: 33.2 -> 32.4 us/iteration; at 512 and 256 rows
+    // the padding measured 0.6-0.8 % slower, so it is not used there).
+    int ld = 0;
     std::vector<int> radA, radB;  // Stockham radices of the remaining factors A/wA, B/wB
     size_t lds_col = 0, lds_row = 0;  // bytes of LDS per workgroup
 };
@@ -92,6 +99,8 @@ void stage_table(int L, const std::vector<int>& rad, LucasStage* out);
 void fill_desc_shape(const Plan& plan, LucasJobDesc* d);
 
 // Device footprint of one job under `plan` (state + tables), bytes.
+// Bytes of one spectrum (A rows of the plan's row stride).
+size_t spec_bytes(const Plan& plan);
 size_t job_bytes(const Plan& plan);
 
 // Outer twiddles omega_M^(c*r) (c < B, r < A) as the product of two table

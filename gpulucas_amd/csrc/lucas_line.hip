@@ -313,7 +313,7 @@ __device__ __forceinline__ void line_row_body(const LucasJobDesc& J, int wg) {
     twr.load(J.lnB, l);
     double2 x[F::NX];
 #pragma unroll
-    for (int e = 0; e < EB; ++e) x[e] = gload2_nt(J.spec, (size_t)R * B + l + 64 * e, J.nt);
+    for (int e = 0; e < EB; ++e) x[e] = gload2_nt(J.spec, (size_t)R * J.ld + l + 64 * e, J.nt);
     // Every twiddle the kernel needs is loaded here, so no later phase waits
     // on a global load: the outer twiddle omega_M^(R*c), c = l + 64e, is
     // rt1[R][l] * rt2[R][e] (the rt2 row goes to LDS for the store), and the
@@ -404,7 +404,7 @@ __device__ __forceinline__ void line_row_body(const LucasJobDesc& J, int wg) {
 #pragma unroll
     for (int e = 0; e < EB; ++e) {
         const double2 tw = T2REG ? t2r[e] : cmul(t1, T2R[w * EB + e]);
-        gstore2_nt(J.spec2, (size_t)R * B + l + 64 * e, cmulc(x[e], tw), J.nt);
+        gstore2_nt(J.spec2, (size_t)R * J.ld + l + 64 * e, cmulc(x[e], tw), J.nt);
     }
     LINE_STAMP(lucas_line_stamps_row, true, 5);
 }
@@ -463,7 +463,7 @@ __device__ __forceinline__ void line_row_split_body(const LucasJobDesc& J, int w
     TwReg<EH> twr;
     twr.load(J.lnB, l);
     double2 x[F::NX];
-    const size_t base = (size_t)R * B + (size_t)h * LH;
+    const size_t base = (size_t)R * J.ld + (size_t)h * LH;
 #pragma unroll
     for (int e = 0; e < EH; ++e) x[e] = gload2_nt(J.spec, base + l + 64 * e, J.nt);
     // outer twiddle omega_M^(R*c), c = h*B/2 + l + 64e: rt1[R][l] * rt2[R][h*EH + e]
@@ -668,7 +668,7 @@ __device__ __forceinline__ void line_col_body(const LucasJobDesc& J, int tile, i
             const int t = tid + T * i;
             if (t < A * NL) {
                 const int r = t / NL, lc = t - r * NL;
-                tile_v[i] = gload2_nt(J.spec2, (size_t)r * B + gcol(lc), J.nt);
+                tile_v[i] = gload2_nt(J.spec2, (size_t)r * J.ld + gcol(lc), J.nt);
             }
         }
     }
@@ -935,7 +935,7 @@ __device__ __forceinline__ void line_col_body(const LucasJobDesc& J, int tile, i
     // store row-major: the outer twiddle is the row kernel's
     for (int t = tid; t < A * K; t += T) {
         const int k = t / K, lo = t - k * K;
-        gstore2_nt(J.spec, (size_t)k * B + c0 + lo, lds[(H + lo) * RS + spec_pos(k)], J.nt);
+        gstore2_nt(J.spec, (size_t)k * J.ld + c0 + lo, lds[(H + lo) * RS + spec_pos(k)], J.nt);
     }
     LINE_STAMP(lucas_line_stamps_col, stamp, 5);
     if (REG && colw && !in_digits) {

@@ -86,7 +86,7 @@ int load_tables(DeviceScratch* ds, int p, int N, hipStream_t stream) {
     Tables t;
     make_tables(pl, &t);
     const size_t o_spec = 0;
-    const size_t o_spec2 = align256(sizeof(double2) * pl.M);
+    const size_t o_spec2 = align256(spec_bytes(pl));
     const size_t o_tab = 2 * o_spec2;
     const TableLayout lay = table_layout(pl);
     if (o_tab + lay.bytes > (size_t)LUCAS_NOALLOC_SCRATCH_BYTES) return LUCAS_ERR_INVALID_ARG;

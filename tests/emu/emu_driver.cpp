@@ -260,7 +260,7 @@ extern "C" int emu_run(int p, int N, int A, int C, int T, const int32_t* digits_
     }
     Tables t;
     make_tables(pl, &t);
-    std::vector<double2> spec(pl.M), spec2(pl.M);
+    std::vector<double2> spec(spec_bytes(pl) / sizeof(double2)), spec2(spec_bytes(pl) / sizeof(double2));
     std::vector<int> dig(digits_in, digits_in + N);
     unsigned err = 0;
     LucasJobDesc d{};
